@@ -1,0 +1,174 @@
+/*
+ * mpdhg.h -- C ABI of the MI355X PDHG LP engine (libmpdhg.so).
+ *
+ * Drop-in boundary for the GPU PDHG path of refinery-optimizer:
+ *   sih::gpu::GpuSolver            core/include/sih/gpu/gpu_solver.hpp:30-45
+ *   sih::gpu::GpuPdhgConfig        core/include/sih/gpu/gpu_solver.hpp:12-22
+ *   sih::gpu::GpuMemoryInfo        core/include/sih/gpu/gpu_solver.hpp:24-28
+ *   sih::model::Solution fields    core/include/sih/model/solution.hpp:37-84
+ *   sih::model::SparseMatrix CSC   core/include/sih/model/sparse_matrix.hpp:38-40
+ * Plain C types only: int64 CSC indices and double values are taken exactly
+ * as model::SparseMatrix stores them, so a caller passes
+ * A().csc_col_ptr().data() etc. without conversion.
+ *
+ * Errors: every entry point returns 0 on success and a negative MPDHG_E_*
+ * code on failure; mpdhg_last_error() gives a thread-local message.  No C++
+ * exception crosses this boundary.
+ * Threading: a context owns one HIP stream and its device buffers; use one
+ * context per host thread.  Calls on different contexts may run concurrently.
+ */
+#ifndef MPDHG_H
+#define MPDHG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define MPDHG_OK 0
+#define MPDHG_E_ARG (-1)      /* invalid argument / malformed LP */
+#define MPDHG_E_HIP (-2)      /* HIP runtime error (message has the call) */
+#define MPDHG_E_NODEV (-3)    /* no HIP device visible */
+#define MPDHG_E_NOMEM (-4)    /* device or host allocation failed */
+#define MPDHG_E_STATE (-5)    /* call not valid in the context's state */
+
+/* ---- model::SolutionStatus values (solution.hpp:11-21) ---------------- */
+#define MPDHG_STATUS_OPTIMAL 0
+#define MPDHG_STATUS_INFEASIBLE 1
+#define MPDHG_STATUS_UNBOUNDED 2
+#define MPDHG_STATUS_TIME_LIMIT 3
+#define MPDHG_STATUS_ITERATION_LIMIT 4
+#define MPDHG_STATUS_NUMERICAL_FAILURE 6
+#define MPDHG_STATUS_UNKNOWN 8
+
+/* ---- model::BasisStatus values (solution.hpp:23-29) ------------------- */
+#define MPDHG_BASIS_BASIC 0
+#define MPDHG_BASIS_AT_LOWER 1
+#define MPDHG_BASIS_AT_UPPER 2
+
+/* ---- solver modes (extension) ----------------------------------------- */
+#define MPDHG_MODE_REFERENCE 0 /* the reference algorithm, FP32, bit-exact restatement */
+#define MPDHG_MODE_ENHANCED 1  /* scaled, restarted reflected-Halpern PDHG + FP64 checks */
+
+/* GpuPdhgConfig (gpu_solver.hpp:12-22): the first nine fields keep the
+ * reference names, meanings and defaults; the rest are extensions. */
+typedef struct mpdhg_config {
+    int32_t max_iterations;       /* 4000 */
+    float tol_primal;             /* 1e-4 */
+    float tol_dual;               /* 1e-4 */
+    float tol_gap;                /* 1e-4 (reference: unread; enhanced: gap part of KKT) */
+    int32_t check_frequency;      /* 25 */
+    float step_tau;               /* 0 = automatic */
+    float step_sigma;             /* 0 = automatic */
+    int32_t enable_restart;       /* 1 (reference mode: provably without effect) */
+    int32_t enable_hybrid_polish; /* 1 (reference: unread) */
+    /* extensions */
+    int32_t mode;                 /* MPDHG_MODE_REFERENCE */
+    double tol_kkt;               /* enhanced: FP64 relative-KKT target; <=0: max(tol_primal,tol_dual,tol_gap) */
+    int32_t max_refinements;      /* enhanced: outer FP64 iterative-refinement rounds (0 = off) */
+    float reflection;             /* enhanced: Halpern reflection coefficient rho in [0,1] (1.0) */
+    int32_t ruiz_iterations;      /* enhanced: Ruiz equilibration passes (10) */
+    int32_t profile;              /* 1: time the fused SpMV kernels with HIP events */
+    int32_t dual_sign;            /* 0: reference PDHG sign (reduced cost = c + A^T y);
+                                     1: simplex/checker sign (c - A^T y, checker.cpp:231) */
+} mpdhg_config;
+
+/* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite
+ * bounds are +-INFINITY (SIH_INFINITY).  The reference solve_pdhg ignores
+ * sense() and obj_offset() (gpu_solver.cpp:300-302); so does this engine. */
+typedef struct mpdhg_lp {
+    int64_t m, n, nnz;
+    const int64_t *col_ptr;  /* [n+1] */
+    const int64_t *row_ind;  /* [nnz], rows ascending within a column */
+    const double *vals;      /* [nnz] */
+    const double *c;         /* [n] */
+    const double *col_lower; /* [n] */
+    const double *col_upper; /* [n] */
+    const double *row_lower; /* [m] */
+    const double *row_upper; /* [m] */
+} mpdhg_lp;
+
+/* model::Solution subset the GPU path fills (solution.hpp:37-84) plus
+ * per-solve metrics.  Vector outputs are caller-allocated; NULL skips one. */
+typedef struct mpdhg_result {
+    double *x;             /* [n] primal (reference mode: best x, gpu_solver.cpp:261-264) */
+    double *y;             /* [m] row duals (sign per mpdhg_config.dual_sign) */
+    double *reduced_costs; /* [n] */
+    double *row_activity;  /* [m] A x ("slack" in model::Solution) */
+    int32_t status;        /* MPDHG_STATUS_* */
+    int32_t restarts;
+    int32_t refinements;
+    int32_t checks;
+    int64_t iterations;    /* reference: loop counter at exit (max_iterations+1 if not converged) */
+    double primal_objective; /* sum_j c_j x_j (FP64) */
+    double dual_objective;
+    double rel_primal, rel_dual, rel_gap, rel_kkt; /* FP64 relative KKT of (x, y) */
+    double ref_primal, ref_dual; /* reference-mode P and D at the last check */
+    double time_wall_sec;    /* whole call, host setup and copies included */
+    double solve_sec;        /* device-resident iteration loop only */
+    double kernel_ms_spmv_csr, kernel_ms_spmv_csc; /* profile=1: summed event times */
+    int64_t kernel_launches_csr, kernel_launches_csc;
+    double bytes_csr_launch, bytes_csc_launch;     /* algorithmic bytes per fused launch */
+} mpdhg_result;
+
+typedef struct mpdhg_ctx mpdhg_ctx;
+
+/* ---- device queries: GpuSolver::is_cuda_available / get_vram_info ----- */
+/* gpu_solver.cpp:42-46 */
+int mpdhg_device_count(int *count);
+/* gpu_solver.cpp:48-58; used = total - free */
+int mpdhg_mem_info(int device, size_t *total_bytes, size_t *free_bytes, size_t *used_bytes);
+const char *mpdhg_last_error(void);
+void mpdhg_config_default(mpdhg_config *cfg);
+
+/* ---- contexts (the reference allocates and frees per call,
+ *      gpu_solver.cpp:153-216 and :276-288; a context keeps them) ------- */
+int mpdhg_create(int device, mpdhg_ctx **out);
+void mpdhg_destroy(mpdhg_ctx *ctx);
+
+/* GpuSolver::solve_pdhg (gpu_solver.cpp:60-307).  ctx == NULL makes a
+ * temporary context on device 0 (the reference's per-call lifetime). */
+int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
+                mpdhg_result *res);
+
+/* Batched solve (extension): `count` LPs of any sizes in one device-resident
+ * sweep; every launch covers all still-running instances.  Results are
+ * per instance, identical to mpdhg_solve on each LP in reference mode. */
+int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count,
+                      const mpdhg_config *cfg, mpdhg_result *res);
+
+/* Basis identification of GpuSolver::solve_hybrid (gpu_solver.cpp:319-395):
+ * the m most interior of the n columns and m row activities become Basic,
+ * the rest sit at the nearer bound.  Ties keep index order (the reference's
+ * std::sort leaves them unspecified).  col_basis[n], row_basis[m]. */
+int mpdhg_identify_basis(const mpdhg_lp *lp, const double *x, int8_t *col_basis,
+                         int8_t *row_basis);
+
+/* ---- synthetic workloads (owned by the library) ----------------------- */
+typedef struct mpdhg_lp_owned mpdhg_lp_owned;
+/* Random equality LP: nnz_per_col distinct rows per column, values U[-1,1],
+ * 0<=x<=10, b = A x0 with x0~U[0,10], c = A^T y0 + s, y0~N(0,1),
+ * s~U[0,1] zeroed on floor(0.4 n) random columns. */
+int mpdhg_gen_random(int64_t m, int64_t n, int32_t nnz_per_col, uint64_t seed,
+                     mpdhg_lp_owned **out);
+/* Refinery staircase: T blocks of R rows x C columns (product balances,
+ * blend-quality rows, unit-capacity rows; crude, sale and inventory
+ * columns; inventory links block t to t+1). */
+int mpdhg_gen_staircase(int32_t T, int32_t R, int32_t C, uint64_t seed, mpdhg_lp_owned **out);
+/* Scenario: copy of `base` with c_j *= exp(shock_sigma N(0,1)); if
+ * zero_capacity and the LP has capacity rows, one of them gets ru = 0. */
+int mpdhg_gen_scenario(const mpdhg_lp_owned *base, uint64_t seed, double shock_sigma,
+                       int32_t zero_capacity, mpdhg_lp_owned **out);
+/* Borrowed view of an owned LP (valid until mpdhg_lp_free). */
+const mpdhg_lp *mpdhg_lp_view(const mpdhg_lp_owned *lp);
+/* Row kinds of a staircase LP (0 balance, 1 blend, 2 capacity), else NULL. */
+const int8_t *mpdhg_lp_row_kind(const mpdhg_lp_owned *lp);
+void mpdhg_lp_free(mpdhg_lp_owned *lp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPDHG_H */

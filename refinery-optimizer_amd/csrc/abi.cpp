@@ -1,0 +1,172 @@
+// extern "C" entry points of libmpdhg (declared in include/mpdhg.h).
+// Every call catches internal exceptions and turns them into an MPDHG_E_*
+// code plus a thread-local message.
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+#include "mpdhg.h"
+#include "mpdhg_internal.hpp"
+
+namespace mpdhg {
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace mpdhg
+
+struct mpdhg_ctx {
+    mpdhg::Context c;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const mpdhg::Error &e) {
+        return mpdhg::fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return mpdhg::fail(MPDHG_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return mpdhg::fail(MPDHG_E_STATE, e.what());
+    }
+}
+
+int check_config(const mpdhg_config *cfg) {
+    if (!cfg) return mpdhg::fail(MPDHG_E_ARG, "config is NULL");
+    if (cfg->mode != MPDHG_MODE_REFERENCE && cfg->mode != MPDHG_MODE_ENHANCED)
+        return mpdhg::fail(MPDHG_E_ARG, "unknown mode");
+    if (cfg->check_frequency <= 0) return mpdhg::fail(MPDHG_E_ARG, "check_frequency must be > 0");
+    return MPDHG_OK;
+}
+
+int make_ctx(int device, mpdhg_ctx **out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return mpdhg::fail(MPDHG_E_NODEV, "no HIP device visible");
+    if (device < 0 || device >= n) return mpdhg::fail(MPDHG_E_ARG, "device index out of range");
+    return guarded([&] {
+        MPDHG_HIP(hipSetDevice(device));
+        auto *c = new mpdhg_ctx();
+        c->c.device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            mpdhg::hip_check(e, "hipStreamCreateWithFlags");
+        }
+        *out = c;
+        return MPDHG_OK;
+    });
+}
+
+void clear_result(mpdhg_result *r) {
+    double *x = r->x, *y = r->y, *rc = r->reduced_costs, *ra = r->row_activity;
+    std::memset(r, 0, sizeof(*r));
+    r->x = x;
+    r->y = y;
+    r->reduced_costs = rc;
+    r->row_activity = ra;
+    r->status = MPDHG_STATUS_UNKNOWN;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpdhg_last_error(void) { return mpdhg::g_last_error.c_str(); }
+
+void mpdhg_config_default(mpdhg_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->max_iterations = 4000;  // gpu_solver.hpp:13-21
+    cfg->tol_primal = 1e-4f;
+    cfg->tol_dual = 1e-4f;
+    cfg->tol_gap = 1e-4f;
+    cfg->check_frequency = 25;
+    cfg->step_tau = 0.0f;
+    cfg->step_sigma = 0.0f;
+    cfg->enable_restart = 1;
+    cfg->enable_hybrid_polish = 1;
+    cfg->mode = MPDHG_MODE_REFERENCE;
+    cfg->tol_kkt = 0.0;
+    cfg->max_refinements = 0;
+    cfg->reflection = 1.0f;
+    cfg->ruiz_iterations = 10;
+    cfg->profile = 0;
+    cfg->dual_sign = 0;
+}
+
+int mpdhg_device_count(int *count) {
+    if (!count) return mpdhg::fail(MPDHG_E_ARG, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return MPDHG_OK;
+}
+
+int mpdhg_mem_info(int device, size_t *total, size_t *free_b, size_t *used) {
+    if (!total || !free_b || !used) return mpdhg::fail(MPDHG_E_ARG, "NULL output");
+    *total = *free_b = *used = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return mpdhg::fail(MPDHG_E_NODEV, "no HIP device visible");
+    if (device < 0 || device >= n) return mpdhg::fail(MPDHG_E_ARG, "device index out of range");
+    return guarded([&] {
+        MPDHG_HIP(hipSetDevice(device));
+        size_t f = 0, t = 0;
+        MPDHG_HIP(hipMemGetInfo(&f, &t));
+        *total = t;
+        *free_b = f;
+        *used = t - f;
+        return MPDHG_OK;
+    });
+}
+
+int mpdhg_create(int device, mpdhg_ctx **out) {
+    if (!out) return mpdhg::fail(MPDHG_E_ARG, "out is NULL");
+    *out = nullptr;
+    return make_ctx(device, out);
+}
+
+void mpdhg_destroy(mpdhg_ctx *ctx) { delete ctx; }
+
+int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdhg_config *cfg,
+                      mpdhg_result *res) {
+    if (count < 0 || (count > 0 && (!lps || !res))) return mpdhg::fail(MPDHG_E_ARG, "bad batch");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    mpdhg_ctx *tmp = nullptr;
+    if (!ctx) {
+        rc = make_ctx(0, &tmp);
+        if (rc) return rc;
+        ctx = tmp;
+    }
+    for (int q = 0; q < count; ++q) clear_result(&res[q]);
+    rc = guarded([&] {
+        MPDHG_HIP(hipSetDevice(ctx->c.device));
+        if (cfg->mode == MPDHG_MODE_REFERENCE)
+            mpdhg::solve_reference(ctx->c, lps, count, *cfg, res);
+        else
+            mpdhg::solve_enhanced(ctx->c, lps, count, *cfg, res);
+        return MPDHG_OK;
+    });
+    delete tmp;
+    return rc;
+}
+
+int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg, mpdhg_result *res) {
+    if (!lp || !res) return mpdhg::fail(MPDHG_E_ARG, "NULL lp or result");
+    return mpdhg_solve_batch(ctx, lp, 1, cfg, res);
+}
+
+}  // extern "C"

@@ -1,0 +1,417 @@
+// Host engine: LP validation and layout, tile planning, and the
+// reference-mode driver loop (gpu_solver.cpp:60-307) over a batch.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace mpdhg {
+
+double now_sec() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(e == hipErrorOutOfMemory ? MPDHG_E_NOMEM : MPDHG_E_HIP,
+                    std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Context::~Context() {
+    for (auto ev : events) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Validates the int64 CSC of model::SparseMatrix (sparse_matrix.hpp:38-40)
+// and narrows it to int32 -- element by element, which is the copy the
+// reference gets wrong at gpu_solver.cpp:202-203 by moving int64 words with
+// int32 byte counts.  Builds the CSR by the counting sort of
+// gpu_solver.cpp:92-109 (columns ascending within each row).
+void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
+    const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
+    if (lp.m < 0 || lp.n < 0 || lp.nnz < 0 || lp.m > lim || lp.n > lim || lp.nnz > lim)
+        throw Error(MPDHG_E_ARG, "LP dimensions out of range (m, n, nnz must fit int32)");
+    h.src = &lp;
+    h.m = static_cast<int>(lp.m);
+    h.n = static_cast<int>(lp.n);
+    h.nnz = static_cast<int>(lp.nnz);
+    if (h.n > 0 && (!lp.col_ptr || !lp.c || !lp.col_lower || !lp.col_upper))
+        throw Error(MPDHG_E_ARG, "LP column arrays are NULL");
+    if (h.m > 0 && (!lp.row_lower || !lp.row_upper))
+        throw Error(MPDHG_E_ARG, "LP row-bound arrays are NULL");
+    if (h.nnz > 0 && (!lp.row_ind || !lp.vals)) throw Error(MPDHG_E_ARG, "LP nonzero arrays are NULL");
+    h.cptr.assign(static_cast<size_t>(h.n) + 1, 0);
+    h.cidx.resize(static_cast<size_t>(h.nnz));
+    if (h.n > 0) {
+        if (lp.col_ptr[0] != 0 || lp.col_ptr[h.n] != lp.nnz)
+            throw Error(MPDHG_E_ARG, "col_ptr must start at 0 and end at nnz");
+        for (int j = 0; j < h.n; ++j) {
+            if (lp.col_ptr[j + 1] < lp.col_ptr[j]) throw Error(MPDHG_E_ARG, "col_ptr not monotone");
+            h.cptr[static_cast<size_t>(j) + 1] = static_cast<int>(lp.col_ptr[j + 1]);
+        }
+    } else if (h.nnz != 0) {
+        throw Error(MPDHG_E_ARG, "nonzeros without columns");
+    }
+    for (int k = 0; k < h.nnz; ++k) {
+        const int64_t r = lp.row_ind[k];
+        if (r < 0 || r >= lp.m) throw Error(MPDHG_E_ARG, "row index out of range");
+        h.cidx[static_cast<size_t>(k)] = static_cast<int>(r);
+    }
+    h.rptr.assign(static_cast<size_t>(h.m) + 1, 0);
+    for (int k = 0; k < h.nnz; ++k) h.rptr[static_cast<size_t>(h.cidx[k]) + 1]++;
+    for (int i = 0; i < h.m; ++i) h.rptr[static_cast<size_t>(i) + 1] += h.rptr[static_cast<size_t>(i)];
+    std::vector<int> fill(h.rptr.begin(), h.rptr.end());
+    h.ridx.resize(static_cast<size_t>(h.nnz));
+    h.csc2csr.resize(static_cast<size_t>(h.nnz));
+    for (int j = 0; j < h.n; ++j)
+        for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) {
+            const int d = fill[static_cast<size_t>(h.cidx[k])]++;
+            h.ridx[static_cast<size_t>(d)] = j;
+            h.csc2csr[static_cast<size_t>(k)] = d;
+        }
+}
+
+// Rows [start, i) form one tile while the tile has at most kBlock rows and
+// at most kTileNnz nonzeros; a single longer row gets a tile of its own
+// (the kernel streams it in several chunks).
+void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst,
+                std::vector<Tile> &out) {
+    int i = 0;
+    while (i < count) {
+        const int start = i;
+        int nz = 0;
+        while (i < count && i - start < kBlock) {
+            const int rn = ptr[static_cast<size_t>(i) + 1] - ptr[static_cast<size_t>(i)];
+            if (i > start && nz + rn > kTileNnz) break;
+            nz += rn;
+            ++i;
+        }
+        out.push_back(Tile{static_cast<int32_t>(global_off + start),
+                           static_cast<int32_t>(global_off + i), inst, 0});
+    }
+}
+
+TileSet make_tileset(const DBuf<Tile> &d, int count) {
+    TileSet ts;
+    ts.tiles = d.get();
+    ts.count = count;
+    ts.chunk = (count + 7) / 8;
+    ts.grid = 8 * ts.chunk;
+    return ts;
+}
+
+static inline bool finite_bound(double v) { return std::isfinite(v); }
+
+void host_kkt(const mpdhg_lp &lp, const HostLp &h, const std::vector<double> &x,
+              const std::vector<double> &y, mpdhg_result &r) {
+    const int m = h.m, n = h.n;
+    std::vector<double> ax(static_cast<size_t>(m), 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k)
+            ax[static_cast<size_t>(h.cidx[k])] += lp.vals[k] * x[static_cast<size_t>(j)];
+    double bn = 0, cn = 0, rowv = 0, colv = 0, dv = 0, pobj = 0, dobj = 0;
+    for (int i = 0; i < m; ++i) {
+        const double lo = lp.row_lower[i], hi = lp.row_upper[i], a = ax[static_cast<size_t>(i)];
+        if (finite_bound(lo)) bn = std::max(bn, std::fabs(lo));
+        if (finite_bound(hi)) bn = std::max(bn, std::fabs(hi));
+        double v = 0;
+        if (finite_bound(lo) && a < lo) v = lo - a;
+        if (finite_bound(hi) && a > hi) v = a - hi;
+        rowv = std::max(rowv, v);
+        // row-dual sign set: y > 0 needs finite ru, y < 0 finite rl
+        const double yr = y[static_cast<size_t>(i)];
+        double yi = yr;
+        if (!finite_bound(hi) && yi > 0) yi = 0;
+        if (!finite_bound(lo) && yi < 0) yi = 0;
+        dv = std::max(dv, std::fabs(yr - yi));
+        if (yi > 0) dobj -= hi * yi;
+        else if (yi < 0) dobj -= lo * yi;
+        if (r.row_activity) r.row_activity[i] = a;
+    }
+    for (int j = 0; j < n; ++j) {
+        const double lo = lp.col_lower[j], hi = lp.col_upper[j], xj = x[static_cast<size_t>(j)];
+        cn = std::max(cn, std::fabs(lp.c[j]));
+        double v = 0;
+        if (finite_bound(lo) && xj < lo) v = lo - xj;
+        if (finite_bound(hi) && xj > hi) v = xj - hi;
+        colv = std::max(colv, v);
+        double lam = lp.c[j];
+        for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k)
+            lam += lp.vals[k] * y[static_cast<size_t>(h.cidx[k])];
+        double p = lam;
+        if (!finite_bound(lo) && p > 0) p = 0;
+        if (!finite_bound(hi) && p < 0) p = 0;
+        dv = std::max(dv, std::fabs(lam - p));
+        if (p > 0) dobj += lo * p;
+        else if (p < 0) dobj += hi * p;
+        pobj += lp.c[j] * xj;
+        if (r.reduced_costs) r.reduced_costs[j] = lam;
+    }
+    r.rel_primal = std::max(rowv, colv) / (1.0 + bn);
+    r.rel_dual = dv / (1.0 + cn);
+    r.rel_gap = std::fabs(pobj - dobj) / (1.0 + std::fabs(pobj) + std::fabs(dobj));
+    r.rel_kkt = std::max(r.rel_primal, std::max(r.rel_dual, r.rel_gap));
+    r.dual_objective = dobj;
+}
+
+namespace {
+
+struct RefHost {
+    std::vector<int> cptr, cidx, rptr, ridx;
+    std::vector<float> cval, rval, c, l, u, tau, rl, ru, sigma, x0;
+    std::vector<Tile> rtiles, ctiles;
+    std::vector<RefInst> inst;
+};
+
+void event_pool(Context &ctx, size_t n) {
+    while (ctx.events.size() < n) {
+        hipEvent_t e;
+        MPDHG_HIP(hipEventCreate(&e));
+        ctx.events.push_back(e);
+    }
+}
+
+}  // namespace
+
+// Reference-mode batch solve.  Per instance this is gpu_solver.cpp:60-307
+// with a correct A^T y; all running instances advance in the same launches.
+void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
+                     mpdhg_result *res) {
+    const double t_call = now_sec();
+    if (cfg.check_frequency <= 0) throw Error(MPDHG_E_ARG, "check_frequency must be positive");
+    std::vector<HostLp> H(static_cast<size_t>(count));
+    std::vector<int> act;
+    for (int q = 0; q < count; ++q) {
+        prepare_structure(lps[q], H[static_cast<size_t>(q)]);
+        mpdhg_result &r = res[q];
+        r.status = MPDHG_STATUS_UNKNOWN;
+        r.iterations = 0;
+        r.restarts = r.refinements = r.checks = 0;
+        if (H[static_cast<size_t>(q)].m == 0 || H[static_cast<size_t>(q)].n == 0) {
+            r.status = MPDHG_STATUS_OPTIMAL;  // gpu_solver.cpp:77-80
+            r.primal_objective = 0;
+        } else {
+            act.push_back(q);
+        }
+    }
+    const int na = static_cast<int>(act.size());
+    if (na == 0) {
+        for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;
+        return;
+    }
+    // ---- host preparation (gpu_solver.cpp:82-151, 192-196) ----
+    RefHost hh;
+    BatchLayout L;
+    L.row_off.resize(static_cast<size_t>(na));
+    L.col_off.resize(static_cast<size_t>(na));
+    L.nnz_off.resize(static_cast<size_t>(na));
+    for (int a = 0; a < na; ++a) {
+        const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+        L.row_off[static_cast<size_t>(a)] = L.rows;
+        L.col_off[static_cast<size_t>(a)] = L.cols;
+        L.nnz_off[static_cast<size_t>(a)] = L.nnz;
+        L.rows += h.m;
+        L.cols += h.n;
+        L.nnz += h.nnz;
+    }
+    if (L.rows > std::numeric_limits<int32_t>::max() - kTileNnz ||
+        L.cols > std::numeric_limits<int32_t>::max() - kTileNnz ||
+        L.nnz > std::numeric_limits<int32_t>::max() - kTileNnz)
+        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
+                 Z = static_cast<size_t>(L.nnz);
+    hh.cptr.resize(C + 1);
+    hh.cidx.resize(Z);
+    hh.rptr.resize(R + 1);
+    hh.ridx.resize(Z);
+    hh.cval.resize(Z);
+    hh.rval.resize(Z);
+    hh.c.resize(C); hh.l.resize(C); hh.u.resize(C); hh.tau.resize(C); hh.x0.resize(C);
+    hh.rl.resize(R); hh.ru.resize(R); hh.sigma.resize(R);
+    hh.inst.resize(static_cast<size_t>(na));
+    for (int a = 0; a < na; ++a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        const mpdhg_lp &lp = lps[q];
+        const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)],
+                      zo = L.nnz_off[static_cast<size_t>(a)];
+        for (int j = 0; j <= h.n; ++j) hh.cptr[static_cast<size_t>(co + j)] = static_cast<int>(zo + h.cptr[j]);
+        for (int i = 0; i <= h.m; ++i) hh.rptr[static_cast<size_t>(ro + i)] = static_cast<int>(zo + h.rptr[i]);
+        for (int k = 0; k < h.nnz; ++k) {
+            const float v = static_cast<float>(lp.vals[k]);  // :88-89
+            hh.cval[static_cast<size_t>(zo + k)] = v;
+            hh.cidx[static_cast<size_t>(zo + k)] = static_cast<int>(ro + h.cidx[k]);
+            const int d = h.csc2csr[static_cast<size_t>(k)];
+            hh.rval[static_cast<size_t>(zo + d)] = v;
+            hh.ridx[static_cast<size_t>(zo + d)] = static_cast<int>(co + h.ridx[d]);
+        }
+        float mac = 1.0f;  // :113-119
+        for (int j = 0; j < h.n; ++j) {
+            const size_t g = static_cast<size_t>(co + j);
+            hh.c[g] = static_cast<float>(lp.c[j]);
+            hh.l[g] = static_cast<float>(lp.col_lower[j]);
+            hh.u[g] = static_cast<float>(lp.col_upper[j]);
+            mac = std::max(mac, std::abs(hh.c[g]));
+        }
+        float mab = 1.0f;  // :120-126
+        for (int i = 0; i < h.m; ++i) {
+            const size_t g = static_cast<size_t>(ro + i);
+            hh.rl[g] = static_cast<float>(lp.row_lower[i]);
+            hh.ru[g] = static_cast<float>(lp.row_upper[i]);
+            if (hh.ru[g] < 1e15f) mab = std::max(mab, std::abs(hh.ru[g]));
+            if (hh.rl[g] > -1e15f) mab = std::max(mab, std::abs(hh.rl[g]));
+        }
+        for (int j = 0; j < h.n; ++j) {  // :130-140
+            float s = 0.0f;
+            for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) s += std::abs(hh.cval[static_cast<size_t>(zo + k)]);
+            const size_t g = static_cast<size_t>(co + j);
+            hh.tau[g] = (cfg.step_tau > 0.0f) ? cfg.step_tau : 0.95f / std::max(1.0f, s);
+            hh.x0[g] = std::max(hh.l[g], std::min(hh.u[g], 0.0f));  // :194-196
+        }
+        for (int i = 0; i < h.m; ++i) {  // :141-151
+            float s = 0.0f;
+            for (int k = h.rptr[i]; k < h.rptr[i + 1]; ++k) s += std::abs(hh.rval[static_cast<size_t>(zo + k)]);
+            hh.sigma[static_cast<size_t>(ro + i)] =
+                (cfg.step_sigma > 0.0f) ? cfg.step_sigma : 0.95f / std::max(1.0f, s);
+        }
+        RefInst &s = hh.inst[static_cast<size_t>(a)];
+        std::memset(&s, 0, sizeof(s));
+        s.status = kRunning;
+        s.best_res = 1e30f;
+        s.max_abs_c = mac;
+        s.max_abs_b = mab;
+        s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
+        plan_tiles(h.rptr, h.m, ro, a, hh.rtiles);
+        s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
+        s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
+        plan_tiles(h.cptr, h.n, co, a, hh.ctiles);
+        s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
+    }
+    // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
+    hipStream_t st = ctx.stream;
+    DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
+    DBuf<float> d_cval, d_rval, d_c, d_l, d_u, d_tau, d_rl, d_ru, d_sigma;
+    DBuf<float> d_x(C), d_xbar(C), d_xbest(C), d_y(R);
+    DBuf<Tile> d_rt, d_ct;
+    DBuf<RefInst> d_inst;
+    d_cptr.upload(hh.cptr, st); d_cidx.upload(hh.cidx, st); d_cval.upload(hh.cval, st);
+    d_rptr.upload(hh.rptr, st); d_ridx.upload(hh.ridx, st); d_rval.upload(hh.rval, st);
+    d_c.upload(hh.c, st); d_l.upload(hh.l, st); d_u.upload(hh.u, st); d_tau.upload(hh.tau, st);
+    d_rl.upload(hh.rl, st); d_ru.upload(hh.ru, st); d_sigma.upload(hh.sigma, st);
+    d_rt.upload(hh.rtiles, st); d_ct.upload(hh.ctiles, st); d_inst.upload(hh.inst, st);
+    MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
+    DBuf<float> d_prow(hh.rtiles.size()), d_pcol(hh.ctiles.size());
+    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
+    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
+    MPDHG_HIP(hipStreamSynchronize(st));
+
+    // ---- profiling samples: HIP events around the fused SpMV launches ----
+    const int K = cfg.max_iterations, F = cfg.check_frequency;
+    const int nsamp = cfg.profile ? std::min(256, std::max(K, 0)) : 0;
+    const int stride = nsamp ? std::max(1, K / nsamp) : 1;
+    if (nsamp) event_pool(ctx, static_cast<size_t>(3 * nsamp));
+    int used = 0;
+
+    // ---- hot loop (gpu_solver.cpp:227-271) ----
+    const double t_loop = now_sec();
+    std::vector<RefInst> hs(static_cast<size_t>(na));
+    int iter;
+    for (iter = 1; iter <= K; ++iter) {
+        const bool samp = nsamp && (iter % stride == 0) && used < nsamp;
+        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
+        launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), d_cidx.get(), d_cval.get(),
+                           d_y.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(), d_x.get(),
+                           d_xbar.get());
+        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
+        launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), d_ridx.get(), d_rval.get(),
+                         d_xbar.get(), d_rl.get(), d_ru.get(), d_sigma.get(), d_y.get());
+        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
+        if (samp) ++used;
+        // (:237-239 restart kernel omitted: it writes xbar = x, and the next
+        //  primal update overwrites xbar before anything reads it.)
+        if (iter % F == 0 || iter == K) {
+            launch::ref_rowres(st, rts, d_inst.get(), d_rptr.get(), d_ridx.get(), d_rval.get(),
+                               d_x.get(), d_rl.get(), d_ru.get(), d_prow.get());
+            launch::ref_colres(st, cts, d_inst.get(), d_cptr.get(), d_cidx.get(), d_cval.get(),
+                               d_y.get(), d_x.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(),
+                               d_pcol.get());
+            launch::ref_finalize(st, d_inst.get(), na, d_prow.get(), d_pcol.get(), iter, K,
+                                 cfg.tol_primal, cfg.tol_dual);
+            launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
+            MPDHG_HIP(hipGetLastError());
+            MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(RefInst),
+                                     hipMemcpyDeviceToHost, st));
+            MPDHG_HIP(hipStreamSynchronize(st));
+            bool any = false;
+            for (const auto &s : hs) any = any || (s.status == kRunning);
+            if (!any) break;
+        }
+    }
+    MPDHG_HIP(hipStreamSynchronize(st));
+    const double loop_sec = now_sec() - t_loop;
+    double ms_csc = 0, ms_csr = 0;
+    for (int s = 0; s < used; ++s) {
+        float a = 0, b = 0;
+        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(3 * s)], ctx.events[static_cast<size_t>(3 * s + 1)]));
+        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(3 * s + 1)], ctx.events[static_cast<size_t>(3 * s + 2)]));
+        ms_csc += a;
+        ms_csr += b;
+    }
+    MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(RefInst),
+                             hipMemcpyDeviceToHost, st));
+    std::vector<float> xb, yv;
+    d_xbest.download(xb, C, st);
+    d_y.download(yv, R, st);
+    MPDHG_HIP(hipStreamSynchronize(st));
+
+    // ---- results (gpu_solver.cpp:293-305) ----
+    for (int a = 0; a < na; ++a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        const mpdhg_lp &lp = lps[q];
+        mpdhg_result &r = res[q];
+        const RefInst &s = hs[static_cast<size_t>(a)];
+        const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)];
+        if (s.status == kRunning) {  // K <= 0: the reference loop never runs
+            r.status = MPDHG_STATUS_ITERATION_LIMIT;
+            r.iterations = 1;
+        } else {
+            r.status = s.status;
+            r.iterations = s.iterations;
+        }
+        r.checks = s.checks;
+        r.ref_primal = s.last_p;
+        r.ref_dual = s.last_d;
+        std::vector<double> x(static_cast<size_t>(h.n)), y(static_cast<size_t>(h.m));
+        double obj = 0.0;
+        for (int j = 0; j < h.n; ++j) {
+            x[static_cast<size_t>(j)] = static_cast<double>(xb[static_cast<size_t>(co + j)]);
+            obj += lp.c[j] * x[static_cast<size_t>(j)];
+        }
+        for (int i = 0; i < h.m; ++i) y[static_cast<size_t>(i)] = static_cast<double>(yv[static_cast<size_t>(ro + i)]);
+        r.primal_objective = obj;
+        if (r.x) std::memcpy(r.x, x.data(), x.size() * sizeof(double));
+        if (r.y) {
+            const double sg = cfg.dual_sign ? -1.0 : 1.0;
+            for (int i = 0; i < h.m; ++i) r.y[i] = sg * y[static_cast<size_t>(i)];
+        }
+        host_kkt(lp, h, x, y, r);
+        r.solve_sec = loop_sec;
+        r.kernel_ms_spmv_csc = ms_csc;
+        r.kernel_ms_spmv_csr = ms_csr;
+        r.kernel_launches_csc = r.kernel_launches_csr = used;
+        r.bytes_csc_launch = 8.0 * L.nnz + 4.0 * (L.cols + 1) + 4.0 * L.rows + 4.0 * 7 * L.cols;
+        r.bytes_csr_launch = 8.0 * L.nnz + 4.0 * (L.rows + 1) + 4.0 * L.cols + 4.0 * 5 * L.rows;
+    }
+    const double wall = now_sec() - t_call;
+    for (int q = 0; q < count; ++q) res[q].time_wall_sec = wall;
+}
+
+}  // namespace mpdhg

@@ -1,0 +1,128 @@
+// Device helpers shared by the tile kernels (HIP translation units only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine_dev.hpp"
+#include "launch.hpp"
+
+namespace mpdhg {
+namespace dev {
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap so that each XCD walks a contiguous range of tiles: tiles
+// that touch neighbouring rows/columns then share that XCD's L2.  The grid
+// is padded to 8 * chunk; the map is a bijection on [0, 8 * chunk).
+__device__ __forceinline__ int xcd_tile(unsigned b, int chunk) {
+    return static_cast<int>(b & 7u) * chunk + static_cast<int>(b >> 3);
+}
+
+template <typename A>
+__device__ __forceinline__ A fma_acc(float v, float x, A s);
+template <>
+__device__ __forceinline__ float fma_acc<float>(float v, float x, float s) {
+    return __builtin_fmaf(v, x, s);
+}
+template <>
+__device__ __forceinline__ double fma_acc<double>(float v, float x, double s) {
+    return __builtin_fma(static_cast<double>(v), static_cast<double>(x), s);
+}
+
+// Row sums of one tile: sum_k val[k] * xs[idx[k]] for every row of the tile,
+// in storage order (k ascending), one row per lane.  The tile's nonzeros
+// are streamed in chunks of kTileNnz: coalesced loads of idx/val, the
+// gathers issued back to back, then (val, x) pairs parked in LDS; each lane
+// then walks its own row from LDS.  A row longer than a chunk is continued
+// across chunks by the same lane, so the summation order never changes.
+template <typename A>
+__device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
+                                      const int *__restrict__ idx, const float *__restrict__ val,
+                                      const float *__restrict__ xs, float2 *lds) {
+    const int tid = static_cast<int>(threadIdx.x);
+    const int r = tl.begin + tid;
+    int k0 = 0, k1 = 0;
+    if (r < tl.end) {
+        k0 = ptr[r];
+        k1 = ptr[r + 1];
+    }
+    const int lo = ptr[tl.begin], hi = ptr[tl.end];
+    A sum = A(0);
+    for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
+        const int c1 = min(hi, c0 + kTileNnz);
+        int id[kPerThread];
+        float v[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int k = c0 + tid + q * kBlock;
+            if (k < c1) {
+                id[q] = idx[k];
+                v[q] = val[k];
+            }
+        }
+        float g[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int k = c0 + tid + q * kBlock;
+            if (k < c1) g[q] = xs[id[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int k = c0 + tid + q * kBlock;
+            if (k < c1) lds[k - c0] = make_float2(v[q], g[q]);
+        }
+        __syncthreads();
+        const int a = max(k0, c0), b = min(k1, c1);
+        for (int k = a; k < b; ++k) {
+            const float2 p = lds[k - c0];
+            sum = fma_acc<A>(p.x, p.y, sum);
+        }
+        __syncthreads();
+    }
+    return sum;
+}
+
+// Block-wide max of non-negative floats (NaN operands are dropped, as
+// fmaxf does); every lane receives the result.
+__device__ __forceinline__ float block_max(float v, float *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    const int w = static_cast<int>(threadIdx.x) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    float r = red[0];
+#pragma unroll
+    for (int q = 1; q < kBlock / 64; ++q) r = fmaxf(r, red[q]);
+    return r;
+}
+
+__device__ __forceinline__ double block_max_d(double v, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int w = static_cast<int>(threadIdx.x) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int q = 1; q < kBlock / 64; ++q) r = fmax(r, red[q]);
+    return r;
+}
+
+// Block-wide FP64 sum in a fixed tree order (deterministic for a given
+// block shape); every lane receives the result.
+__device__ __forceinline__ double block_sum_d(double v, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = static_cast<int>(threadIdx.x) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int q = 1; q < kBlock / 64; ++q) r += red[q];
+    return r;
+}
+
+}  // namespace dev
+}  // namespace mpdhg

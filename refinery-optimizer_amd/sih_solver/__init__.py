@@ -1,0 +1,503 @@
+"""MI355X PDHG LP engine -- Python mirror of refinery-optimizer's GPU PDHG API.
+
+Names, fields and defaults follow the reference's binding of its GPU path
+(core/bindings/pybind_module.cpp:84-152 and :270-296,
+py/sih_solver/__init__.py:81-91), so code written against
+``sih_solver.solve_gpu_pdhg`` runs unchanged.  All arithmetic happens in
+libmpdhg.so (HIP kernels for gfx950 behind the C ABI of include/mpdhg.h).
+
+Deviation from the reference: with no HIP device visible, ``solve_pdhg``
+raises instead of running a CPU dual simplex (gpu_solver.cpp:66-71);
+``GpuSolver.is_cuda_available()`` tells the caller beforehand.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+from dataclasses import dataclass, field
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import MPDHG_MODE_ENHANCED, MPDHG_MODE_REFERENCE, MpdhgConfig, MpdhgLp, MpdhgResult
+
+INF = math.inf
+SIH_INFINITY = math.inf
+
+
+class SolutionStatus(enum.IntEnum):  # solution.hpp:11-21
+    Optimal = 0
+    Infeasible = 1
+    Unbounded = 2
+    TimeLimit = 3
+    IterationLimit = 4
+    NodeLimit = 5
+    NumericalFailure = 6
+    Interrupted = 7
+    Unknown = 8
+
+
+class BasisStatus(enum.IntEnum):  # solution.hpp:23-29
+    Basic = 0
+    AtLower = 1
+    AtUpper = 2
+    Free = 3
+    Superbasic = 4
+
+
+class ObjectiveSense(enum.IntEnum):  # problem.hpp:11-14
+    Minimize = 1
+    Maximize = -1
+
+
+@dataclass
+class Triplet:  # sparse_matrix.hpp:10-17
+    row: int = 0
+    col: int = 0
+    value: float = 0.0
+
+
+class SparseMatrix:
+    """CSC + CSR matrix with int64 indices and double values (sparse_matrix.hpp:20-77)."""
+
+    def __init__(self, rows: int = 0, cols: int = 0):
+        self._m, self._n = int(rows), int(cols)
+        self._cp = np.zeros(self._n + 1, np.int64)
+        self._ri = np.zeros(0, np.int64)
+        self._cv = np.zeros(0, np.float64)
+        self._csr = None
+
+    @staticmethod
+    def from_triplets(m: int, n: int, triplets: Iterable[Triplet], zero_tol: float = 1e-15):
+        """sparse_matrix.cpp:13-88: duplicates summed, |v| <= zero_tol dropped,
+        CSC ordered by (col, row)."""
+        acc = {}
+        for t in triplets:
+            r, c_ = int(t.row), int(t.col)
+            if not (0 <= r < m and 0 <= c_ < n):
+                raise IndexError("Triplet indices out of matrix bounds")
+            acc[(c_, r)] = acc.get((c_, r), 0.0) + float(t.value)
+        keys = sorted(k for k, v in acc.items() if abs(v) > zero_tol)
+        cols = np.array([k[0] for k in keys], np.int64)
+        rows = np.array([k[1] for k in keys], np.int64)
+        vals = np.array([acc[k] for k in keys], np.float64)
+        cp = np.zeros(n + 1, np.int64)
+        np.add.at(cp, cols + 1, 1)
+        return SparseMatrix.from_csc(m, n, np.cumsum(cp), rows, vals)
+
+    @staticmethod
+    def from_csc(m: int, n: int, col_ptr, row_ind, vals):
+        A = SparseMatrix(m, n)
+        A._cp = np.ascontiguousarray(col_ptr, np.int64)
+        A._ri = np.ascontiguousarray(row_ind, np.int64)
+        A._cv = np.ascontiguousarray(vals, np.float64)
+        if A._cp.shape != (n + 1,) or A._ri.shape != A._cv.shape or A._cp[-1] != A._ri.size:
+            raise ValueError("inconsistent CSC arrays")
+        return A
+
+    def num_rows(self) -> int:
+        return self._m
+
+    def num_cols(self) -> int:
+        return self._n
+
+    def num_nonzeros(self) -> int:
+        return int(self._cv.size)
+
+    def csc_col_ptr(self):
+        return self._cp
+
+    def csc_row_ind(self):
+        return self._ri
+
+    def csc_values(self):
+        return self._cv
+
+    def _build_csr(self):
+        if self._csr is None:
+            cols = np.repeat(np.arange(self._n, dtype=np.int64), np.diff(self._cp))
+            order = np.lexsort((cols, self._ri))
+            rp = np.zeros(self._m + 1, np.int64)
+            np.add.at(rp, self._ri + 1, 1)
+            self._csr = (np.cumsum(rp), cols[order], self._cv[order])
+        return self._csr
+
+    def csr_row_ptr(self):
+        return self._build_csr()[0]
+
+    def csr_col_ind(self):
+        return self._build_csr()[1]
+
+    def csr_values(self):
+        return self._build_csr()[2]
+
+    def mat_vec(self, x):
+        x = np.asarray(x, np.float64)
+        if x.size != self._n:
+            raise ValueError("Vector length does not match matrix column count in mat_vec")
+        rp, ci, v = self._build_csr()
+        out = np.zeros(self._m)
+        np.add.at(out, np.repeat(np.arange(self._m), np.diff(rp)), v * x[ci])
+        return out
+
+    def mat_trans_vec(self, y):
+        y = np.asarray(y, np.float64)
+        if y.size != self._m:
+            raise ValueError("Vector length does not match matrix row count in mat_trans_vec")
+        out = np.zeros(self._n)
+        np.add.at(out, np.repeat(np.arange(self._n), np.diff(self._cp)), self._cv * y[self._ri])
+        return out
+
+
+class Problem:
+    """The LP fields of model::Problem (problem.hpp:37-137) that the GPU path reads."""
+
+    def __init__(self, name: str = "problem"):
+        self.name = name
+        self.sense = ObjectiveSense.Minimize
+        self.obj_offset = 0.0
+        self._A = SparseMatrix(0, 0)
+        self.c = np.zeros(0)
+        self.row_lower = np.zeros(0)
+        self.row_upper = np.zeros(0)
+        self.col_lower = np.zeros(0)
+        self.col_upper = np.zeros(0)
+        self.row_kind = None  # staircase workloads: 0 balance, 1 blend, 2 capacity
+
+    def resize(self, num_rows: int, num_cols: int):  # problem.cpp:8-23
+        self._A = SparseMatrix(num_rows, num_cols)
+        self.c = np.zeros(num_cols)
+        self.row_lower = np.full(num_rows, -INF)
+        self.row_upper = np.full(num_rows, INF)
+        self.col_lower = np.zeros(num_cols)
+        self.col_upper = np.full(num_cols, INF)
+
+    @property
+    def A(self) -> SparseMatrix:
+        return self._A
+
+    @A.setter
+    def A(self, mat: SparseMatrix):
+        if mat.num_rows() != self.num_rows() or mat.num_cols() != self.num_cols():
+            raise ValueError("Constraint matrix A dimensions do not match problem dimensions")
+        self._A = mat
+
+    def num_rows(self) -> int:
+        return self._A.num_rows()
+
+    def num_cols(self) -> int:
+        return self._A.num_cols()
+
+    def num_nonzeros(self) -> int:
+        return self._A.num_nonzeros()
+
+    def is_mip(self) -> bool:
+        return False
+
+    # -- C view ------------------------------------------------------------
+    def _c_view(self):
+        """An MpdhgLp pointing into this problem's arrays (plus keep-alive refs)."""
+        m, n = self.num_rows(), self.num_cols()
+        arrs = [
+            np.ascontiguousarray(self._A.csc_col_ptr(), np.int64),
+            np.ascontiguousarray(self._A.csc_row_ind(), np.int64),
+            np.ascontiguousarray(self._A.csc_values(), np.float64),
+            np.ascontiguousarray(self.c, np.float64),
+            np.ascontiguousarray(self.col_lower, np.float64),
+            np.ascontiguousarray(self.col_upper, np.float64),
+            np.ascontiguousarray(self.row_lower, np.float64),
+            np.ascontiguousarray(self.row_upper, np.float64),
+        ]
+        if any(a.size != s for a, s in zip(arrs[3:], (n, n, n, m, m))):
+            raise ValueError("vector sizes do not match the problem dimensions")
+        lp = MpdhgLp()
+        lp.m, lp.n, lp.nnz = m, n, self._A.num_nonzeros()
+        lp.col_ptr = arrs[0].ctypes.data_as(C.POINTER(C.c_int64))
+        lp.row_ind = arrs[1].ctypes.data_as(C.POINTER(C.c_int64))
+        lp.vals = arrs[2].ctypes.data_as(C.POINTER(C.c_double))
+        for name, a in zip(("c", "col_lower", "col_upper", "row_lower", "row_upper"), arrs[3:]):
+            setattr(lp, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+        return lp, arrs
+
+
+@dataclass
+class Solution:
+    """model::Solution (solution.hpp:37-84) fields the GPU path fills, plus metrics."""
+
+    status: SolutionStatus = SolutionStatus.Unknown
+    x: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    slack: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    row_duals: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    reduced_costs: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    col_basis: List[BasisStatus] = field(default_factory=list)
+    row_basis: List[BasisStatus] = field(default_factory=list)
+    primal_objective: float = 0.0
+    dual_bound: float = 0.0
+    mip_gap: float = 0.0
+    simplex_iterations: int = 0
+    barrier_iterations: int = 0
+    nodes_explored: int = 0
+    time_wall_sec: float = 0.0
+    time_cpu_sec: float = 0.0
+    time_presolve_sec: float = 0.0
+    time_solver_sec: float = 0.0
+    # extension metrics
+    restarts: int = 0
+    refinements: int = 0
+    checks: int = 0
+    rel_primal: float = math.nan
+    rel_dual: float = math.nan
+    rel_gap: float = math.nan
+    rel_kkt: float = math.nan
+    ref_primal: float = math.nan
+    ref_dual: float = math.nan
+    kernel_ms_spmv_csr: float = 0.0
+    kernel_ms_spmv_csc: float = 0.0
+    kernel_launches_csr: int = 0
+    kernel_launches_csc: int = 0
+    bytes_csr_launch: float = 0.0
+    bytes_csc_launch: float = 0.0
+
+    def is_optimal(self) -> bool:
+        return self.status == SolutionStatus.Optimal
+
+    def is_feasible(self) -> bool:
+        return self.status in (SolutionStatus.Optimal, SolutionStatus.TimeLimit,
+                               SolutionStatus.IterationLimit, SolutionStatus.NodeLimit)
+
+
+@dataclass
+class GpuPdhgConfig:
+    """GpuPdhgConfig (gpu_solver.hpp:12-22) plus engine extensions."""
+
+    max_iterations: int = 4000
+    tol_primal: float = 1e-4
+    tol_dual: float = 1e-4
+    tol_gap: float = 1e-4
+    check_frequency: int = 25
+    step_tau: float = 0.0
+    step_sigma: float = 0.0
+    enable_restart: bool = True
+    enable_hybrid_polish: bool = True
+    # extensions
+    mode: int = MPDHG_MODE_REFERENCE
+    tol_kkt: float = 0.0
+    max_refinements: int = 0
+    reflection: float = 1.0
+    ruiz_iterations: int = 10
+    profile: bool = False
+    dual_sign: int = 0
+
+    def _c(self) -> MpdhgConfig:
+        c = MpdhgConfig()
+        _abi.lib().mpdhg_config_default(C.byref(c))
+        for name, _ in MpdhgConfig._fields_:
+            setattr(c, name, type(getattr(c, name))(getattr(self, name)))
+        return c
+
+
+@dataclass
+class GpuMemoryInfo:  # gpu_solver.hpp:24-28
+    total_bytes: int = 0
+    free_bytes: int = 0
+    used_bytes: int = 0
+
+
+def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
+    s = Solution()
+    s.status = SolutionStatus(r.status)
+    s.x, s.row_duals, s.reduced_costs, s.slack = x, y, rc, ax
+    s.primal_objective = r.primal_objective
+    s.dual_bound = r.dual_objective
+    s.simplex_iterations = int(r.iterations)
+    s.time_wall_sec = r.time_wall_sec
+    s.time_solver_sec = r.solve_sec
+    for name in ("restarts", "refinements", "checks", "rel_primal", "rel_dual", "rel_gap",
+                 "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
+                 "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
+                 "bytes_csr_launch", "bytes_csc_launch"):
+        setattr(s, name, getattr(r, name))
+    return s
+
+
+class _Ctx:
+    """A libmpdhg context (device buffers + one HIP stream) for reuse across solves."""
+
+    def __init__(self, device: int = 0):
+        self._p = C.c_void_p()
+        _abi.check(_abi.lib().mpdhg_create(device, C.byref(self._p)), "mpdhg_create")
+
+    def close(self):
+        if self._p:
+            _abi.lib().mpdhg_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[_Ctx] = None
+
+
+def _ctx() -> _Ctx:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = _Ctx(0)
+    return _default_ctx
+
+
+def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None) -> List[Solution]:
+    """Solve many LPs in one device-resident batch (extension of the reference API)."""
+    config = config or GpuPdhgConfig()
+    cfg = config._c()
+    nb = len(problems)
+    lps = (MpdhgLp * max(nb, 1))()
+    res = (MpdhgResult * max(nb, 1))()
+    keep, outs = [], []
+    for q, p in enumerate(problems):
+        lp, arrs = p._c_view()
+        lps[q] = lp
+        keep.append(arrs)
+        m, n = p.num_rows(), p.num_cols()
+        o = (np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(m))
+        outs.append(o)
+        res[q].x = o[0].ctypes.data_as(C.POINTER(C.c_double))
+        res[q].y = o[1].ctypes.data_as(C.POINTER(C.c_double))
+        res[q].reduced_costs = o[2].ctypes.data_as(C.POINTER(C.c_double))
+        res[q].row_activity = o[3].ctypes.data_as(C.POINTER(C.c_double))
+    _abi.check(_abi.lib().mpdhg_solve_batch(_ctx()._p, lps, nb, C.byref(cfg), res),
+               "mpdhg_solve_batch")
+    return [_result_to_solution(res[q], *outs[q]) for q in range(nb)]
+
+
+class GpuSolver:
+    """gpu_solver.hpp:30-45 static interface."""
+
+    @staticmethod
+    def is_cuda_available() -> bool:  # gpu_solver.cpp:42-46
+        n = C.c_int(0)
+        return _abi.lib().mpdhg_device_count(C.byref(n)) == 0 and n.value > 0
+
+    @staticmethod
+    def get_vram_info() -> GpuMemoryInfo:  # gpu_solver.cpp:48-58
+        info = GpuMemoryInfo()
+        if not GpuSolver.is_cuda_available():
+            return info
+        t, f, u = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _abi.check(_abi.lib().mpdhg_mem_info(0, C.byref(t), C.byref(f), C.byref(u)),
+                   "mpdhg_mem_info")
+        return GpuMemoryInfo(t.value, f.value, u.value)
+
+    @staticmethod
+    def solve_pdhg(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
+        return solve_batch([problem], config)[0]
+
+    @staticmethod
+    def solve_hybrid(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
+        """PDHG, then the basis hand-off of gpu_solver.cpp:319-395.  The FP64
+        simplex polish that consumes it is not part of this engine: the
+        returned Solution carries the PDHG point with col_basis/row_basis set."""
+        sol = GpuSolver.solve_pdhg(problem, config)
+        cb, rb = identify_basis(problem, sol.x)
+        sol.col_basis = [BasisStatus(int(v)) for v in cb]
+        sol.row_basis = [BasisStatus(int(v)) for v in rb]
+        return sol
+
+    @staticmethod
+    def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None):
+        return solve_batch(problems, config)
+
+
+def identify_basis(problem: Problem, x) -> tuple:
+    """Basis statuses from a primal point (gpu_solver.cpp:319-395), as int8 arrays."""
+    lp, keep = problem._c_view()
+    x = np.ascontiguousarray(x, np.float64)
+    cb = np.zeros(problem.num_cols(), np.int8)
+    rb = np.zeros(problem.num_rows(), np.int8)
+    _abi.check(_abi.lib().mpdhg_identify_basis(
+        C.byref(lp), x.ctypes.data_as(C.POINTER(C.c_double)),
+        cb.ctypes.data_as(C.POINTER(C.c_int8)), rb.ctypes.data_as(C.POINTER(C.c_int8))),
+        "mpdhg_identify_basis")
+    return cb, rb
+
+
+def solve_gpu_pdhg(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
+    """py/sih_solver/__init__.py:81-85."""
+    if config is None:
+        config = GpuPdhgConfig()
+    return GpuSolver.solve_pdhg(problem, config)
+
+
+def solve_gpu_hybrid(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
+    """py/sih_solver/__init__.py:87-91."""
+    if config is None:
+        config = GpuPdhgConfig()
+    return GpuSolver.solve_hybrid(problem, config)
+
+
+# ---- synthetic workloads --------------------------------------------------
+def _owned_to_problem(ptr: C.c_void_p, name: str) -> Problem:
+    L = _abi.lib()
+    v = L.mpdhg_lp_view(ptr).contents
+    m, n, nnz = v.m, v.n, v.nnz
+
+    def arr(p, k, dt):
+        return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True) if k else np.zeros(0, dt)
+
+    p = Problem(name)
+    p._A = SparseMatrix.from_csc(m, n, arr(v.col_ptr, n + 1, np.int64),
+                                 arr(v.row_ind, nnz, np.int64), arr(v.vals, nnz, np.float64))
+    p.c = arr(v.c, n, np.float64)
+    p.col_lower = arr(v.col_lower, n, np.float64)
+    p.col_upper = arr(v.col_upper, n, np.float64)
+    p.row_lower = arr(v.row_lower, m, np.float64)
+    p.row_upper = arr(v.row_upper, m, np.float64)
+    rk = L.mpdhg_lp_row_kind(ptr)
+    if rk:
+        p.row_kind = np.ctypeslib.as_array(rk, shape=(m,)).astype(np.int8, copy=True)
+    L.mpdhg_lp_free(ptr)
+    return p
+
+
+def gen_random(m: int, n: int, nnz_per_col: int, seed: int) -> Problem:
+    ptr = C.c_void_p()
+    _abi.check(_abi.lib().mpdhg_gen_random(m, n, nnz_per_col, seed, C.byref(ptr)), "gen_random")
+    return _owned_to_problem(ptr, f"random_{m}x{n}_s{seed}")
+
+
+def gen_staircase(T: int, R: int = 400, Cb: int = 900, seed: int = 0) -> Problem:
+    ptr = C.c_void_p()
+    _abi.check(_abi.lib().mpdhg_gen_staircase(T, R, Cb, seed, C.byref(ptr)), "gen_staircase")
+    return _owned_to_problem(ptr, f"staircase_T{T}_s{seed}")
+
+
+def gen_scenario(kind: str, size: tuple, base_seed: int, scenario_seed: int,
+                 shock_sigma: float = 0.3, zero_capacity: bool = False) -> Problem:
+    """A price-shocked scenario of a base workload: kind "random" with size
+    (m, n, nnz_per_col) or kind "staircase" with size (T, R, C)."""
+    L = _abi.lib()
+    base = C.c_void_p()
+    if kind == "random":
+        _abi.check(L.mpdhg_gen_random(*size, base_seed, C.byref(base)), "gen_random")
+    elif kind == "staircase":
+        _abi.check(L.mpdhg_gen_staircase(*size, base_seed, C.byref(base)), "gen_staircase")
+    else:
+        raise ValueError(f"unknown workload kind {kind!r}")
+    out = C.c_void_p()
+    rc = L.mpdhg_gen_scenario(base, scenario_seed, shock_sigma, int(zero_capacity), C.byref(out))
+    L.mpdhg_lp_free(base)
+    _abi.check(rc, "gen_scenario")
+    return _owned_to_problem(out, f"{kind}_{'x'.join(map(str, size))}_s{base_seed}_{scenario_seed}")
+
+
+__all__ = [
+    "SparseMatrix", "Triplet", "Problem", "Solution", "SolutionStatus", "BasisStatus",
+    "ObjectiveSense", "GpuSolver", "GpuPdhgConfig", "GpuMemoryInfo", "solve_gpu_pdhg",
+    "solve_gpu_hybrid", "solve_batch", "identify_basis", "gen_random", "gen_staircase",
+    "MPDHG_MODE_REFERENCE", "MPDHG_MODE_ENHANCED", "SIH_INFINITY",
+]

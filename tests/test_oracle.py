@@ -1,0 +1,181 @@
+"""Pins the oracle (CPU restatement of the reference GPU PDHG path) against
+closed forms, the reference's own known-answer tests and independent solvers."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+import oracle as O
+import sih_solver as S
+from lpfix import kat3x3, random_lp, toy01
+
+
+def test_spmv_kat_reference_unit_test():
+    # tests/unit/test_sparse_matrix.cpp:45-49 and :59-63
+    A = kat3x3()
+    cp, ri, v = A.csc_col_ptr(), A.csc_row_ind(), A.csc_values()
+    lp = O.LpArrays(3, 3, cp, ri, v, np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3),
+                    np.zeros(3))
+    rp, ci, cv = O.csc_to_csr(lp)
+    assert np.array_equal(O.spmv_csr(3, rp, ci, cv, [1.0, 2.0, 3.0]), [7.0, 6.0, 32.0])
+    assert np.array_equal(O.spmv_csc(3, cp, ri, v, [1.0, 0.0, 1.0]), [5.0, 5.0, 8.0])
+    assert np.array_equal(O.spmv_seq_f32(3, rp, ci, cv, [1, 2, 3]), [7, 6, 32])
+    assert np.array_equal(O.spmv_seq_f32(3, cp, ri, v, [1, 0, 1]), [5, 5, 8])
+
+
+def test_reference_csc_upload_defect_kat():
+    """The reference device reads int64 CSC arrays as int32 words
+    (gpu_solver.cpp:202-203); the restated A^T y on that view gives [0,5,0]
+    where the correct product is [5,5,8].  The engine narrows element-wise."""
+    A = kat3x3()
+    cp64, ri64 = A.csc_col_ptr().astype(np.int64), A.csc_row_ind().astype(np.int64)
+    d_cp = cp64.view(np.int32)[:4]  # (n+1)*sizeof(int) bytes of the int64 array
+    d_ri = ri64.view(np.int32)[:6]
+    assert d_cp.tolist() == [0, 0, 2, 0] and d_ri.tolist() == [0, 0, 2, 0, 1, 0]
+    y = np.array([1, 0, 1], np.float32)
+    out = np.zeros(3, np.float32)
+    vals = A.csc_values().astype(np.float32)
+    for j in range(3):  # cuda_pdhg.cu:33-49 on the corrupted arrays
+        s = np.float32(0)
+        for k in range(d_cp[j], d_cp[j + 1]):
+            s = np.float32(s + vals[k] * y[d_ri[k]])
+        out[j] = s
+    assert out.tolist() == [0.0, 5.0, 0.0]
+    assert O.spmv_seq_f32(3, A.csc_col_ptr(), A.csc_row_ind(), vals, y).tolist() == [5, 5, 8]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_csc_to_csr_matches_dense(seed):
+    p, dense = random_lp(37, 53, 0.15, seed)
+    lp = O.LpArrays.of(p)
+    rp, ci, cv = O.csc_to_csr(lp)
+    rebuilt = np.zeros_like(dense)
+    for i in range(lp.m):
+        cols = ci[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(cols) > 0)  # ascending columns within a row
+        rebuilt[i, cols] = cv[rp[i]:rp[i + 1]]
+    assert np.array_equal(rebuilt, dense)
+    x = np.random.default_rng(seed).normal(size=lp.n)
+    y = np.random.default_rng(seed + 9).normal(size=lp.m)
+    np.testing.assert_allclose(O.spmv_csr(lp.m, rp, ci, cv, x), dense @ x, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(O.spmv_csc(lp.n, lp.cp, lp.ri, lp.v, y), dense.T @ y, rtol=1e-12,
+                               atol=1e-12)
+
+
+def test_first_iterate_closed_form():
+    """One reference iteration from x0 = clamp(0,l,u), y0 = 0 (gpu_solver.cpp:192-196):
+    x1 = clamp(x0 - tau*c), xbar = 2 x1 - x0, y1 = sigma * (A xbar - clamp(A xbar, rl, ru)),
+    with tau_j = 0.95/max(1, |A_:j|_1), sigma_i = 0.95/max(1, |A_i:|_1)."""
+    p, dense = random_lp(20, 30, 0.3, 7)
+    lp = O.LpArrays.of(p)
+    r = O.pdhg_ref(lp, max_iterations=1, check_frequency=1, fp64=True)
+    tau = 0.95 / np.maximum(1.0, np.abs(dense).sum(axis=0))
+    sigma = 0.95 / np.maximum(1.0, np.abs(dense).sum(axis=1))
+    x0 = np.clip(0.0, lp.l, lp.u)
+    x1 = np.clip(x0 - tau * lp.c, lp.l, lp.u)
+    xbar = 2 * x1 - x0
+    w = dense @ xbar
+    y1 = sigma * (w - np.clip(w, lp.rl, lp.ru))
+    np.testing.assert_allclose(r["x_last"], x1, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(r["xbar_last"], xbar, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(r["y"], y1, rtol=1e-12, atol=1e-14)
+    assert r["iterations"] == 2 and r["status"] == 4  # K+1 when not converged
+
+
+def test_toy01_reference_pdhg_reaches_known_optimum():
+    lp = O.LpArrays.of(toy01())
+    for fp64 in (False, True):
+        r = O.pdhg_ref(lp, max_iterations=20000, tol_primal=1e-6, tol_dual=1e-6, fp64=fp64)
+        assert r["status"] == 0
+        np.testing.assert_allclose(r["x"], [1.5, 2.5], atol=2e-4)
+        assert abs(r["objective"] + 6.5) < 1e-3
+        np.testing.assert_allclose(r["y"], [0.5, 0.5], atol=2e-3)  # PDHG sign: c + A^T y = 0
+
+
+def test_restart_kernel_is_a_no_op():
+    """gpu_solver.cpp:237-239 sets xbar = x every 250 iterations; the next
+    primal update overwrites xbar before any read, so iterates are identical."""
+    p, _ = random_lp(40, 70, 0.2, 3)
+    lp = O.LpArrays.of(p)
+    a = O.pdhg_ref(lp, max_iterations=777, tol_primal=0, tol_dual=0, enable_restart=True)
+    b = O.pdhg_ref(lp, max_iterations=777, tol_primal=0, tol_dual=0, enable_restart=False)
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    assert np.array_equal(a["x_last"], b["x_last"])
+
+
+def test_iteration_limit_counter_quirk():
+    p, _ = random_lp(10, 15, 0.4, 1)
+    r = O.pdhg_ref(O.LpArrays.of(p), max_iterations=60, tol_primal=0, tol_dual=0,
+                   check_frequency=25)
+    assert r["status"] == 4 and r["iterations"] == 61 and r["checks"] == 3  # 25, 50, 60
+
+
+def test_kkt_closed_forms_toy01():
+    lp = O.LpArrays.of(toy01())
+    k = O.kkt(lp, [1.5, 2.5], [0.5, 0.5])
+    assert k["kkt"] < 1e-15 and k["pobj"] == -6.5 and k["dobj"] == -6.5
+    k0 = O.kkt(lp, [0.0, 0.0], [0.0, 0.0])
+    # lambda = c = (-1,-2) with finite boxes: dual objective = -10*1 - 10*2 = -30
+    assert k0["eps_p"] == 0 and k0["eps_d"] == 0 and k0["dobj"] == -30.0
+    assert math.isclose(k0["eps_gap"], 30.0 / 31.0, rel_tol=1e-15)
+    kv = O.kkt(lp, [2.0, 2.5], [0.5, 0.5])  # row 1: 4.5 > 4, row 2: 9.5 > 9
+    assert math.isclose(kv["eps_p"], 0.5 / 10.0, rel_tol=1e-15)
+
+
+def _highs(lp):
+    A = sp.csc_matrix((lp.v, lp.ri, lp.cp), shape=(lp.m, lp.n))
+    eq = lp.rl == lp.ru
+    ub_rows, ub_rhs = [], []
+    fin_u = (~eq) & np.isfinite(lp.ru)
+    fin_l = (~eq) & np.isfinite(lp.rl)
+    Aub = sp.vstack([A[fin_u], -A[fin_l]]).tocsr()
+    bub = np.concatenate([lp.ru[fin_u], -lp.rl[fin_l]])
+    s = linprog(lp.c, A_ub=Aub if Aub.shape[0] else None, b_ub=bub if bub.size else None,
+                A_eq=A[eq] if eq.any() else None, b_eq=lp.rl[eq] if eq.any() else None,
+                bounds=list(zip(lp.l, np.where(np.isfinite(lp.u), lp.u, None))), method="highs")
+    assert s.status == 0
+    y = np.zeros(lp.m)  # reference PDHG sign: y = -d obj / d b
+    if eq.any():
+        y[eq] = -s.eqlin.marginals
+    mu = -s.ineqlin.marginals if bub.size else np.zeros(0)
+    nu_ = int(fin_u.sum())
+    y[fin_u] += mu[:nu_]
+    y[fin_l] -= mu[nu_:]
+    return s, y
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_kkt_zero_at_highs_optimum_and_pdhg_converges_to_it(seed):
+    p, _ = random_lp(30, 45, 0.25, seed)
+    lp = O.LpArrays.of(p)
+    s, y = _highs(lp)
+    k = O.kkt(lp, s.x, y)
+    assert k["kkt"] < 1e-8
+    k_bad = O.kkt(lp, s.x, -y)
+    assert k_bad["kkt"] > 1e-3 or np.allclose(y, 0)
+    r = O.pdhg_ref(lp, max_iterations=200000, tol_primal=1e-7, tol_dual=1e-7, fp64=True)
+    if r["status"] == 0:  # the reference method is slow; when it stops, it is right
+        assert abs(r["objective"] - s.fun) <= 1e-3 * (1 + abs(s.fun))
+
+
+def test_identify_basis_hand_computed_toy01():
+    lp = O.LpArrays.of(toy01())
+    cb, rb = O.identify_basis(lp, [1.5, 2.5])
+    # distances: x1 1.5, x2 2.5 (interior); rows tight at upper bounds (0)
+    assert cb.tolist() == [0, 0] and rb.tolist() == [2, 2]
+    cb, rb = O.identify_basis(lp, [0.0, 3.0])  # row2 tight, row1 slack 1, x1 at lower
+    # dists: x1 0 (lower), x2 3, row1 1, row2 0 (upper) -> basic: x2, row1
+    assert cb.tolist() == [1, 0] and rb.tolist() == [0, 2]
+
+
+def test_identify_basis_ties_keep_index_order():
+    p = S.Problem("ties")
+    p.resize(1, 3)
+    p.A = S.SparseMatrix.from_triplets(1, 3, [S.Triplet(0, j, 1.0) for j in range(3)])
+    p.row_lower, p.row_upper = np.array([-np.inf]), np.array([np.inf])
+    p.col_upper = np.full(3, 2.0)
+    cb, rb = O.identify_basis(O.LpArrays.of(p), [1.0, 1.0, 1.0])
+    # row activity 3 with infinite bounds -> distance 1e30 -> basic; all columns tie at 1
+    assert rb.tolist() == [0] and cb.tolist() == [1, 1, 1]
