@@ -74,6 +74,9 @@ typedef struct mpdhg_config {
     int32_t profile;              /* 1: time the fused SpMV kernels with HIP events */
     int32_t dual_sign;            /* 0: reference PDHG sign (reduced cost = c + A^T y);
                                      1: simplex/checker sign (c - A^T y, checker.cpp:231) */
+    double refine_ratio;          /* enhanced: start a refinement round once the round's best
+                                     KKT <= ratio * its starting KKT (1e-4) ... */
+    int32_t refine_patience;      /* ... or after this many checks without a new best (8) */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite
@@ -112,6 +115,8 @@ typedef struct mpdhg_result {
     double kernel_ms_spmv_csr, kernel_ms_spmv_csc; /* profile=1: summed event times */
     int64_t kernel_launches_csr, kernel_launches_csc;
     double bytes_csr_launch, bytes_csc_launch;     /* algorithmic bytes per fused launch */
+    double step_size;        /* enhanced: eta (tau = eta/omega, sigma = eta*omega) */
+    double primal_weight;    /* enhanced: final omega */
 } mpdhg_result;
 
 typedef struct mpdhg_ctx mpdhg_ctx;

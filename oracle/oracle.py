@@ -33,12 +33,13 @@ class RefOut(C.Structure):
 class EnhCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("check_frequency", C.c_int),
                 ("tol", C.c_double), ("reflection", C.c_double), ("ruiz_iters", C.c_int),
-                ("threads", C.c_int)]
+                ("threads", C.c_int), ("step", C.c_double), ("max_refinements", C.c_int),
+                ("refine_ratio", C.c_double), ("patience", C.c_int)]
 
 
 class EnhOut(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("restarts", C.c_int),
-                ("eps_p", C.c_double), ("eps_d", C.c_double), ("eps_gap", C.c_double),
+                ("refinements", C.c_int), ("eps_p", C.c_double), ("eps_d", C.c_double), ("eps_gap", C.c_double),
                 ("kkt", C.c_double), ("primal_obj", C.c_double), ("dual_obj", C.c_double),
                 ("step", C.c_double)]
 
@@ -177,8 +178,11 @@ def spmv_seq_f32(rows, ptr, idx, v, x):
 
 
 def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
-             reflection=1.0, ruiz_iters=10, threads=1):
-    cfg = EnhCfg(max_iterations, check_frequency, tol, reflection, ruiz_iters, threads)
+             reflection=1.0, ruiz_iters=10, threads=1, step=0.0, max_refinements=0,
+             refine_ratio=1e-3, patience=8):
+    """CPU port of the enhanced mode; returns a dict with the best point."""
+    cfg = EnhCfg(max_iterations, check_frequency, tol, reflection, ruiz_iters, threads, step,
+                 max_refinements, refine_ratio, patience)
     out = EnhOut()
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)
@@ -186,5 +190,5 @@ def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
                             C.byref(out))
     assert rc == 0
     return dict(status=out.status, iterations=out.iterations, restarts=out.restarts,
-                eps_p=out.eps_p, eps_d=out.eps_d, eps_gap=out.eps_gap, kkt=out.kkt,
+                refinements=out.refinements, eps_p=out.eps_p, eps_d=out.eps_d, eps_gap=out.eps_gap, kkt=out.kkt,
                 primal_obj=out.primal_obj, dual_obj=out.dual_obj, step=out.step, x=x, y=y)

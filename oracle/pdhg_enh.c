@@ -1,5 +1,347 @@
 /*
  * oracle/pdhg_enh.c -- TEST INFRASTRUCTURE ONLY.
- * CPU port of the engine's enhanced mode (filled in with the GPU enhanced path).
+ *
+ * CPU port of the engine's enhanced mode: the same algorithm the GPU runs,
+ * written as plain loops so that tests can compare iterates and the bench
+ * can time a CPU baseline of the same method.  No reference counterpart:
+ * the reference's solver has fixed diagonal steps and no restarts
+ * (gpu_solver.cpp:128-151, :237-239).  Published methods:
+ *   - Ruiz equilibration + Pock-Chambolle (alpha = 1) diagonal scaling
+ *     (Pock & Chambolle, ICCV 2011; Applegate et al., NeurIPS 2021);
+ *   - reflected restarted Halpern PDHG with fixed-point-residual restarts
+ *     (Lu & Yang, "Restarted Halpern PDHG for linear programming", 2024);
+ *   - primal weight smoothing at restarts (Applegate et al. 2021, theta=1/2);
+ *   - iterative refinement on the residual LP (Gleixner, Steffy & Wolter,
+ *     INFORMS J. Comput. 2016), FP32 inner solves with FP64 outer updates.
+ * FP32 iterate arithmetic follows the GPU kernels operation by operation
+ * (explicit fmaf; compiled with -ffp-contract=off).  FP64 reductions and
+ * the KKT check run in a different order than on the GPU, so agreement is
+ * to tolerance once a reduction feeds back into the iterates.
  */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
 #include "pdhg_oracle.h"
+
+typedef struct {
+    int m, n;
+    const int *cp, *ri;
+    const double *v;
+    int *rp, *ci, *c2r;
+    double *dr, *dc;          /* row / column scaling */
+    float *cv_s, *rv_s;       /* scaled FP32 values, CSC and CSR order */
+} enh_lp;
+
+static void *xmalloc(size_t n) { return malloc(n ? n : 1); }
+
+/* Ruiz (infinity-norm) passes, then one Pock-Chambolle (alpha = 1) pass. */
+static void scale_lp(enh_lp *P, int ruiz) {
+    const int m = P->m, n = P->n;
+    double *rs = (double *)calloc((size_t)m + 1, sizeof(double));
+    double *cs = (double *)calloc((size_t)n + 1, sizeof(double));
+    for (int i = 0; i < m; ++i) P->dr[i] = 1.0;
+    for (int j = 0; j < n; ++j) P->dc[j] = 1.0;
+    for (int pass = 0; pass <= ruiz; ++pass) {
+        const int l1 = (pass == ruiz);
+        memset(rs, 0, sizeof(double) * (size_t)m);
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = P->cp[j]; k < P->cp[j + 1]; ++k) {
+                const int i = P->ri[k];
+                const double a = fabs(P->v[k]) * P->dr[i] * P->dc[j];
+                if (l1) { acc += a; rs[i] += a; }
+                else { if (a > acc) acc = a; if (a > rs[i]) rs[i] = a; }
+            }
+            cs[j] = acc;
+        }
+        for (int i = 0; i < m; ++i) if (rs[i] > 0.0) P->dr[i] /= sqrt(rs[i]);
+        for (int j = 0; j < n; ++j) if (cs[j] > 0.0) P->dc[j] /= sqrt(cs[j]);
+    }
+    for (int j = 0; j < n; ++j)
+        for (int k = P->cp[j]; k < P->cp[j + 1]; ++k) {
+            const float s = (float)(P->dr[P->ri[k]] * P->v[k] * P->dc[j]);
+            P->cv_s[k] = s;
+            P->rv_s[P->c2r[k]] = s;
+        }
+    free(rs);
+    free(cs);
+}
+
+/* sequential FP32 row sums, as the tile kernels compute them */
+static void spmv32(int rows, const int *ptr, const int *idx, const float *v, const float *x,
+                   float *out) {
+    for (int r = 0; r < rows; ++r) {
+        float s = 0.0f;
+        for (int k = ptr[r]; k < ptr[r + 1]; ++k) s = fmaf(v[k], x[idx[k]], s);
+        out[r] = s;
+    }
+}
+
+/* Power iteration on A^T A for sigma_max of the scaled matrix. */
+static double power_sigma(const enh_lp *P, int iters) {
+    const int m = P->m, n = P->n;
+    float *v = (float *)xmalloc(sizeof(float) * (size_t)n);
+    float *w = (float *)xmalloc(sizeof(float) * (size_t)m);
+    float *z = (float *)xmalloc(sizeof(float) * (size_t)n);
+    double nv = 0.0, s = 0.0;
+    for (int j = 0; j < n; ++j) {
+        v[j] = 1.0f + 0.5f * (float)((j * 2654435761u) % 1000u) / 1000.0f;
+        nv += (double)v[j] * v[j];
+    }
+    for (int it = 0; it < iters && nv > 0.0; ++it) {
+        const float inv = (float)(1.0 / sqrt(nv));
+        for (int j = 0; j < n; ++j) v[j] *= inv;
+        spmv32(m, P->rp, P->ci, P->rv_s, v, w);
+        spmv32(n, P->cp, P->ri, P->cv_s, w, z);
+        double nz = 0.0;
+        for (int j = 0; j < n; ++j) nz += (double)z[j] * z[j];
+        s = sqrt(sqrt(nz)); /* ||A^T A v|| with ||v|| = 1 -> sigma^2 */
+        memcpy(v, z, sizeof(float) * (size_t)n);
+        nv = nz;
+    }
+    free(v); free(w); free(z);
+    return s;
+}
+
+/* FP64 relative KKT of (x, y) on the original LP; also A x and lambda. */
+static void kkt_full(const enh_lp *P, const double *c, const double *l, const double *u,
+                     const double *rl, const double *ru, const double *x, const double *y,
+                     double *ax, double *lam, double *out) {
+    const int m = P->m, n = P->n;
+    memset(ax, 0, sizeof(double) * (size_t)m);
+    for (int j = 0; j < n; ++j)
+        for (int k = P->cp[j]; k < P->cp[j + 1]; ++k) ax[P->ri[k]] += P->v[k] * x[j];
+    for (int j = 0; j < n; ++j) {
+        double s = c[j];
+        for (int k = P->cp[j]; k < P->cp[j + 1]; ++k) s += P->v[k] * y[P->ri[k]];
+        lam[j] = s;
+    }
+    orc_kkt(m, n, P->cp, P->ri, P->v, c, l, u, rl, ru, x, y, out);
+}
+
+static double finite_or_zero(float a, float b) {
+    return isfinite(a) ? (double)a : (isfinite(b) ? (double)b : 0.0);
+}
+
+int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const double *vals,
+                 const double *c, const double *l, const double *u, const double *rl,
+                 const double *ru, const orc_enh_cfg *cfg, double *x_out, double *y_out,
+                 orc_enh_out *out) {
+    memset(out, 0, sizeof(*out));
+    out->status = 8;
+    if (m == 0 || n == 0) { out->status = 0; return 0; }
+    enh_lp P;
+    const int nnz = col_ptr[n];
+    P.m = m; P.n = n; P.cp = col_ptr; P.ri = row_ind; P.v = vals;
+    P.rp = (int *)calloc((size_t)m + 1, sizeof(int));
+    P.ci = (int *)xmalloc(sizeof(int) * (size_t)nnz);
+    P.c2r = (int *)xmalloc(sizeof(int) * (size_t)nnz);
+    for (int k = 0; k < nnz; ++k) P.rp[row_ind[k] + 1]++;
+    for (int i = 0; i < m; ++i) P.rp[i + 1] += P.rp[i];
+    {
+        int *f = (int *)xmalloc(sizeof(int) * (size_t)(m + 1));
+        memcpy(f, P.rp, sizeof(int) * (size_t)(m + 1));
+        for (int j = 0; j < n; ++j)
+            for (int k = col_ptr[j]; k < col_ptr[j + 1]; ++k) {
+                const int d = f[row_ind[k]]++;
+                P.ci[d] = j;
+                P.c2r[k] = d;
+            }
+        free(f);
+    }
+    P.dr = (double *)xmalloc(sizeof(double) * (size_t)m);
+    P.dc = (double *)xmalloc(sizeof(double) * (size_t)n);
+    P.cv_s = (float *)xmalloc(sizeof(float) * (size_t)nnz);
+    P.rv_s = (float *)xmalloc(sizeof(float) * (size_t)nnz);
+    scale_lp(&P, cfg->ruiz_iters);
+
+    /* round data (scaled, FP32): costs, bounds, dual offset */
+    float *cs = (float *)xmalloc(sizeof(float) * (size_t)n), *ls = (float *)xmalloc(sizeof(float) * (size_t)n);
+    float *us = (float *)xmalloc(sizeof(float) * (size_t)n), *rls = (float *)xmalloc(sizeof(float) * (size_t)m);
+    float *rus = (float *)xmalloc(sizeof(float) * (size_t)m), *ybar = (float *)calloc((size_t)m, sizeof(float));
+    for (int j = 0; j < n; ++j) {
+        cs[j] = (float)(c[j] * P.dc[j]);
+        ls[j] = (float)(l[j] / P.dc[j]);
+        us[j] = (float)(u[j] / P.dc[j]);
+    }
+    for (int i = 0; i < m; ++i) {
+        rls[i] = (float)(rl[i] * P.dr[i]);
+        rus[i] = (float)(ru[i] * P.dr[i]);
+    }
+    /* FP64 base point of the refinement and its scales */
+    double *xbase = (double *)calloc((size_t)n, sizeof(double));
+    double *ybase = (double *)calloc((size_t)m, sizeof(double));
+    double inv_ap = 1.0, inv_ad = 1.0, alpha_p = 1.0, alpha_d = 1.0;
+
+    const double sig = power_sigma(&P, 64);
+    const double eta = (cfg->step > 0.0) ? cfg->step : 0.998 / (sig > 1e-12 ? sig : 1.0);
+    double cn = 0.0, bn = 0.0;
+    for (int j = 0; j < n; ++j) cn += (double)cs[j] * cs[j];
+    for (int i = 0; i < m; ++i) { double b = finite_or_zero(rus[i], rls[i]); bn += b * b; }
+    cn = sqrt(cn);
+    bn = sqrt(bn);
+    double omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+
+    float *x = (float *)xmalloc(sizeof(float) * (size_t)n), *x0 = (float *)xmalloc(sizeof(float) * (size_t)n);
+    float *xp = (float *)xmalloc(sizeof(float) * (size_t)n), *xb = (float *)xmalloc(sizeof(float) * (size_t)n);
+    float *y = (float *)calloc((size_t)m, sizeof(float)), *y0 = (float *)calloc((size_t)m, sizeof(float));
+    float *yp = (float *)calloc((size_t)m, sizeof(float));
+    float *g = (float *)xmalloc(sizeof(float) * (size_t)n), *h = (float *)xmalloc(sizeof(float) * (size_t)m);
+    double *xc = (double *)xmalloc(sizeof(double) * (size_t)n), *yc_ = (double *)xmalloc(sizeof(double) * (size_t)m);
+    double *ax = (double *)xmalloc(sizeof(double) * (size_t)m), *lam = (double *)xmalloc(sizeof(double) * (size_t)n);
+    double *xbest = (double *)xmalloc(sizeof(double) * (size_t)n), *ybest = (double *)calloc((size_t)m, sizeof(double));
+    double *axbest = (double *)xmalloc(sizeof(double) * (size_t)m), *lambest = (double *)xmalloc(sizeof(double) * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        float v = 0.0f;
+        if (v < ls[j]) v = ls[j];
+        if (v > us[j]) v = us[j];
+        x[j] = x0[j] = xp[j] = v;
+    }
+    const float rho = (float)cfg->reflection, rho1 = 1.0f + rho;
+    const int F = cfg->check_frequency;
+    int last_restart = 0, round_start = 0, restarts = 0, restart_next = 0, rounds = 0;
+    int since_best = 0;
+    double r_anchor = -1.0, r_prev = -1.0;
+    double best[10] = {0}, kk[10] = {0};
+    double best_kkt = INFINITY, round_start_kkt = INFINITY, round_best = INFINITY;
+    int it, status = 4, have_best = 0;
+    for (it = 0; it < cfg->max_iterations; ++it) {
+        if (restart_next) { /* take z+ as iterate and anchor */
+            memcpy(x, xp, sizeof(float) * (size_t)n);
+            memcpy(x0, xp, sizeof(float) * (size_t)n);
+            memcpy(y, yp, sizeof(float) * (size_t)m);
+            memcpy(y0, yp, sizeof(float) * (size_t)m);
+            restart_next = 0;
+        }
+        const float tau = (float)(eta / omega), sigma = (float)(eta * omega);
+        const float inv_sigma = (float)(1.0 / (eta * omega));
+        const int k = it - last_restart;
+        const float a = (float)(k + 1) / (float)(k + 2), b = 1.0f / (float)(k + 2);
+        const int check = ((it + 1) % F == 0) || (it + 1 == cfg->max_iterations);
+        double dx2 = 0.0, dxa2 = 0.0, dy2 = 0.0, dya2 = 0.0;
+        spmv32(n, P.cp, P.ri, P.cv_s, y, g);
+        for (int j = 0; j < n; ++j) {
+            const float xo = x[j];
+            const float t = cs[j] + g[j];
+            float v = fmaf(-tau, t, xo);
+            if (v < ls[j]) v = ls[j];
+            if (v > us[j]) v = us[j];
+            xb[j] = 2.0f * v - xo;
+            const float r = fmaf(rho1, v, -(rho * xo));
+            x[j] = fmaf(a, r, b * x0[j]);
+            if (check) {
+                xp[j] = v;
+                const double d = (double)xo - (double)v, e = (double)v - (double)x0[j];
+                dx2 += d * d;
+                dxa2 += e * e;
+            }
+        }
+        spmv32(m, P.rp, P.ci, P.rv_s, xb, h);
+        for (int i = 0; i < m; ++i) {
+            /* y+ = sigma (w - clamp(w, rl, ru)) - ybar with w = h + (y + ybar) / sigma,
+             * evaluated as y + sigma (h - bound) on an active bound (exact
+             * subtraction near the bound) and -ybar in between. */
+            const float yo = y[i];
+            const float w = fmaf(yo + ybar[i], inv_sigma, h[i]);
+            float yn = -ybar[i];
+            if (w < rls[i]) yn = fmaf(sigma, h[i] - rls[i], yo);
+            else if (w > rus[i]) yn = fmaf(sigma, h[i] - rus[i], yo);
+            const float r = fmaf(rho1, yn, -(rho * yo));
+            y[i] = fmaf(a, r, b * y0[i]);
+            if (check) {
+                yp[i] = yn;
+                const double d = (double)yo - (double)yn, e = (double)yn - (double)y0[i];
+                dy2 += d * d;
+                dya2 += e * e;
+            }
+        }
+        if (!check) continue;
+        /* ---- check: FP64 KKT of the unscaled T-output ---- */
+        for (int j = 0; j < n; ++j) xc[j] = xbase[j] + P.dc[j] * (double)xp[j] * inv_ap;
+        for (int i = 0; i < m; ++i) yc_[i] = ybase[i] + P.dr[i] * (double)yp[i] * inv_ad;
+        kkt_full(&P, c, l, u, rl, ru, xc, yc_, ax, lam, kk);
+        const int done = it + 1;
+        if (isnan(kk[3])) { status = 6; it = done; break; } /* NumericalFailure */
+        if (!(kk[3] >= best_kkt)) { /* improvement (NaN never counts) */
+            best_kkt = kk[3];
+            memcpy(best, kk, sizeof(kk));
+            memcpy(xbest, xc, sizeof(double) * (size_t)n);
+            memcpy(ybest, yc_, sizeof(double) * (size_t)m);
+            memcpy(axbest, ax, sizeof(double) * (size_t)m);
+            memcpy(lambest, lam, sizeof(double) * (size_t)n);
+            have_best = 1;
+        }
+        if (kk[3] <= cfg->tol) { status = 0; it = done; break; }
+        if (done >= cfg->max_iterations) { it = done; break; }
+        if (kk[3] < round_best) { round_best = kk[3]; since_best = 0; } else { since_best++; }
+        if (round_start_kkt == INFINITY) round_start_kkt = kk[3]; /* first check of the solve */
+        if (rounds < cfg->max_refinements && have_best &&
+            (round_best <= cfg->refine_ratio * round_start_kkt || since_best >= cfg->patience)) {
+            /* ---- refinement: residual LP around the best point ---- */
+            const double dp = fmax(best[0], best[2]), dd = fmax(best[1], best[2]);
+            double ap = fmin(1.0 / fmax(dp, 1e-300), 1e12), ad = fmin(1.0 / fmax(dd, 1e-300), 1e12);
+            if (ap < 1.0) ap = 1.0;
+            if (ad < 1.0) ad = 1.0;
+            omega *= (ad / ap) / (alpha_d / alpha_p);
+            alpha_p = ap; alpha_d = ad; inv_ap = 1.0 / ap; inv_ad = 1.0 / ad;
+            memcpy(xbase, xbest, sizeof(double) * (size_t)n);
+            memcpy(ybase, ybest, sizeof(double) * (size_t)m);
+            for (int j = 0; j < n; ++j) {
+                cs[j] = (float)(P.dc[j] * (ad * lambest[j]));
+                ls[j] = (float)(ap * (l[j] - xbase[j]) / P.dc[j]);
+                us[j] = (float)(ap * (u[j] - xbase[j]) / P.dc[j]);
+                float v = 0.0f;
+                if (v < ls[j]) v = ls[j];
+                if (v > us[j]) v = us[j];
+                x[j] = x0[j] = xp[j] = v;
+            }
+            for (int i = 0; i < m; ++i) {
+                rls[i] = (float)(ap * (rl[i] - axbest[i]) * P.dr[i]);
+                rus[i] = (float)(ap * (ru[i] - axbest[i]) * P.dr[i]);
+                ybar[i] = (float)(ad * ybase[i] / P.dr[i]);
+                y[i] = y0[i] = yp[i] = 0.0f;
+            }
+            rounds++;
+            last_restart = round_start = done;
+            round_start_kkt = best_kkt;
+            round_best = INFINITY;
+            since_best = 0;
+            r_anchor = r_prev = -1.0;
+            continue;
+        }
+        const double r = sqrt(omega * dx2 + dy2 / omega);
+        const int kc = done - last_restart;
+        if (r_anchor < 0.0) { r_anchor = r_prev = r; continue; }
+        const int rs = (r <= 0.2 * r_anchor) || (r <= 0.8 * r_anchor && r > r_prev) ||
+                       ((double)kc >= 0.36 * (double)(done - round_start));
+        r_prev = r;
+        if (rs) {
+            const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
+            if (ddx > 1e-10 && ddy > 1e-10) omega = sqrt(omega * ddy / ddx);
+            last_restart = done;
+            restart_next = 1;
+            restarts++;
+            r_anchor = r;
+        }
+    }
+    if (status != 0 && !have_best) {
+        for (int j = 0; j < n; ++j) xbest[j] = xbase[j] + P.dc[j] * (double)xp[j] * inv_ap;
+        for (int i = 0; i < m; ++i) ybest[i] = ybase[i] + P.dr[i] * (double)yp[i] * inv_ad;
+        kkt_full(&P, c, l, u, rl, ru, xbest, ybest, axbest, lambest, best);
+    }
+    out->status = status;
+    out->iterations = it;
+    out->restarts = restarts;
+    out->refinements = rounds;
+    out->eps_p = best[0]; out->eps_d = best[1]; out->eps_gap = best[2]; out->kkt = best[3];
+    out->primal_obj = best[4]; out->dual_obj = best[5];
+    out->step = eta;
+    if (x_out) memcpy(x_out, xbest, sizeof(double) * (size_t)n);
+    if (y_out) memcpy(y_out, ybest, sizeof(double) * (size_t)m);
+    free(P.rp); free(P.ci); free(P.c2r); free(P.dr); free(P.dc); free(P.cv_s); free(P.rv_s);
+    free(cs); free(ls); free(us); free(rls); free(rus); free(ybar); free(xbase); free(ybase);
+    free(x); free(x0); free(xp); free(xb); free(y); free(y0); free(yp); free(g); free(h);
+    free(xc); free(yc_); free(ax); free(lam); free(xbest); free(ybest); free(axbest); free(lambest);
+    return 0;
+}

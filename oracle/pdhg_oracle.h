@@ -97,9 +97,13 @@ typedef struct orc_enh_cfg {
     double reflection;     /* rho in [0,1] */
     int ruiz_iters;
     int threads;           /* OpenMP threads (1 = scalar) */
+    double step;           /* eta; <= 0: 0.998 / sigma_max by power iteration */
+    int max_refinements;   /* outer FP64 refinement rounds */
+    double refine_ratio;   /* refine once the round's best KKT <= ratio * its start */
+    int patience;          /* ... or after this many checks without a new round best */
 } orc_enh_cfg;
 typedef struct orc_enh_out {
-    int status, iterations, restarts;
+    int status, iterations, restarts, refinements;
     double eps_p, eps_d, eps_gap, kkt, primal_obj, dual_obj, step;
 } orc_enh_out;
 int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const double *vals,

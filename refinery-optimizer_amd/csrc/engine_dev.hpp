@@ -36,23 +36,56 @@ struct RefInst {
     int32_t pad;
 };
 
-// Enhanced-mode per-instance state.
+// Per-tile partial reductions written by the enhanced-mode kernels.
+// Column (CSC) tiles:
+enum : int {
+    kPcDx2 = 0,     // sum (x - x+)^2            (primal step, check iterations)
+    kPcDxa2 = 1,    // sum (x+ - x0)^2           (distance to the anchor)
+    kPcDualViol = 2,  // max |lambda - proj(lambda)|
+    kPcColViol = 3,   // max bound violation of x
+    kPcPobj = 4,      // sum c_j x_j
+    kPcDobj = 5,      // sum l_j lambda_j^+ - u_j lambda_j^-
+    kPcPow = 6,       // power iteration: sum z_j^2
+    kPcCount = 8
+};
+// Row (CSR) tiles:
+enum : int {
+    kPrDy2 = 0,      // sum (y - y+)^2
+    kPrDya2 = 1,     // sum (y+ - y0)^2
+    kPrRowViol = 2,  // max row-activity violation
+    kPrSignViol = 3, // max row-dual sign violation
+    kPrDobj = 4,     // - sum (ru_i y_i^+ - rl_i y_i^-)
+    kPrCount = 8
+};
+
+// Enhanced-mode per-instance state (device resident; written by the
+// per-instance finalize kernel, read by every tile of the instance).
 struct EnhInst {
-    int32_t status;
-    int32_t iterations;       // iterations done (total, this solve)
+    int32_t status;          // kRunning, then model::SolutionStatus
+    int32_t iterations;      // iterations done when stopped
     int32_t restarts;
     int32_t checks;
-    int32_t last_restart;     // iteration index of the last restart
-    int32_t restart_at;       // iteration at which K1/K2 take z+ as iterate and anchor
+    int32_t last_restart;    // iteration count at the last restart (Halpern k origin)
+    int32_t restart_at;      // iteration index that takes z+ as iterate and anchor
+    int32_t round_start;     // iteration count at the start of the refinement round
+    int32_t rounds;          // refinement rounds done
+    int32_t since_best;      // checks without a new best KKT in this round
+    int32_t copy_best;       // flag: this check's point becomes the best point
+    int32_t refine;          // flag: build the residual LP now
+    int32_t max_iter;
     int32_t row_tile_begin, row_tile_end;
     int32_t col_tile_begin, col_tile_end;
-    float tau, sigma;         // eta / omega, eta * omega
+    float tau, sigma, inv_sigma, pad0;
     double eta, omega;
-    double r_anchor, r_prev;  // fixed-point residual at last restart / previous check
-    double bnorm_inf, cnorm_inf;
-    double eps_p, eps_d, eps_gap, kkt, pobj, dobj;
+    double r_anchor, r_prev;   // fixed-point residuals (restart criteria)
+    double bnorm, cnorm;       // |b|_inf, |c|_inf of the original LP
     double tol;
-    double best_kkt;
+    double alpha_p, alpha_d;   // refinement scales of the current round
+    double inv_ap, inv_ad;
+    double round_start_kkt, round_best;
+    double best_kkt, best_eps_p, best_eps_d, best_eps_gap, best_pobj, best_dobj;
+    double eps_p, eps_d, eps_gap, kkt, pobj, dobj;  // last check
+    double pow_norm2;          // power iteration: |A^T A v|^2
 };
 
 }  // namespace mpdhg

@@ -1,10 +1,432 @@
-// Enhanced-mode driver (restarted reflected-Halpern PDHG with FP64 checks).
+// Enhanced-mode driver: per-instance scaling on the host, then a fully
+// device-resident loop.  Each check cycle is F fused primal/dual launch
+// pairs followed by the FP64 check launches; restart, primal-weight and
+// refinement decisions are taken on the device by the per-instance finalize
+// kernel, so the host only polls a status word per instance, one cycle
+// behind the device.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <thread>
+
 #include "engine.hpp"
 
 namespace mpdhg {
 
-void solve_enhanced(Context &, const mpdhg_lp *, int, const mpdhg_config &, mpdhg_result *) {
-    throw Error(MPDHG_E_STATE, "enhanced mode is not available in this build");
+namespace {
+
+struct EnhHost {
+    std::vector<int> cptr, cidx, rptr, ridx;
+    std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
+    std::vector<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
+    std::vector<Tile> rtiles, ctiles;
+    std::vector<EnhInst> inst;
+};
+
+double finite_or_zero(float a, float b) {
+    return std::isfinite(a) ? static_cast<double>(a) : (std::isfinite(b) ? static_cast<double>(b) : 0.0);
+}
+
+// Ruiz equilibration (infinity norm) then one Pock-Chambolle pass (alpha = 1,
+// l1 norms); the scaled matrix is D_r A D_c with || . ||_2 <= 1.
+void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, double *dc) {
+    const int m = h.m, n = h.n;
+    std::vector<double> rs(static_cast<size_t>(m)), cs(static_cast<size_t>(n));
+    for (int i = 0; i < m; ++i) dr[i] = 1.0;
+    for (int j = 0; j < n; ++j) dc[j] = 1.0;
+    for (int pass = 0; pass <= ruiz; ++pass) {
+        const bool l1 = (pass == ruiz);
+        std::fill(rs.begin(), rs.end(), 0.0);
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) {
+                const int i = h.cidx[static_cast<size_t>(k)];
+                const double a = std::fabs(lp.vals[k]) * dr[i] * dc[j];
+                if (l1) {
+                    acc += a;
+                    rs[static_cast<size_t>(i)] += a;
+                } else {
+                    if (a > acc) acc = a;
+                    if (a > rs[static_cast<size_t>(i)]) rs[static_cast<size_t>(i)] = a;
+                }
+            }
+            cs[static_cast<size_t>(j)] = acc;
+        }
+        for (int i = 0; i < m; ++i)
+            if (rs[static_cast<size_t>(i)] > 0.0) dr[i] /= std::sqrt(rs[static_cast<size_t>(i)]);
+        for (int j = 0; j < n; ++j)
+            if (cs[static_cast<size_t>(j)] > 0.0) dc[j] /= std::sqrt(cs[static_cast<size_t>(j)]);
+    }
+}
+
+template <typename F>
+void parallel_for(int count, F &&f) {
+    const int nt = std::max(1, std::min<int>(count, static_cast<int>(std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        for (int i = 0; i < count; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::atomic<int> next{0};
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            for (int i = next++; i < count; i = next++) f(i);
+        });
+    for (auto &th : pool) th.join();
+}
+
+struct Pinned {
+    void *p = nullptr;
+    explicit Pinned(size_t bytes) { MPDHG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
+    ~Pinned() { if (p) (void)hipHostFree(p); }
+    Pinned(const Pinned &) = delete;
+    Pinned &operator=(const Pinned &) = delete;
+};
+
+}  // namespace
+
+void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
+                    mpdhg_result *res) {
+    const double t_call = now_sec();
+    const int F = cfg.check_frequency, K = cfg.max_iterations;
+    const double tol = cfg.tol_kkt > 0.0 ? cfg.tol_kkt
+                                         : std::max<double>(cfg.tol_primal, std::max(cfg.tol_dual, cfg.tol_gap));
+    if (cfg.reflection < 0.0f || cfg.reflection > 1.0f) throw Error(MPDHG_E_ARG, "reflection must be in [0, 1]");
+    if (cfg.ruiz_iterations < 0) throw Error(MPDHG_E_ARG, "ruiz_iterations must be >= 0");
+    std::vector<HostLp> H(static_cast<size_t>(count));
+    std::vector<int> act;
+    for (int q = 0; q < count; ++q) {
+        HostLp &h = H[static_cast<size_t>(q)];
+        prepare_structure(lps[q], h);
+        mpdhg_result &r = res[q];
+        if (h.m == 0 || h.n == 0) {
+            r.status = MPDHG_STATUS_OPTIMAL;
+        } else if (K <= 0) {  // no iteration: report the projected origin
+            std::vector<double> x(static_cast<size_t>(h.n)), y(static_cast<size_t>(h.m), 0.0);
+            for (int j = 0; j < h.n; ++j)
+                x[static_cast<size_t>(j)] = std::max(lps[q].col_lower[j], std::min(lps[q].col_upper[j], 0.0));
+            host_kkt(lps[q], h, x, y, r);
+            double obj = 0;
+            for (int j = 0; j < h.n; ++j) obj += lps[q].c[j] * x[static_cast<size_t>(j)];
+            r.primal_objective = obj;
+            if (r.x) std::memcpy(r.x, x.data(), x.size() * sizeof(double));
+            if (r.y) std::fill(r.y, r.y + h.m, 0.0);
+            r.status = MPDHG_STATUS_ITERATION_LIMIT;
+        } else {
+            act.push_back(q);
+        }
+    }
+    const int na = static_cast<int>(act.size());
+    if (na == 0) {
+        for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;
+        return;
+    }
+    BatchLayout L;
+    L.row_off.resize(static_cast<size_t>(na));
+    L.col_off.resize(static_cast<size_t>(na));
+    L.nnz_off.resize(static_cast<size_t>(na));
+    for (int a = 0; a < na; ++a) {
+        const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+        L.row_off[static_cast<size_t>(a)] = L.rows;
+        L.col_off[static_cast<size_t>(a)] = L.cols;
+        L.nnz_off[static_cast<size_t>(a)] = L.nnz;
+        L.rows += h.m;
+        L.cols += h.n;
+        L.nnz += h.nnz;
+    }
+    const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
+    if (L.rows > lim || L.cols > lim || L.nnz > lim)
+        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
+                 Z = static_cast<size_t>(L.nnz);
+    EnhHost hh;
+    hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.ridx.resize(Z);
+    hh.cval.resize(Z); hh.rval.resize(Z); hh.cval64.resize(Z); hh.rval64.resize(Z);
+    hh.cs.resize(C); hh.ls.resize(C); hh.us.resize(C); hh.x0.resize(C); hh.powv.resize(C);
+    hh.c64.resize(C); hh.l64.resize(C); hh.u64.resize(C); hh.dc.resize(C);
+    hh.rls.resize(R); hh.rus.resize(R); hh.rl64.resize(R); hh.ru64.resize(R); hh.dr.resize(R);
+    hh.inst.resize(static_cast<size_t>(na));
+    std::vector<double> pow0(static_cast<size_t>(na));
+
+    // ---- per-instance host preparation (independent, in parallel) ----
+    parallel_for(na, [&](int a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        const mpdhg_lp &lp = lps[q];
+        const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)],
+                      zo = L.nnz_off[static_cast<size_t>(a)];
+        double *dr = hh.dr.data() + ro, *dc = hh.dc.data() + co;
+        scale_instance(lp, h, cfg.ruiz_iterations, dr, dc);
+        for (int j = 0; j < h.n; ++j) hh.cptr[static_cast<size_t>(co + j)] = static_cast<int>(zo + h.cptr[j]);
+        for (int i = 0; i < h.m; ++i) hh.rptr[static_cast<size_t>(ro + i)] = static_cast<int>(zo + h.rptr[i]);
+        for (int j = 0; j < h.n; ++j)
+            for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) {
+                const int i = h.cidx[static_cast<size_t>(k)];
+                const float s = static_cast<float>(dr[i] * lp.vals[k] * dc[j]);
+                const int d = h.csc2csr[static_cast<size_t>(k)];
+                const size_t gk = static_cast<size_t>(zo + k), gd = static_cast<size_t>(zo + d);
+                hh.cval[gk] = s;
+                hh.cval64[gk] = lp.vals[k];
+                hh.cidx[gk] = static_cast<int>(ro + i);
+                hh.rval[gd] = s;
+                hh.rval64[gd] = lp.vals[k];
+                hh.ridx[gd] = static_cast<int>(co + j);
+            }
+        double cn2 = 0.0, bn2 = 0.0, cinf = 0.0, binf = 0.0, pn = 0.0;
+        for (int j = 0; j < h.n; ++j) {
+            const size_t g = static_cast<size_t>(co + j);
+            hh.c64[g] = lp.c[j];
+            hh.l64[g] = lp.col_lower[j];
+            hh.u64[g] = lp.col_upper[j];
+            hh.cs[g] = static_cast<float>(lp.c[j] * dc[j]);
+            hh.ls[g] = static_cast<float>(lp.col_lower[j] / dc[j]);
+            hh.us[g] = static_cast<float>(lp.col_upper[j] / dc[j]);
+            float v = 0.0f;
+            if (v < hh.ls[g]) v = hh.ls[g];
+            if (v > hh.us[g]) v = hh.us[g];
+            hh.x0[g] = v;
+            cn2 += static_cast<double>(hh.cs[g]) * hh.cs[g];
+            cinf = std::max(cinf, std::fabs(lp.c[j]));
+            const float pv = 1.0f + 0.5f * static_cast<float>((static_cast<uint32_t>(j) * 2654435761u) % 1000u) / 1000.0f;
+            hh.powv[g] = pv;
+            pn += static_cast<double>(pv) * pv;
+        }
+        for (int i = 0; i < h.m; ++i) {
+            const size_t g = static_cast<size_t>(ro + i);
+            hh.rl64[g] = lp.row_lower[i];
+            hh.ru64[g] = lp.row_upper[i];
+            hh.rls[g] = static_cast<float>(lp.row_lower[i] * dr[i]);
+            hh.rus[g] = static_cast<float>(lp.row_upper[i] * dr[i]);
+            const double b = finite_or_zero(hh.rus[g], hh.rls[g]);
+            bn2 += b * b;
+            if (std::isfinite(lp.row_lower[i])) binf = std::max(binf, std::fabs(lp.row_lower[i]));
+            if (std::isfinite(lp.row_upper[i])) binf = std::max(binf, std::fabs(lp.row_upper[i]));
+        }
+        const double cn = std::sqrt(cn2), bn = std::sqrt(bn2);
+        EnhInst &s = hh.inst[static_cast<size_t>(a)];
+        std::memset(&s, 0, sizeof(s));
+        s.status = kRunning;
+        s.restart_at = -1;
+        s.max_iter = K;
+        s.omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+        s.r_anchor = s.r_prev = -1.0;
+        s.bnorm = binf;
+        s.cnorm = cinf;
+        s.tol = tol;
+        s.alpha_p = s.alpha_d = s.inv_ap = s.inv_ad = 1.0;
+        s.round_start_kkt = s.round_best = s.best_kkt = HUGE_VAL;
+        s.pow_norm2 = pn;
+        pow0[static_cast<size_t>(a)] = pn;
+    });
+    hh.cptr[C] = static_cast<int>(Z);
+    hh.rptr[R] = static_cast<int>(Z);
+    for (int a = 0; a < na; ++a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        EnhInst &s = hh.inst[static_cast<size_t>(a)];
+        s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
+        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, hh.rtiles);
+        s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
+        s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
+        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, hh.ctiles);
+        s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
+    }
+
+    // ---- device buffers ----
+    hipStream_t st = ctx.stream;
+    DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
+    DBuf<float> d_cval, d_rval, d_cs, d_ls, d_us, d_rls, d_rus;
+    DBuf<double> d_cval64, d_rval64, d_c64, d_l64, d_u64, d_dc, d_rl64, d_ru64, d_dr;
+    d_cptr.upload(hh.cptr, st); d_cidx.upload(hh.cidx, st); d_rptr.upload(hh.rptr, st);
+    d_ridx.upload(hh.ridx, st); d_cval.upload(hh.cval, st); d_rval.upload(hh.rval, st);
+    d_cval64.upload(hh.cval64, st); d_rval64.upload(hh.rval64, st);
+    d_cs.upload(hh.cs, st); d_ls.upload(hh.ls, st); d_us.upload(hh.us, st);
+    d_rls.upload(hh.rls, st); d_rus.upload(hh.rus, st);
+    d_c64.upload(hh.c64, st); d_l64.upload(hh.l64, st); d_u64.upload(hh.u64, st);
+    d_dc.upload(hh.dc, st); d_rl64.upload(hh.rl64, st); d_ru64.upload(hh.ru64, st);
+    d_dr.upload(hh.dr, st);
+    DBuf<float> d_x(C), d_x0(C), d_xp(C), d_xbar(C), d_y(R), d_y0(R), d_yp(R), d_ybar(R);
+    DBuf<double> d_xbase(C), d_xchk(C), d_lam(C), d_xbest(C), d_lambest(C);
+    DBuf<double> d_ybase(R), d_ychk(R), d_ax(R), d_ybest(R), d_axbest(R);
+    DBuf<Tile> d_rt, d_ct;
+    DBuf<EnhInst> d_inst;
+    d_rt.upload(hh.rtiles, st);
+    d_ct.upload(hh.ctiles, st);
+    d_inst.upload(hh.inst, st);
+    DBuf<double> d_pc(hh.ctiles.size() * kPcCount), d_pr(hh.rtiles.size() * kPrCount);
+    DBuf<int32_t> d_status(static_cast<size_t>(na));
+    MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemcpyAsync(d_x0.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemcpyAsync(d_xp.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    for (float *p : {d_y.get(), d_y0.get(), d_yp.get(), d_ybar.get()})
+        MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(float), st));
+    for (double *p : {d_xbase.get(), d_xbest.get(), d_lambest.get()})
+        MPDHG_HIP(hipMemsetAsync(p, 0, C * sizeof(double), st));
+    for (double *p : {d_ybase.get(), d_ybest.get(), d_axbest.get()})
+        MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(double), st));
+    {
+        std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
+        d_status.upload(run, st);
+    }
+    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
+    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
+    // power iteration vectors reuse the iterate buffers before the solve
+    MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipStreamSynchronize(st));
+    const double t_loop = now_sec();
+
+    // ---- step size: power iteration on (D_r A D_c)^T (D_r A D_c) ----
+    for (int p = 0; p < 64; ++p) {
+        launch::pow_scale(st, cts, d_inst.get(), d_xbar.get(), d_xp.get());
+        launch::pow_rows(st, rts, d_rptr.get(), d_ridx.get(), d_rval.get(), d_xp.get(), d_yp.get());
+        launch::pow_cols(st, cts, d_cptr.get(), d_cidx.get(), d_cval.get(), d_yp.get(), d_xbar.get(),
+                         d_pc.get());
+        launch::pow_fin(st, d_inst.get(), na, d_pc.get());
+    }
+    MPDHG_HIP(hipGetLastError());
+    std::vector<EnhInst> hs(static_cast<size_t>(na));
+    MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(EnhInst), hipMemcpyDeviceToHost, st));
+    MPDHG_HIP(hipStreamSynchronize(st));
+    for (int a = 0; a < na; ++a) {
+        EnhInst &s = hh.inst[static_cast<size_t>(a)];
+        const double sig = std::sqrt(std::sqrt(hs[static_cast<size_t>(a)].pow_norm2));
+        s.eta = 0.998 / (sig > 1e-12 ? sig : 1.0);
+        s.tau = static_cast<float>(s.eta / s.omega);
+        s.sigma = static_cast<float>(s.eta * s.omega);
+        s.inv_sigma = static_cast<float>(1.0 / (s.eta * s.omega));
+        s.pow_norm2 = hs[static_cast<size_t>(a)].pow_norm2;
+    }
+    d_inst.upload(hh.inst, st);
+    MPDHG_HIP(hipMemcpyAsync(d_xp.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemsetAsync(d_yp.get(), 0, R * sizeof(float), st));
+
+    // ---- iteration loop ----
+    const float rho = cfg.reflection;
+    launch::EnhPrimal P{d_cptr.get(), d_cidx.get(), d_cval.get(), d_y.get(), d_yp.get(), d_cs.get(),
+                        d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_xbar.get(),
+                        d_xbase.get(), d_dc.get(), d_xchk.get(), d_pc.get(), rho};
+    launch::EnhDual D{d_rptr.get(), d_ridx.get(), d_rval.get(), d_xbar.get(), d_rls.get(), d_rus.get(),
+                      d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_ybase.get(), d_dr.get(),
+                      d_ychk.get(), d_pr.get(), rho};
+    launch::EnhCheck CK{d_rptr.get(), d_ridx.get(), d_cptr.get(), d_cidx.get(), d_rval64.get(),
+                        d_cval64.get(), d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
+                        d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
+                        d_dr.get(), d_pr.get(), d_pc.get(), d_status.get(), cfg.refine_ratio,
+                        cfg.refine_patience, cfg.max_refinements, d_xbest.get(), d_lambest.get(),
+                        d_ybest.get(), d_axbest.get(), d_xbase.get(), d_ybase.get(), d_cs.get(),
+                        d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_rls.get(),
+                        d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get()};
+    const int nsamp = cfg.profile ? 256 : 0;
+    if (nsamp) {
+        while (ctx.events.size() < static_cast<size_t>(3 * nsamp + 4)) {
+            hipEvent_t e;
+            MPDHG_HIP(hipEventCreate(&e));
+            ctx.events.push_back(e);
+        }
+    }
+    const int stride = nsamp ? std::max(1, K / nsamp) : 1;
+    int used = 0;
+    Pinned pin(2 * static_cast<size_t>(na) * sizeof(int32_t));
+    int32_t *slots = static_cast<int32_t *>(pin.p);
+    hipEvent_t poll_ev[2];
+    MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
+    MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
+    int it = 0, cycle = 0;
+    bool finished = false;
+    try {
+        while (it < K && !finished) {
+            const int end = std::min(K, (it / F + 1) * F);
+            for (; it < end; ++it) {
+                const int check = (it + 1 == end);
+                const bool samp = nsamp && used < nsamp && (it % stride == 0) && !check;
+                if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
+                launch::enh_primal(st, cts, d_inst.get(), P, it, check);
+                if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
+                launch::enh_dual(st, rts, d_inst.get(), D, it, check);
+                if (samp) {
+                    MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
+                    ++used;
+                }
+            }
+            launch::enh_check(st, rts, cts, d_inst.get(), na, CK, end - 1);
+            MPDHG_HIP(hipGetLastError());
+            const int slot = cycle & 1;
+            MPDHG_HIP(hipMemcpyAsync(slots + slot * na, d_status.get(), static_cast<size_t>(na) * sizeof(int32_t),
+                                     hipMemcpyDeviceToHost, st));
+            MPDHG_HIP(hipEventRecord(poll_ev[slot], st));
+            if (cycle > 0) {  // inspect the previous cycle while this one runs
+                const int prev = slot ^ 1;
+                MPDHG_HIP(hipEventSynchronize(poll_ev[prev]));
+                finished = std::none_of(slots + prev * na, slots + prev * na + na,
+                                        [](int32_t v) { return v == kRunning; });
+            }
+            ++cycle;
+        }
+        MPDHG_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipEventDestroy(poll_ev[0]);
+        (void)hipEventDestroy(poll_ev[1]);
+        throw;
+    }
+    (void)hipEventDestroy(poll_ev[0]);
+    (void)hipEventDestroy(poll_ev[1]);
+    const double loop_sec = now_sec() - t_loop;
+    double ms_csc = 0, ms_csr = 0;
+    for (int s = 0; s < used; ++s) {
+        float a = 0, b = 0;
+        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(3 * s)], ctx.events[static_cast<size_t>(3 * s + 1)]));
+        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(3 * s + 1)], ctx.events[static_cast<size_t>(3 * s + 2)]));
+        ms_csc += a;
+        ms_csr += b;
+    }
+
+    // ---- results: the best point of each instance ----
+    std::vector<double> xb, yb, lb, ab;
+    d_xbest.download(xb, C, st);
+    d_ybest.download(yb, R, st);
+    d_lambest.download(lb, C, st);
+    d_axbest.download(ab, R, st);
+    MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(EnhInst), hipMemcpyDeviceToHost, st));
+    MPDHG_HIP(hipStreamSynchronize(st));
+    const double sg = cfg.dual_sign ? -1.0 : 1.0;
+    for (int a = 0; a < na; ++a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        mpdhg_result &r = res[q];
+        const EnhInst &s = hs[static_cast<size_t>(a)];
+        const size_t ro = static_cast<size_t>(L.row_off[static_cast<size_t>(a)]),
+                     co = static_cast<size_t>(L.col_off[static_cast<size_t>(a)]);
+        r.status = (s.status == kRunning) ? MPDHG_STATUS_ITERATION_LIMIT : s.status;
+        r.iterations = (s.status == kRunning) ? it : s.iterations;
+        r.restarts = s.restarts;
+        r.refinements = s.rounds;
+        r.checks = s.checks;
+        r.primal_objective = s.best_pobj;
+        r.dual_objective = s.best_dobj;
+        r.rel_primal = s.best_eps_p;
+        r.rel_dual = s.best_eps_d;
+        r.rel_gap = s.best_eps_gap;
+        r.rel_kkt = s.best_kkt;
+        r.step_size = s.eta;
+        r.primal_weight = s.omega;
+        if (r.x) std::memcpy(r.x, xb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
+        if (r.reduced_costs) std::memcpy(r.reduced_costs, lb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
+        if (r.row_activity) std::memcpy(r.row_activity, ab.data() + ro, static_cast<size_t>(h.m) * sizeof(double));
+        if (r.y)
+            for (int i = 0; i < h.m; ++i) r.y[i] = sg * yb[ro + static_cast<size_t>(i)];
+        r.solve_sec = loop_sec;
+        r.kernel_ms_spmv_csc = ms_csc;
+        r.kernel_ms_spmv_csr = ms_csr;
+        r.kernel_launches_csc = r.kernel_launches_csr = used;
+        // K1 streams CSC idx+val, gathers y once, reads c,l,u,x,x0 and writes x,xbar;
+        // K2 streams CSR idx+val, gathers xbar once, reads rl,ru,ybar,y,y0, writes y.
+        r.bytes_csc_launch = 8.0 * L.nnz + 4.0 * (L.cols + 1) + 4.0 * L.rows + 4.0 * 7 * L.cols;
+        r.bytes_csr_launch = 8.0 * L.nnz + 4.0 * (L.rows + 1) + 4.0 * L.cols + 4.0 * 6 * L.rows;
+    }
+    const double wall = now_sec() - t_call;
+    for (int q = 0; q < count; ++q) res[q].time_wall_sec = wall;
 }
 
 }  // namespace mpdhg

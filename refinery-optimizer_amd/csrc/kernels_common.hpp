@@ -17,27 +17,45 @@ __device__ __forceinline__ int xcd_tile(unsigned b, int chunk) {
     return static_cast<int>(b & 7u) * chunk + static_cast<int>(b >> 3);
 }
 
-template <typename A>
-__device__ __forceinline__ A fma_acc(float v, float x, A s);
+// s + v * x with one rounding, in the accumulator type A.
+template <typename A, typename V>
+__device__ __forceinline__ A fma_acc(V v, V x, A s);
 template <>
-__device__ __forceinline__ float fma_acc<float>(float v, float x, float s) {
+__device__ __forceinline__ float fma_acc<float, float>(float v, float x, float s) {
     return __builtin_fmaf(v, x, s);
 }
 template <>
-__device__ __forceinline__ double fma_acc<double>(float v, float x, double s) {
+__device__ __forceinline__ double fma_acc<double, float>(float v, float x, double s) {
     return __builtin_fma(static_cast<double>(v), static_cast<double>(x), s);
 }
+template <>
+__device__ __forceinline__ double fma_acc<double, double>(double v, double x, double s) {
+    return __builtin_fma(v, x, s);
+}
+
+template <typename V, typename X>
+struct Pair {
+    V v;
+    X x;
+};
+
+// LDS bytes every tile kernel reserves for the staged (value, x) pairs.
+constexpr int kStageBytes = kTileNnz * 8;
 
 // Row sums of one tile: sum_k val[k] * xs[idx[k]] for every row of the tile,
 // in storage order (k ascending), one row per lane.  The tile's nonzeros
-// are streamed in chunks of kTileNnz: coalesced loads of idx/val, the
-// gathers issued back to back, then (val, x) pairs parked in LDS; each lane
-// then walks its own row from LDS.  A row longer than a chunk is continued
-// across chunks by the same lane, so the summation order never changes.
-template <typename A>
-__device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
-                                      const int *__restrict__ idx, const float *__restrict__ val,
-                                      const float *__restrict__ xs, float2 *lds) {
+// are streamed in chunks: coalesced loads of idx/val, the gathers issued
+// back to back, then (val, x) pairs parked in LDS; each lane then walks its
+// own row from LDS.  A row longer than a chunk is continued across chunks
+// by the same lane, so the summation order never changes.  `lds` must hold
+// kStageBytes.
+template <typename V, typename X, typename A>
+__device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
+                                        const int *__restrict__ idx, const V *__restrict__ val,
+                                        const X *__restrict__ xs, void *lds_raw) {
+    constexpr int CH = kStageBytes / static_cast<int>(sizeof(Pair<V, X>));
+    constexpr int PT = CH / kBlock;
+    Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
     const int r = tl.begin + tid;
     int k0 = 0, k1 = 0;
@@ -47,38 +65,45 @@ __device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ pt
     }
     const int lo = ptr[tl.begin], hi = ptr[tl.end];
     A sum = A(0);
-    for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
-        const int c1 = min(hi, c0 + kTileNnz);
-        int id[kPerThread];
-        float v[kPerThread];
+    for (int c0 = lo; c0 < hi; c0 += CH) {
+        const int c1 = min(hi, c0 + CH);
+        int id[PT];
+        V v[PT];
 #pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
+        for (int q = 0; q < PT; ++q) {
             const int k = c0 + tid + q * kBlock;
             if (k < c1) {
                 id[q] = idx[k];
                 v[q] = val[k];
             }
         }
-        float g[kPerThread];
+        X g[PT];
 #pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
+        for (int q = 0; q < PT; ++q) {
             const int k = c0 + tid + q * kBlock;
             if (k < c1) g[q] = xs[id[q]];
         }
 #pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
+        for (int q = 0; q < PT; ++q) {
             const int k = c0 + tid + q * kBlock;
-            if (k < c1) lds[k - c0] = make_float2(v[q], g[q]);
+            if (k < c1) lds[k - c0] = Pair<V, X>{v[q], g[q]};
         }
         __syncthreads();
         const int a = max(k0, c0), b = min(k1, c1);
         for (int k = a; k < b; ++k) {
-            const float2 p = lds[k - c0];
-            sum = fma_acc<A>(p.x, p.y, sum);
+            const Pair<V, X> p = lds[k - c0];
+            sum = fma_acc<A, V>(p.v, p.x, sum);
         }
         __syncthreads();
     }
     return sum;
+}
+
+template <typename A>
+__device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
+                                      const int *__restrict__ idx, const float *__restrict__ val,
+                                      const float *__restrict__ xs, void *lds) {
+    return tile_dot_t<float, float, A>(tl, ptr, idx, val, xs, lds);
 }
 
 // Block-wide max of non-negative floats (NaN operands are dropped, as

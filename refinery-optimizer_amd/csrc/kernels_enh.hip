@@ -1,3 +1,631 @@
-// Enhanced-mode kernels.
+// gfx950 kernels of the enhanced mode: scaled, reflected restarted Halpern
+// PDHG with FP32 iterates, FP64 checks and on-device restart / refinement
+// decisions.  Same tile machinery as the reference mode (one workgroup per
+// tile of rows, LDS-staged (value, x) pairs, one lane per row); every launch
+// covers all running instances of a batch and each tile reads its
+// instance's scalars from the device-resident EnhInst array, so the host
+// never has to synchronise inside a check cycle.
 #include <hip/hip_runtime.h>
+
+#include "engine_dev.hpp"
 #include "kernels_common.hpp"
+
+namespace mpdhg {
+namespace dev {
+
+__device__ __forceinline__ void halpern_weights(int it, int last_restart, float &a, float &b) {
+    const int k = it - last_restart;
+    a = static_cast<float>(k + 1) / static_cast<float>(k + 2);
+    b = 1.0f / static_cast<float>(k + 2);
+}
+
+// ---- power iteration on A^T A (step size) ----------------------------------
+__global__ __launch_bounds__(kBlock) void k_pow_scale(const Tile *__restrict__ tiles, int ntiles,
+                                                      int chunk, const EnhInst *__restrict__ st,
+                                                      const float *__restrict__ z,
+                                                      float *__restrict__ v) {
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const double n2 = st[tl.inst].pow_norm2;
+    const float inv = n2 > 0.0 ? static_cast<float>(1.0 / sqrt(n2)) : 0.0f;
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    if (j < tl.end) v[j] = z[j] * inv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ tiles, int ntiles,
+                                                     int chunk, const int *__restrict__ rptr,
+                                                     const int *__restrict__ ridx,
+                                                     const float *__restrict__ rval,
+                                                     const float *__restrict__ v,
+                                                     float *__restrict__ w) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const float s = tile_dot<float>(tl, rptr, ridx, rval, v, lds);
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    if (i < tl.end) w[i] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ tiles, int ntiles,
+                                                     int chunk, const int *__restrict__ cptr,
+                                                     const int *__restrict__ cidx,
+                                                     const float *__restrict__ cval,
+                                                     const float *__restrict__ w,
+                                                     float *__restrict__ z,
+                                                     double *__restrict__ pc) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ double red[kBlock / 64];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const float s = tile_dot<float>(tl, cptr, cidx, cval, w, lds);
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    double q = 0.0;
+    if (j < tl.end) {
+        z[j] = s;
+        q = static_cast<double>(s) * static_cast<double>(s);
+    }
+    q = block_sum_d(q, red);
+    if (threadIdx.x == 0) pc[static_cast<size_t>(t) * kPcCount + kPcPow] = q;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pow_fin(EnhInst *__restrict__ st, int ninst,
+                                                    const double *__restrict__ pc) {
+    __shared__ double red[kBlock / 64];
+    const int q = blockIdx.x;
+    if (q >= ninst) return;
+    EnhInst &s = st[q];
+    double acc = 0.0;
+    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock)
+        acc += pc[static_cast<size_t>(t) * kPcCount + kPcPow];
+    acc = block_sum_d(acc, red);
+    if (threadIdx.x == 0) s.pow_norm2 = acc;
+}
+
+// ---- K1: A^T y fused with the projected primal step and Halpern update ----
+struct PrimalArgs {
+    const int *cptr, *cidx;
+    const float *cval;
+    const float *y, *yp;
+    const float *c, *l, *u;
+    float *x, *x0, *xp, *xbar;
+    const double *xbase, *dc;
+    double *xchk;
+    double *pc;
+    float rho;
+};
+
+__global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
+                                                       int chunk, const EnhInst *__restrict__ st,
+                                                       PrimalArgs A, int it, int check) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ double red[kBlock / 64];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const EnhInst &s = st[tl.inst];
+    if (s.status != kRunning) return;
+    const bool restart = (s.restart_at == it);
+    const float g = tile_dot<float>(tl, A.cptr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    float a, b;
+    halpern_weights(it, s.last_restart, a, b);
+    const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    double d2 = 0.0, e2 = 0.0;
+    if (j < tl.end) {
+        float xo, x0j;
+        if (restart) {
+            xo = A.xp[j];
+            x0j = xo;
+            A.x0[j] = x0j;
+        } else {
+            xo = A.x[j];
+            x0j = A.x0[j];
+        }
+        const float tt = A.c[j] + g;
+        float v = __builtin_fmaf(-tau, tt, xo);
+        const float lo = A.l[j], hi = A.u[j];
+        if (v < lo) v = lo;
+        if (v > hi) v = hi;
+        A.xbar[j] = 2.0f * v - xo;
+        const float r = __builtin_fmaf(rho1, v, -(rho * xo));
+        A.x[j] = __builtin_fmaf(a, r, b * x0j);
+        if (check) {
+            A.xp[j] = v;
+            const double d = static_cast<double>(xo) - static_cast<double>(v);
+            const double e = static_cast<double>(v) - static_cast<double>(x0j);
+            d2 = d * d;
+            e2 = e * e;
+            A.xchk[j] = A.xbase[j] + A.dc[j] * static_cast<double>(v) * s.inv_ap;
+        }
+    }
+    if (check) {
+        d2 = block_sum_d(d2, red);
+        e2 = block_sum_d(e2, red);
+        if (threadIdx.x == 0) {
+            A.pc[static_cast<size_t>(t) * kPcCount + kPcDx2] = d2;
+            A.pc[static_cast<size_t>(t) * kPcCount + kPcDxa2] = e2;
+        }
+    }
+}
+
+// ---- K2: A xbar fused with the shifted dual prox and Halpern update -------
+struct DualArgs {
+    const int *rptr, *ridx;
+    const float *rval;
+    const float *xbar;
+    const float *rl, *ru, *ybar;
+    float *y, *y0, *yp;
+    const double *ybase, *dr;
+    double *ychk;
+    double *pr;
+    float rho;
+};
+
+__global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
+                                                     int chunk, const EnhInst *__restrict__ st,
+                                                     DualArgs A, int it, int check) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ double red[kBlock / 64];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const EnhInst &s = st[tl.inst];
+    if (s.status != kRunning) return;
+    const bool restart = (s.restart_at == it);
+    const float h = tile_dot<float>(tl, A.rptr, A.ridx, A.rval, A.xbar, lds);
+    float a, b;
+    halpern_weights(it, s.last_restart, a, b);
+    const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    double d2 = 0.0, e2 = 0.0;
+    if (i < tl.end) {
+        float yo, y0i;
+        if (restart) {
+            yo = A.yp[i];
+            y0i = yo;
+            A.y0[i] = y0i;
+        } else {
+            yo = A.y[i];
+            y0i = A.y0[i];
+        }
+        // y+ = sigma (w - clamp(w, rl, ru)) - ybar, w = h + (y + ybar) / sigma:
+        // y + sigma (h - bound) on an active bound, -ybar in between.
+        const float yb = A.ybar[i];
+        const float w = __builtin_fmaf(yo + yb, inv_sigma, h);
+        const float lo = A.rl[i], hi = A.ru[i];
+        float yn = -yb;
+        if (w < lo) yn = __builtin_fmaf(sigma, h - lo, yo);
+        else if (w > hi) yn = __builtin_fmaf(sigma, h - hi, yo);
+        const float r = __builtin_fmaf(rho1, yn, -(rho * yo));
+        A.y[i] = __builtin_fmaf(a, r, b * y0i);
+        if (check) {
+            A.yp[i] = yn;
+            const double d = static_cast<double>(yo) - static_cast<double>(yn);
+            const double e = static_cast<double>(yn) - static_cast<double>(y0i);
+            d2 = d * d;
+            e2 = e * e;
+            A.ychk[i] = A.ybase[i] + A.dr[i] * static_cast<double>(yn) * s.inv_ad;
+        }
+    }
+    if (check) {
+        d2 = block_sum_d(d2, red);
+        e2 = block_sum_d(e2, red);
+        if (threadIdx.x == 0) {
+            A.pr[static_cast<size_t>(t) * kPrCount + kPrDy2] = d2;
+            A.pr[static_cast<size_t>(t) * kPrCount + kPrDya2] = e2;
+        }
+    }
+}
+
+// ---- FP64 checks on the original (unscaled) LP ----------------------------
+__device__ __forceinline__ bool fin(double v) { return isfinite(v); }
+
+struct CheckRowArgs {
+    const int *rptr, *ridx;
+    const double *rval;
+    const double *xchk, *ychk;
+    const double *rl, *ru;
+    double *ax;
+    double *pr;
+};
+
+__global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict__ tiles, int ntiles,
+                                                         int chunk, const EnhInst *__restrict__ st,
+                                                         CheckRowArgs A) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ double red[kBlock / 64];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    if (st[tl.inst].status != kRunning) return;
+    const double axi = tile_dot_t<double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    double viol = 0.0, sviol = 0.0, dobj = 0.0;
+    if (i < tl.end) {
+        const double lo = A.rl[i], hi = A.ru[i];
+        if (fin(lo) && axi < lo) viol = lo - axi;
+        if (fin(hi) && axi > hi) viol = axi - hi;
+        A.ax[i] = axi;
+        const double yv = A.ychk[i];
+        double yi = yv;
+        if (!fin(hi) && yi > 0.0) yi = 0.0;
+        if (!fin(lo) && yi < 0.0) yi = 0.0;
+        sviol = fabs(yv - yi);
+        if (yi > 0.0) dobj = -hi * yi;
+        else if (yi < 0.0) dobj = -lo * yi;
+    }
+    viol = block_max_d(viol, red);
+    sviol = block_max_d(sviol, red);
+    dobj = block_sum_d(dobj, red);
+    if (threadIdx.x == 0) {
+        double *p = A.pr + static_cast<size_t>(t) * kPrCount;
+        p[kPrRowViol] = viol;
+        p[kPrSignViol] = sviol;
+        p[kPrDobj] = dobj;
+    }
+}
+
+struct CheckColArgs {
+    const int *cptr, *cidx;
+    const double *cval;
+    const double *ychk, *xchk;
+    const double *c, *l, *u;
+    double *lam;
+    double *pc;
+};
+
+__global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict__ tiles, int ntiles,
+                                                         int chunk, const EnhInst *__restrict__ st,
+                                                         CheckColArgs A) {
+    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ double red[kBlock / 64];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    if (st[tl.inst].status != kRunning) return;
+    const double aty = tile_dot_t<double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
+    if (j < tl.end) {
+        const double lo = A.l[j], hi = A.u[j], xj = A.xchk[j], cj = A.c[j];
+        const double lam = cj + aty;
+        A.lam[j] = lam;
+        double p = lam;
+        if (!fin(lo) && p > 0.0) p = 0.0;
+        if (!fin(hi) && p < 0.0) p = 0.0;
+        dv = fabs(lam - p);
+        if (p > 0.0) dobj = lo * p;
+        else if (p < 0.0) dobj = hi * p;
+        if (fin(lo) && xj < lo) cv = lo - xj;
+        if (fin(hi) && xj > hi) cv = xj - hi;
+        pobj = cj * xj;
+    }
+    dv = block_max_d(dv, red);
+    cv = block_max_d(cv, red);
+    pobj = block_sum_d(pobj, red);
+    dobj = block_sum_d(dobj, red);
+    if (threadIdx.x == 0) {
+        double *p = A.pc + static_cast<size_t>(t) * kPcCount;
+        p[kPcDualViol] = dv;
+        p[kPcColViol] = cv;
+        p[kPcPobj] = pobj;
+        p[kPcDobj] = dobj;
+    }
+}
+
+// ---- per-instance decisions -------------------------------------------------
+struct FinalizeArgs {
+    const double *pc, *pr;
+    int32_t *status_out;
+    double refine_ratio;
+    int patience;
+    int max_refinements;
+};
+
+__device__ __forceinline__ void set_steps(EnhInst &s) {
+    s.tau = static_cast<float>(s.eta / s.omega);
+    s.sigma = static_cast<float>(s.eta * s.omega);
+    s.inv_sigma = static_cast<float>(1.0 / (s.eta * s.omega));
+}
+
+__global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ st, int ninst,
+                                                         FinalizeArgs A, int it) {
+    __shared__ double red[kBlock / 64];
+    const int q = blockIdx.x;
+    if (q >= ninst) return;
+    EnhInst &s = st[q];
+    if (s.status != kRunning) {
+        if (threadIdx.x == 0) {
+            s.copy_best = 0;
+            s.refine = 0;
+        }
+        return;
+    }
+    double dx2 = 0, dxa2 = 0, dv = 0, cv = 0, pobj = 0, dobjc = 0;
+    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock) {
+        const double *p = A.pc + static_cast<size_t>(t) * kPcCount;
+        dx2 += p[kPcDx2];
+        dxa2 += p[kPcDxa2];
+        dv = fmax(dv, p[kPcDualViol]);
+        cv = fmax(cv, p[kPcColViol]);
+        pobj += p[kPcPobj];
+        dobjc += p[kPcDobj];
+    }
+    double dy2 = 0, dya2 = 0, rv = 0, sv = 0, dobjr = 0;
+    for (int t = s.row_tile_begin + static_cast<int>(threadIdx.x); t < s.row_tile_end; t += kBlock) {
+        const double *p = A.pr + static_cast<size_t>(t) * kPrCount;
+        dy2 += p[kPrDy2];
+        dya2 += p[kPrDya2];
+        rv = fmax(rv, p[kPrRowViol]);
+        sv = fmax(sv, p[kPrSignViol]);
+        dobjr += p[kPrDobj];
+    }
+    dx2 = block_sum_d(dx2, red);
+    dxa2 = block_sum_d(dxa2, red);
+    dv = block_max_d(dv, red);
+    cv = block_max_d(cv, red);
+    pobj = block_sum_d(pobj, red);
+    dobjc = block_sum_d(dobjc, red);
+    dy2 = block_sum_d(dy2, red);
+    dya2 = block_sum_d(dya2, red);
+    rv = block_max_d(rv, red);
+    sv = block_max_d(sv, red);
+    dobjr = block_sum_d(dobjr, red);
+    if (threadIdx.x != 0) return;
+
+    const int done = it + 1;
+    const double dobj = dobjr + dobjc;
+    const double ep = fmax(rv, cv) / (1.0 + s.bnorm);
+    const double ed = fmax(dv, sv) / (1.0 + s.cnorm);
+    const double eg = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+    const double kkt = fmax(ep, fmax(ed, eg));
+    s.checks += 1;
+    s.eps_p = ep;
+    s.eps_d = ed;
+    s.eps_gap = eg;
+    s.kkt = kkt;
+    s.pobj = pobj;
+    s.dobj = dobj;
+    s.copy_best = 0;
+    s.refine = 0;
+    if (isnan(kkt)) {  // NaN/Inf reached the iterates: stop instead of propagating
+        s.status = 6;  // NumericalFailure
+        s.iterations = done;
+        A.status_out[q] = 6;
+        return;
+    }
+    if (!(kkt >= s.best_kkt)) {
+        s.best_kkt = kkt;
+        s.best_eps_p = ep;
+        s.best_eps_d = ed;
+        s.best_eps_gap = eg;
+        s.best_pobj = pobj;
+        s.best_dobj = dobj;
+        s.copy_best = 1;
+    }
+    if (kkt <= s.tol) {
+        s.status = 0;
+        s.iterations = done;
+        A.status_out[q] = 0;
+        return;
+    }
+    if (done >= s.max_iter) {
+        s.status = 4;
+        s.iterations = done;
+        A.status_out[q] = 4;
+        return;
+    }
+    if (kkt < s.round_best) {
+        s.round_best = kkt;
+        s.since_best = 0;
+    } else {
+        s.since_best += 1;
+    }
+    if (s.round_start_kkt == HUGE_VAL) s.round_start_kkt = kkt;
+    if (s.rounds < A.max_refinements && s.best_kkt < HUGE_VAL &&
+        (s.round_best <= A.refine_ratio * s.round_start_kkt || s.since_best >= A.patience)) {
+        const double dp = fmax(s.best_eps_p, s.best_eps_gap), dd = fmax(s.best_eps_d, s.best_eps_gap);
+        double ap = fmin(1.0 / fmax(dp, 1e-300), 1e12), ad = fmin(1.0 / fmax(dd, 1e-300), 1e12);
+        if (ap < 1.0) ap = 1.0;
+        if (ad < 1.0) ad = 1.0;
+        s.omega *= (ad / ap) / (s.alpha_d / s.alpha_p);
+        s.alpha_p = ap;
+        s.alpha_d = ad;
+        s.inv_ap = 1.0 / ap;
+        s.inv_ad = 1.0 / ad;
+        s.rounds += 1;
+        s.last_restart = done;
+        s.round_start = done;
+        s.round_start_kkt = s.best_kkt;
+        s.round_best = HUGE_VAL;
+        s.since_best = 0;
+        s.r_anchor = -1.0;
+        s.r_prev = -1.0;
+        s.refine = 1;
+        set_steps(s);
+        return;
+    }
+    const double r = sqrt(s.omega * dx2 + dy2 / s.omega);
+    const int kc = done - s.last_restart;
+    if (s.r_anchor < 0.0) {
+        s.r_anchor = r;
+        s.r_prev = r;
+        return;
+    }
+    const bool rs = (r <= 0.2 * s.r_anchor) || (r <= 0.8 * s.r_anchor && r > s.r_prev) ||
+                    (static_cast<double>(kc) >= 0.36 * static_cast<double>(done - s.round_start));
+    s.r_prev = r;
+    if (rs) {
+        const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
+        if (ddx > 1e-10 && ddy > 1e-10) s.omega = sqrt(s.omega * ddy / ddx);
+        s.last_restart = done;
+        s.restart_at = done;
+        s.restarts += 1;
+        s.r_anchor = r;
+        set_steps(s);
+    }
+}
+
+// ---- best-point copies and residual-LP construction -----------------------
+__global__ __launch_bounds__(kBlock) void k_enh_best_cols(const Tile *__restrict__ tiles, int ntiles,
+                                                          int chunk, const EnhInst *__restrict__ st,
+                                                          const double *__restrict__ xchk,
+                                                          const double *__restrict__ lam,
+                                                          double *__restrict__ xbest,
+                                                          double *__restrict__ lambest) {
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    if (!st[tl.inst].copy_best) return;
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    if (j < tl.end) {
+        xbest[j] = xchk[j];
+        lambest[j] = lam[j];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_enh_best_rows(const Tile *__restrict__ tiles, int ntiles,
+                                                          int chunk, const EnhInst *__restrict__ st,
+                                                          const double *__restrict__ ychk,
+                                                          const double *__restrict__ ax,
+                                                          double *__restrict__ ybest,
+                                                          double *__restrict__ axbest) {
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    if (!st[tl.inst].copy_best) return;
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    if (i < tl.end) {
+        ybest[i] = ychk[i];
+        axbest[i] = ax[i];
+    }
+}
+
+struct RefineColArgs {
+    const double *xbest, *lambest, *dc, *l64, *u64;
+    double *xbase;
+    float *c, *l, *u, *x, *x0, *xp;
+};
+
+// Residual LP of the best point (x~, y~): costs alpha_d * lambda~, bounds
+// alpha_p * ([l, u] - x~), in the scaled space; the correction restarts at 0.
+__global__ __launch_bounds__(kBlock) void k_enh_refine_cols(const Tile *__restrict__ tiles,
+                                                            int ntiles, int chunk,
+                                                            const EnhInst *__restrict__ st,
+                                                            RefineColArgs A) {
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const EnhInst &s = st[tl.inst];
+    if (!s.refine) return;
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    if (j < tl.end) {
+        const double xb = A.xbest[j], d = A.dc[j];
+        A.xbase[j] = xb;
+        A.c[j] = static_cast<float>(d * (s.alpha_d * A.lambest[j]));
+        const float lo = static_cast<float>(s.alpha_p * (A.l64[j] - xb) / d);
+        const float hi = static_cast<float>(s.alpha_p * (A.u64[j] - xb) / d);
+        A.l[j] = lo;
+        A.u[j] = hi;
+        float v = 0.0f;
+        if (v < lo) v = lo;
+        if (v > hi) v = hi;
+        A.x[j] = v;
+        A.x0[j] = v;
+        A.xp[j] = v;
+    }
+}
+
+struct RefineRowArgs {
+    const double *ybest, *axbest, *dr, *rl64, *ru64;
+    double *ybase;
+    float *rl, *ru, *ybar, *y, *y0, *yp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_enh_refine_rows(const Tile *__restrict__ tiles,
+                                                            int ntiles, int chunk,
+                                                            const EnhInst *__restrict__ st,
+                                                            RefineRowArgs A) {
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const EnhInst &s = st[tl.inst];
+    if (!s.refine) return;
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    if (i < tl.end) {
+        const double yb = A.ybest[i], ab = A.axbest[i], d = A.dr[i];
+        A.ybase[i] = yb;
+        A.rl[i] = static_cast<float>(s.alpha_p * (A.rl64[i] - ab) * d);
+        A.ru[i] = static_cast<float>(s.alpha_p * (A.ru64[i] - ab) * d);
+        A.ybar[i] = static_cast<float>(s.alpha_d * yb / d);
+        A.y[i] = 0.0f;
+        A.y0[i] = 0.0f;
+        A.yp[i] = 0.0f;
+    }
+}
+
+}  // namespace dev
+
+// ---- launchers --------------------------------------------------------------
+namespace launch {
+
+#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
+
+void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v) {
+    if (cts.count) hipLaunchKernelGGL(dev::k_pow_scale, MPDHG_TILE_GRID(cts), st, z, v);
+}
+void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
+              const float *rval, const float *v, float *w) {
+    if (rts.count) hipLaunchKernelGGL(dev::k_pow_rows, MPDHG_TILE_GRID(rts), rptr, ridx, rval, v, w);
+}
+void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
+              const float *cval, const float *w, float *z, double *pc) {
+    if (cts.count) hipLaunchKernelGGL(dev::k_pow_cols, MPDHG_TILE_GRID(cts), cptr, cidx, cval, w, z, pc);
+}
+void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc) {
+    hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kBlock), 0, s, st, ninst, pc);
+}
+void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int it,
+                int check) {
+    dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
+                      a.xbar, a.xbase, a.dc, a.xchk, a.pc, a.rho};
+    if (cts.count) hipLaunchKernelGGL(dev::k_enh_primal, MPDHG_TILE_GRID(cts), st, A, it, check);
+}
+void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
+              int check) {
+    dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
+                    a.ybase, a.dr, a.ychk, a.pr, a.rho};
+    if (rts.count) hipLaunchKernelGGL(dev::k_enh_dual, MPDHG_TILE_GRID(rts), st, A, it, check);
+}
+void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
+               const EnhCheck &a, int it) {
+    dev::CheckRowArgs R{a.rptr, a.ridx, a.rval64, a.xchk, a.ychk, a.rl64, a.ru64, a.ax, a.pr};
+    dev::CheckColArgs Cc{a.cptr, a.cidx, a.cval64, a.ychk, a.xchk, a.c64, a.l64, a.u64, a.lam,
+                         a.pc};
+    if (rts.count) hipLaunchKernelGGL(dev::k_enh_chk_rows, MPDHG_TILE_GRID(rts), st, R);
+    if (cts.count) hipLaunchKernelGGL(dev::k_enh_chk_cols, MPDHG_TILE_GRID(cts), st, Cc);
+    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.patience, a.max_refinements};
+    hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, it);
+    if (cts.count)
+        hipLaunchKernelGGL(dev::k_enh_best_cols, MPDHG_TILE_GRID(cts), st, a.xchk, a.lam, a.xbest,
+                           a.lambest);
+    if (rts.count)
+        hipLaunchKernelGGL(dev::k_enh_best_rows, MPDHG_TILE_GRID(rts), st, a.ychk, a.ax, a.ybest,
+                           a.axbest);
+    if (a.max_refinements > 0) {
+        dev::RefineColArgs RC{a.xbest, a.lambest, a.dc, a.l64, a.u64, a.xbase,
+                              a.cs, a.ls, a.us, a.x, a.x0, a.xp};
+        dev::RefineRowArgs RR{a.ybest, a.axbest, a.dr, a.rl64, a.ru64, a.ybase,
+                              a.rls, a.rus, a.ybar, a.y, a.y0, a.yp};
+        if (cts.count) hipLaunchKernelGGL(dev::k_enh_refine_cols, MPDHG_TILE_GRID(cts), st, RC);
+        if (rts.count) hipLaunchKernelGGL(dev::k_enh_refine_rows, MPDHG_TILE_GRID(rts), st, RR);
+    }
+}
+
+#undef MPDHG_TILE_GRID
+
+}  // namespace launch
+}  // namespace mpdhg

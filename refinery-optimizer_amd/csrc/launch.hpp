@@ -35,5 +35,56 @@ void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, cons
 void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const float *x,
                    float *xbest);
 
+// enhanced mode
+struct EnhPrimal {
+    const int *cptr, *cidx;
+    const float *cval;
+    const float *y, *yp;
+    const float *c, *l, *u;
+    float *x, *x0, *xp, *xbar;
+    const double *xbase, *dc;
+    double *xchk;
+    double *pc;
+    float rho;
+};
+struct EnhDual {
+    const int *rptr, *ridx;
+    const float *rval;
+    const float *xbar;
+    const float *rl, *ru, *ybar;
+    float *y, *y0, *yp;
+    const double *ybase, *dr;
+    double *ychk;
+    double *pr;
+    float rho;
+};
+struct EnhCheck {
+    const int *rptr, *ridx, *cptr, *cidx;
+    const double *rval64, *cval64;
+    double *xchk, *ychk, *ax, *lam;
+    const double *rl64, *ru64, *c64, *l64, *u64, *dc, *dr;
+    double *pr, *pc;
+    int32_t *status_out;
+    double refine_ratio;
+    int patience, max_refinements;
+    double *xbest, *lambest, *ybest, *axbest, *xbase, *ybase;
+    float *cs, *ls, *us, *x, *x0, *xp, *rls, *rus, *ybar, *y, *y0, *yp;
+};
+
+void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v);
+void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
+              const float *rval, const float *v, float *w);
+void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
+              const float *cval, const float *w, float *z, double *pc);
+void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc);
+void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int it,
+                int check);
+void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
+              int check);
+// FP64 check, per-instance decisions, best-point copies and (when enabled)
+// residual-LP construction, in that stream order.
+void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
+               const EnhCheck &a, int it);
+
 }  // namespace launch
 }  // namespace mpdhg

@@ -259,6 +259,8 @@ class Solution:
     kernel_launches_csc: int = 0
     bytes_csr_launch: float = 0.0
     bytes_csc_launch: float = 0.0
+    step_size: float = 0.0
+    primal_weight: float = 0.0
 
     def is_optimal(self) -> bool:
         return self.status == SolutionStatus.Optimal
@@ -289,6 +291,18 @@ class GpuPdhgConfig:
     ruiz_iterations: int = 10
     profile: bool = False
     dual_sign: int = 0
+    refine_ratio: float = 1e-4
+    refine_patience: int = 8
+
+    @staticmethod
+    def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
+                 max_refinements: int = 8, **kw) -> "GpuPdhgConfig":
+        """Configuration of the enhanced mode (restarted Halpern PDHG, FP64 KKT target)."""
+        c = GpuPdhgConfig(max_iterations=max_iterations, check_frequency=check_frequency,
+                          mode=MPDHG_MODE_ENHANCED, tol_kkt=tol, max_refinements=max_refinements)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
 
     def _c(self) -> MpdhgConfig:
         c = MpdhgConfig()
@@ -317,7 +331,7 @@ def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
     for name in ("restarts", "refinements", "checks", "rel_primal", "rel_dual", "rel_gap",
                  "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
                  "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
-                 "bytes_csr_launch", "bytes_csc_launch"):
+                 "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight"):
         setattr(s, name, getattr(r, name))
     return s
 

@@ -34,6 +34,8 @@ class MpdhgConfig(C.Structure):
         ("ruiz_iterations", C.c_int32),
         ("profile", C.c_int32),
         ("dual_sign", C.c_int32),
+        ("refine_ratio", C.c_double),
+        ("refine_patience", C.c_int32),
     ]
 
 
@@ -80,6 +82,8 @@ class MpdhgResult(C.Structure):
         ("kernel_launches_csc", C.c_int64),
         ("bytes_csr_launch", C.c_double),
         ("bytes_csc_launch", C.c_double),
+        ("step_size", C.c_double),
+        ("primal_weight", C.c_double),
     ]
 
 

@@ -1,0 +1,97 @@
+"""Enhanced mode on the GPU against its CPU port and against HiGHS.
+
+Tolerances:
+  * first check cycle (64 iterations, no restart decision yet): x and y equal
+    the CPU port bit for bit when both use the same step size eta;
+  * a batch equals the same LPs solved one by one, bit for bit;
+  * converged solutions: the FP64 relative KKT recomputed by the oracle from
+    the returned (x, y) is <= the requested tolerance, and the objective
+    matches HiGHS to 10 x that tolerance (relative).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sih_solver as S
+from lpfix import random_lp, toy01
+from test_oracle import _highs
+
+pytestmark = pytest.mark.gpu
+
+
+def enh(**kw):
+    return S.GpuPdhgConfig.enhanced(**kw)
+
+
+def port(p, sol, **kw):
+    return O.pdhg_enh(O.LpArrays.of(p), step=sol.step_size, **kw)
+
+
+def test_first_cycle_bitexact_vs_cpu_port(need_gpu):
+    for p in (S.gen_random(300, 720, 8, 3), S.gen_staircase(2, 40, 90, 4), random_lp(60, 90, 0.1, 2)[0]):
+        sol = S.solve_gpu_pdhg(p, enh(tol=0.0, max_iterations=64, max_refinements=0))
+        ref = port(p, sol, tol=0.0, max_iterations=64, check_frequency=64, max_refinements=0)
+        assert sol.simplex_iterations == ref["iterations"] == 64, p.name
+        assert np.array_equal(sol.x, ref["x"]), p.name
+        assert np.array_equal(sol.row_duals, ref["y"]), p.name
+        assert math.isclose(sol.rel_kkt, ref["kkt"], rel_tol=1e-9), p.name
+        free = O.pdhg_enh(O.LpArrays.of(p), tol=0.0, max_iterations=64, max_refinements=0)
+        assert math.isclose(sol.step_size, free["step"], rel_tol=1e-5), p.name  # power iteration
+
+
+def test_batch_equals_single_solves(need_gpu):
+    probs = [S.gen_random(200, 480, 8, 11), S.gen_staircase(1, 200, 480, 12), toy01(),
+             random_lp(80, 120, 0.08, 13)[0]]
+    cfg = enh(tol=1e-6)
+    sols = S.solve_batch(probs, cfg)
+    for p, s in zip(probs, sols):
+        one = S.solve_gpu_pdhg(p, cfg)
+        assert one.simplex_iterations == s.simplex_iterations, p.name
+        assert np.array_equal(one.x, s.x) and np.array_equal(one.row_duals, s.row_duals), p.name
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-8])
+def test_converges_and_matches_highs(need_gpu, tol):
+    probs = [S.gen_random(300, 720, 8, 3), S.gen_random(1000, 2500, 8, 0),
+             S.gen_staircase(2, 400, 900, 1), toy01(), random_lp(120, 200, 0.05, 21)[0]]
+    sols = S.solve_batch(probs, enh(tol=tol, max_iterations=200000))
+    for p, s in zip(probs, sols):
+        lp = O.LpArrays.of(p)
+        assert s.status == S.SolutionStatus.Optimal, (p.name, s.rel_kkt)
+        k = O.kkt(lp, s.x, s.row_duals)
+        assert k["kkt"] <= tol * 1.0001, (p.name, k)
+        assert math.isclose(k["kkt"], s.rel_kkt, rel_tol=1e-6, abs_tol=1e-12), p.name
+        h, _ = _highs(lp)
+        assert abs(s.primal_objective - h.fun) <= 10 * tol * (1 + abs(h.fun)), (p.name, s.primal_objective, h.fun)
+
+
+def test_full_solve_tracks_cpu_port(need_gpu):
+    p = S.gen_staircase(2, 400, 900, 1)
+    sol = S.solve_gpu_pdhg(p, enh(tol=1e-6))
+    ref = port(p, sol, tol=1e-6, max_iterations=100000, check_frequency=64, max_refinements=8,
+               refine_ratio=1e-4, patience=8)
+    assert ref["status"] == 0 and sol.status == S.SolutionStatus.Optimal
+    assert abs(sol.primal_objective - ref["primal_obj"]) <= 1e-5 * (1 + abs(ref["primal_obj"]))
+    assert 0.5 <= sol.simplex_iterations / ref["iterations"] <= 2.0
+
+
+def test_nan_cost_reports_numerical_failure(need_gpu):
+    p = S.gen_random(50, 120, 4, 5)
+    p.c = p.c.copy()
+    p.c[7] = math.nan
+    sol = S.solve_gpu_pdhg(p, enh(tol=1e-6, max_iterations=1000))
+    assert sol.status == S.SolutionStatus.NumericalFailure
+
+
+def test_iteration_limit_and_short_runs(need_gpu):
+    p = S.gen_random(100, 240, 8, 6)
+    for K in (1, 10, 64, 65, 130):
+        s = S.solve_gpu_pdhg(p, enh(tol=1e-12, max_iterations=K, max_refinements=0))
+        assert s.status == S.SolutionStatus.IterationLimit and s.simplex_iterations == K
+        k = O.kkt(O.LpArrays.of(p), s.x, s.row_duals)
+        assert math.isclose(k["kkt"], s.rel_kkt, rel_tol=1e-6), K
+    e = S.Problem("empty")
+    e.resize(3, 0)
+    assert S.solve_gpu_pdhg(e, enh()).status == S.SolutionStatus.Optimal
