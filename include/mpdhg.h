@@ -74,9 +74,9 @@ typedef struct mpdhg_config {
     int32_t profile;              /* 1: time the fused SpMV kernels with HIP events */
     int32_t dual_sign;            /* 0: reference PDHG sign (reduced cost = c + A^T y);
                                      1: simplex/checker sign (c - A^T y, checker.cpp:231) */
-    double refine_ratio;          /* enhanced: start a refinement round once the round's best
-                                     KKT <= ratio * its starting KKT (1e-4) ... */
-    int32_t refine_patience;      /* ... or after this many checks without a new best (8) */
+    double refine_ratio;          /* enhanced: at a restart, re-centre the FP32 iterates on the
+                                     restart point (one refinement round) once the FP64 KKT fell
+                                     below ratio * its value at the last re-centring (1e-2) */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite

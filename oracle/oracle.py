@@ -34,7 +34,7 @@ class EnhCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("check_frequency", C.c_int),
                 ("tol", C.c_double), ("reflection", C.c_double), ("ruiz_iters", C.c_int),
                 ("threads", C.c_int), ("step", C.c_double), ("max_refinements", C.c_int),
-                ("refine_ratio", C.c_double), ("patience", C.c_int)]
+                ("refine_ratio", C.c_double)]
 
 
 class EnhOut(C.Structure):
@@ -178,11 +178,11 @@ def spmv_seq_f32(rows, ptr, idx, v, x):
 
 
 def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
-             reflection=1.0, ruiz_iters=10, threads=1, step=0.0, max_refinements=0,
-             refine_ratio=1e-3, patience=8):
+             reflection=1.0, ruiz_iters=10, threads=1, step=0.0, max_refinements=8,
+             refine_ratio=1e-2):
     """CPU port of the enhanced mode; returns a dict with the best point."""
     cfg = EnhCfg(max_iterations, check_frequency, tol, reflection, ruiz_iters, threads, step,
-                 max_refinements, refine_ratio, patience)
+                 max_refinements, refine_ratio)
     out = EnhOut()
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)

@@ -12,7 +12,8 @@
  *     (Lu & Yang, "Restarted Halpern PDHG for linear programming", 2024);
  *   - primal weight smoothing at restarts (Applegate et al. 2021, theta=1/2);
  *   - iterative refinement on the residual LP (Gleixner, Steffy & Wolter,
- *     INFORMS J. Comput. 2016), FP32 inner solves with FP64 outer updates.
+ *     INFORMS J. Comput. 2016), FP32 inner solves with FP64 outer updates,
+ *     taken at restarts once the KKT error fell by refine_ratio.
  * FP32 iterate arithmetic follows the GPU kernels operation by operation
  * (explicit fmaf; compiled with -ffp-contract=off).  FP64 reductions and
  * the KKT check run in a different order than on the GPU, so agreement is
@@ -200,11 +201,10 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     }
     const float rho = (float)cfg->reflection, rho1 = 1.0f + rho;
     const int F = cfg->check_frequency;
-    int last_restart = 0, round_start = 0, restarts = 0, restart_next = 0, rounds = 0;
-    int since_best = 0;
+    int last_restart = 0, restarts = 0, restart_next = 0, rounds = 0;
     double r_anchor = -1.0, r_prev = -1.0;
     double best[10] = {0}, kk[10] = {0};
-    double best_kkt = INFINITY, round_start_kkt = INFINITY, round_best = INFINITY;
+    double best_kkt = INFINITY, round_start_kkt = INFINITY;
     int it, status = 4, have_best = 0;
     for (it = 0; it < cfg->max_iterations; ++it) {
         if (restart_next) { /* take z+ as iterate and anchor */
@@ -274,21 +274,38 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
         }
         if (kk[3] <= cfg->tol) { status = 0; it = done; break; }
         if (done >= cfg->max_iterations) { it = done; break; }
-        if (kk[3] < round_best) { round_best = kk[3]; since_best = 0; } else { since_best++; }
         if (round_start_kkt == INFINITY) round_start_kkt = kk[3]; /* first check of the solve */
-        if (rounds < cfg->max_refinements && have_best &&
-            (round_best <= cfg->refine_ratio * round_start_kkt || since_best >= cfg->patience)) {
-            /* ---- refinement: residual LP around the best point ---- */
-            const double dp = fmax(best[0], best[2]), dd = fmax(best[1], best[2]);
+        const double r = sqrt(omega * dx2 + dy2 / omega);
+        const int kc = done - last_restart;
+        if (r_anchor < 0.0) { r_anchor = r_prev = r; continue; }
+        const int rs = (r <= 0.2 * r_anchor) || (r <= 0.8 * r_anchor && r > r_prev) ||
+                       ((double)kc >= 0.36 * (double)done);
+        r_prev = r;
+        if (!rs) continue;
+        /* ---- restart to z+ = this check's point ---- */
+        {
+            const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
+            if (ddx > 1e-10 && ddy > 1e-10) omega = sqrt(omega * ddy / ddx);
+        }
+        last_restart = done;
+        restarts++;
+        r_anchor = r;
+        if (rounds < cfg->max_refinements && kk[3] <= cfg->refine_ratio * round_start_kkt) {
+            /* Refinement at the restart: the same restart point, re-centred in
+             * FP64 as the residual LP (x = x~ + D_c d / alpha_p,
+             * y = y~ + D_r e / alpha_d) so that the FP32 iterates carry the
+             * correction at full precision.  In exact arithmetic this is the
+             * plain restart (same effective primal weight). */
+            const double dp = fmax(kk[0], kk[2]), dd = fmax(kk[1], kk[2]);
             double ap = fmin(1.0 / fmax(dp, 1e-300), 1e12), ad = fmin(1.0 / fmax(dd, 1e-300), 1e12);
             if (ap < 1.0) ap = 1.0;
             if (ad < 1.0) ad = 1.0;
             omega *= (ad / ap) / (alpha_d / alpha_p);
             alpha_p = ap; alpha_d = ad; inv_ap = 1.0 / ap; inv_ad = 1.0 / ad;
-            memcpy(xbase, xbest, sizeof(double) * (size_t)n);
-            memcpy(ybase, ybest, sizeof(double) * (size_t)m);
+            memcpy(xbase, xc, sizeof(double) * (size_t)n);
+            memcpy(ybase, yc_, sizeof(double) * (size_t)m);
             for (int j = 0; j < n; ++j) {
-                cs[j] = (float)(P.dc[j] * (ad * lambest[j]));
+                cs[j] = (float)(P.dc[j] * (ad * lam[j]));
                 ls[j] = (float)(ap * (l[j] - xbase[j]) / P.dc[j]);
                 us[j] = (float)(ap * (u[j] - xbase[j]) / P.dc[j]);
                 float v = 0.0f;
@@ -297,33 +314,16 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
                 x[j] = x0[j] = xp[j] = v;
             }
             for (int i = 0; i < m; ++i) {
-                rls[i] = (float)(ap * (rl[i] - axbest[i]) * P.dr[i]);
-                rus[i] = (float)(ap * (ru[i] - axbest[i]) * P.dr[i]);
+                rls[i] = (float)(ap * (rl[i] - ax[i]) * P.dr[i]);
+                rus[i] = (float)(ap * (ru[i] - ax[i]) * P.dr[i]);
                 ybar[i] = (float)(ad * ybase[i] / P.dr[i]);
                 y[i] = y0[i] = yp[i] = 0.0f;
             }
             rounds++;
-            last_restart = round_start = done;
-            round_start_kkt = best_kkt;
-            round_best = INFINITY;
-            since_best = 0;
-            r_anchor = r_prev = -1.0;
+            round_start_kkt = kk[3];
             continue;
         }
-        const double r = sqrt(omega * dx2 + dy2 / omega);
-        const int kc = done - last_restart;
-        if (r_anchor < 0.0) { r_anchor = r_prev = r; continue; }
-        const int rs = (r <= 0.2 * r_anchor) || (r <= 0.8 * r_anchor && r > r_prev) ||
-                       ((double)kc >= 0.36 * (double)(done - round_start));
-        r_prev = r;
-        if (rs) {
-            const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
-            if (ddx > 1e-10 && ddy > 1e-10) omega = sqrt(omega * ddy / ddx);
-            last_restart = done;
-            restart_next = 1;
-            restarts++;
-            r_anchor = r;
-        }
+        restart_next = 1;
     }
     if (status != 0 && !have_best) {
         for (int j = 0; j < n; ++j) xbest[j] = xbase[j] + P.dc[j] * (double)xp[j] * inv_ap;

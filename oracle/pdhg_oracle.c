@@ -154,7 +154,7 @@ int orc_kkt(int m, int n, const int *col_ptr, const int *row_ind, const double *
     double ed = dv / (1.0 + cn);
     double eg = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
     out[0] = ep; out[1] = ed; out[2] = eg;
-    out[3] = fmax(ep, fmax(ed, eg));
+    out[3] = (isnan(ep) || isnan(ed) || isnan(eg)) ? ep + ed + eg : fmax(ep, fmax(ed, eg));
     out[4] = pobj; out[5] = dobj; out[6] = rowv; out[7] = colv; out[8] = dv; out[9] = bn;
     free(ax);
     return 0;

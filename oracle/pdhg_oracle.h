@@ -99,8 +99,7 @@ typedef struct orc_enh_cfg {
     int threads;           /* OpenMP threads (1 = scalar) */
     double step;           /* eta; <= 0: 0.998 / sigma_max by power iteration */
     int max_refinements;   /* outer FP64 refinement rounds */
-    double refine_ratio;   /* refine once the round's best KKT <= ratio * its start */
-    int patience;          /* ... or after this many checks without a new round best */
+    double refine_ratio;   /* at a restart, refine once KKT <= ratio * KKT at the last re-centring */
 } orc_enh_cfg;
 typedef struct orc_enh_out {
     int status, iterations, restarts, refinements;

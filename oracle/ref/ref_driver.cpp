@@ -1,0 +1,109 @@
+// TEST INFRASTRUCTURE ONLY: command-line driver around the reference's own
+// FP64 dual simplex (core/src/simplex, built from the reference sources by
+// oracle/ref/Makefile into oracle/_ref/).  Used by tests/golden/make_fixtures.py
+// to write objective / primal / dual / basis fixtures; never shipped.
+//
+// Input (text, whitespace separated; "inf"/"-inf" allowed):
+//   m n nnz | col_ptr[n+1] | row_ind[nnz] | vals[nnz] | c[n] | l[n] | u[n] | rl[m] | ru[m]
+// Or: refsimplex --mps file.mps dump.txt -- parse an MPS file with the
+// reference's own reader (core/src/io/mps_io.cpp), write it in the format
+// above (plus a first line "sense S") and solve it.
+// Output: "status S", "objective V", "iterations K", then x, row_duals,
+// reduced_costs, col_basis, row_basis, one vector per line.
+#include <sih/io/mps_io.hpp>
+#include <sih/model/problem.hpp>
+#include <sih/model/options.hpp>
+#include <sih/simplex/simplex_solver.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <string>
+#include <vector>
+
+static double rd(std::istream &in) {
+    std::string t;
+    in >> t;
+    if (t == "inf") return std::numeric_limits<double>::infinity();
+    if (t == "-inf") return -std::numeric_limits<double>::infinity();
+    return std::stod(t);
+}
+
+static void dump(const sih::model::Problem &p, const char *path) {
+    std::FILE *f = std::fopen(path, "w");
+    const auto &A = p.A();
+    std::fprintf(f, "sense %d\n%lld %lld %lld\n", static_cast<int>(p.sense()),
+                 static_cast<long long>(p.num_rows()), static_cast<long long>(p.num_cols()),
+                 static_cast<long long>(p.num_nonzeros()));
+    auto ivec = [&](const std::vector<int64_t> &x) { for (auto e : x) std::fprintf(f, "%lld ", static_cast<long long>(e)); std::fprintf(f, "\n"); };
+    auto dvec = [&](const std::vector<double> &x) {
+        for (double e : x) {
+            if (std::isinf(e)) std::fprintf(f, "%s ", e > 0 ? "inf" : "-inf");
+            else std::fprintf(f, "%.17g ", e);
+        }
+        std::fprintf(f, "\n");
+    };
+    ivec(A.csc_col_ptr()); ivec(A.csc_row_ind()); dvec(A.csc_values());
+    dvec(p.c()); dvec(p.col_lower()); dvec(p.col_upper()); dvec(p.row_lower()); dvec(p.row_upper());
+    std::fclose(f);
+}
+
+static void solve_and_print(const sih::model::Problem &p);
+
+int main(int argc, char **argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: refsimplex lp.txt | --mps file.mps dump.txt\n"); return 2; }
+    if (std::string(argv[1]) == "--mps" && argc >= 4) {
+        sih::model::Problem p = sih::io::read_mps(argv[2]);
+        dump(p, argv[3]);
+        solve_and_print(p);
+        return 0;
+    }
+    std::ifstream in(argv[1]);
+    long m, n, nnz;
+    in >> m >> n >> nnz;
+    std::vector<long> cp(n + 1), ri(nnz);
+    std::vector<double> v(nnz);
+    for (auto &e : cp) in >> e;
+    for (auto &e : ri) in >> e;
+    for (auto &e : v) e = rd(in);
+    sih::model::Problem p("fixture");
+    p.resize(m, n);
+    std::vector<sih::model::Triplet> trip;
+    for (long j = 0; j < n; ++j)
+        for (long k = cp[j]; k < cp[j + 1]; ++k) trip.emplace_back(ri[k], j, v[k]);
+    p.set_A(sih::model::SparseMatrix::from_triplets(m, n, trip));
+    std::vector<double> c(n);
+    for (auto &e : c) e = rd(in);
+    p.set_c(c);
+    for (long j = 0; j < n; ++j) p.col_lower()[j] = rd(in);
+    for (long j = 0; j < n; ++j) p.col_upper()[j] = rd(in);
+    for (long i = 0; i < m; ++i) p.row_lower()[i] = rd(in);
+    for (long i = 0; i < m; ++i) p.row_upper()[i] = rd(in);
+    solve_and_print(p);
+    return 0;
+}
+
+static void solve_and_print(const sih::model::Problem &p) {
+    sih::model::Options opts;
+    opts.log_to_console = false;
+    opts.telemetry_log_path = "";
+    opts.strategy.algorithm = sih::model::AlgorithmChoice::DualSimplex;
+    auto s = sih::simplex::SimplexSolver::solve(p, opts);
+    std::printf("status %d\nobjective %.17g\niterations %lld\n", static_cast<int>(s.status),
+                s.primal_objective, static_cast<long long>(s.simplex_iterations));
+    auto vec = [](const char *name, const std::vector<double> &x) {
+        std::printf("%s", name);
+        for (double e : x) std::printf(" %.17g", e);
+        std::printf("\n");
+    };
+    vec("x", s.x);
+    vec("row_duals", s.row_duals);
+    vec("reduced_costs", s.reduced_costs);
+    std::printf("col_basis");
+    for (auto b : s.col_basis) std::printf(" %d", static_cast<int>(b));
+    std::printf("\nrow_basis");
+    for (auto b : s.row_basis) std::printf(" %d", static_cast<int>(b));
+    std::printf("\n");
+}

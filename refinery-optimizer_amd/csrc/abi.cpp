@@ -104,8 +104,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->ruiz_iterations = 10;
     cfg->profile = 0;
     cfg->dual_sign = 0;
-    cfg->refine_ratio = 1e-4;
-    cfg->refine_patience = 8;
+    cfg->refine_ratio = 1e-2;
 }
 
 int mpdhg_device_count(int *count) {

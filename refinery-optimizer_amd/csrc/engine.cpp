@@ -154,7 +154,9 @@ void host_kkt(const mpdhg_lp &lp, const HostLp &h, const std::vector<double> &x,
     r.rel_primal = std::max(rowv, colv) / (1.0 + bn);
     r.rel_dual = dv / (1.0 + cn);
     r.rel_gap = std::fabs(pobj - dobj) / (1.0 + std::fabs(pobj) + std::fabs(dobj));
-    r.rel_kkt = std::max(r.rel_primal, std::max(r.rel_dual, r.rel_gap));
+    r.rel_kkt = (std::isnan(r.rel_primal) || std::isnan(r.rel_dual) || std::isnan(r.rel_gap))
+                    ? r.rel_primal + r.rel_dual + r.rel_gap
+                    : std::max(r.rel_primal, std::max(r.rel_dual, r.rel_gap));
     r.dual_objective = dobj;
 }
 

@@ -67,9 +67,7 @@ struct EnhInst {
     int32_t checks;
     int32_t last_restart;    // iteration count at the last restart (Halpern k origin)
     int32_t restart_at;      // iteration index that takes z+ as iterate and anchor
-    int32_t round_start;     // iteration count at the start of the refinement round
     int32_t rounds;          // refinement rounds done
-    int32_t since_best;      // checks without a new best KKT in this round
     int32_t copy_best;       // flag: this check's point becomes the best point
     int32_t refine;          // flag: build the residual LP now
     int32_t max_iter;
@@ -82,7 +80,7 @@ struct EnhInst {
     double tol;
     double alpha_p, alpha_d;   // refinement scales of the current round
     double inv_ap, inv_ad;
-    double round_start_kkt, round_best;
+    double round_start_kkt;    // KKT at the last re-centring (refinement trigger)
     double best_kkt, best_eps_p, best_eps_d, best_eps_gap, best_pobj, best_dobj;
     double eps_p, eps_d, eps_gap, kkt, pobj, dobj;  // last check
     double pow_norm2;          // power iteration: |A^T A v|^2

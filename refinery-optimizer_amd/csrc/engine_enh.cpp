@@ -216,7 +216,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.cnorm = cinf;
         s.tol = tol;
         s.alpha_p = s.alpha_d = s.inv_ap = s.inv_ad = 1.0;
-        s.round_start_kkt = s.round_best = s.best_kkt = HUGE_VAL;
+        s.round_start_kkt = s.best_kkt = HUGE_VAL;
         s.pow_norm2 = pn;
         pow0[static_cast<size_t>(a)] = pn;
     });
@@ -314,7 +314,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                         d_cval64.get(), d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
                         d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
                         d_dr.get(), d_pr.get(), d_pc.get(), d_status.get(), cfg.refine_ratio,
-                        cfg.refine_patience, cfg.max_refinements, d_xbest.get(), d_lambest.get(),
+                        cfg.max_refinements, d_xbest.get(), d_lambest.get(),
                         d_ybest.get(), d_axbest.get(), d_xbase.get(), d_ybase.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_rls.get(),
                         d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get()};

@@ -321,7 +321,6 @@ struct FinalizeArgs {
     const double *pc, *pr;
     int32_t *status_out;
     double refine_ratio;
-    int patience;
     int max_refinements;
 };
 
@@ -381,7 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ s
     const double ep = fmax(rv, cv) / (1.0 + s.bnorm);
     const double ed = fmax(dv, sv) / (1.0 + s.cnorm);
     const double eg = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
-    const double kkt = fmax(ep, fmax(ed, eg));
+    // fmax drops NaN operands; a NaN anywhere must surface as NaN here
+    const double kkt = (isnan(ep) || isnan(ed) || isnan(eg)) ? ep + ed + eg : fmax(ep, fmax(ed, eg));
     s.checks += 1;
     s.eps_p = ep;
     s.eps_d = ed;
@@ -418,16 +418,32 @@ __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ s
         A.status_out[q] = 4;
         return;
     }
-    if (kkt < s.round_best) {
-        s.round_best = kkt;
-        s.since_best = 0;
-    } else {
-        s.since_best += 1;
+    if (s.round_start_kkt == HUGE_VAL) s.round_start_kkt = kkt;  // first check of the solve
+    const double r = sqrt(s.omega * dx2 + dy2 / s.omega);
+    const int kc = done - s.last_restart;
+    if (s.r_anchor < 0.0) {
+        s.r_anchor = r;
+        s.r_prev = r;
+        return;
     }
-    if (s.round_start_kkt == HUGE_VAL) s.round_start_kkt = kkt;
-    if (s.rounds < A.max_refinements && s.best_kkt < HUGE_VAL &&
-        (s.round_best <= A.refine_ratio * s.round_start_kkt || s.since_best >= A.patience)) {
-        const double dp = fmax(s.best_eps_p, s.best_eps_gap), dd = fmax(s.best_eps_d, s.best_eps_gap);
+    const bool rs = (r <= 0.2 * s.r_anchor) || (r <= 0.8 * s.r_anchor && r > s.r_prev) ||
+                    (static_cast<double>(kc) >= 0.36 * static_cast<double>(done));
+    s.r_prev = r;
+    if (!rs) return;
+    // restart to z+ = this check's point; primal weight from the anchor move
+    {
+        const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
+        if (ddx > 1e-10 && ddy > 1e-10) s.omega = sqrt(s.omega * ddy / ddx);
+    }
+    s.last_restart = done;
+    s.restarts += 1;
+    s.r_anchor = r;
+    if (s.rounds < A.max_refinements && kkt <= A.refine_ratio * s.round_start_kkt) {
+        // Refinement at the restart: the same point, re-centred in FP64 as the
+        // residual LP (x = x~ + D_c d / alpha_p, y = y~ + D_r e / alpha_d); the
+        // refine kernels build it from this check's buffers.  The primal weight
+        // is rescaled so that the effective weight is unchanged.
+        const double dp = fmax(ep, eg), dd = fmax(ed, eg);
         double ap = fmin(1.0 / fmax(dp, 1e-300), 1e12), ad = fmin(1.0 / fmax(dd, 1e-300), 1e12);
         if (ap < 1.0) ap = 1.0;
         if (ad < 1.0) ad = 1.0;
@@ -437,36 +453,12 @@ __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ s
         s.inv_ap = 1.0 / ap;
         s.inv_ad = 1.0 / ad;
         s.rounds += 1;
-        s.last_restart = done;
-        s.round_start = done;
-        s.round_start_kkt = s.best_kkt;
-        s.round_best = HUGE_VAL;
-        s.since_best = 0;
-        s.r_anchor = -1.0;
-        s.r_prev = -1.0;
+        s.round_start_kkt = kkt;
         s.refine = 1;
-        set_steps(s);
-        return;
-    }
-    const double r = sqrt(s.omega * dx2 + dy2 / s.omega);
-    const int kc = done - s.last_restart;
-    if (s.r_anchor < 0.0) {
-        s.r_anchor = r;
-        s.r_prev = r;
-        return;
-    }
-    const bool rs = (r <= 0.2 * s.r_anchor) || (r <= 0.8 * s.r_anchor && r > s.r_prev) ||
-                    (static_cast<double>(kc) >= 0.36 * static_cast<double>(done - s.round_start));
-    s.r_prev = r;
-    if (rs) {
-        const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
-        if (ddx > 1e-10 && ddy > 1e-10) s.omega = sqrt(s.omega * ddy / ddx);
-        s.last_restart = done;
+    } else {
         s.restart_at = done;
-        s.restarts += 1;
-        s.r_anchor = r;
-        set_steps(s);
     }
+    set_steps(s);
 }
 
 // ---- best-point copies and residual-LP construction -----------------------
@@ -505,13 +497,14 @@ __global__ __launch_bounds__(kBlock) void k_enh_best_rows(const Tile *__restrict
 }
 
 struct RefineColArgs {
-    const double *xbest, *lambest, *dc, *l64, *u64;
+    const double *xchk, *lam, *dc, *l64, *u64;
     double *xbase;
     float *c, *l, *u, *x, *x0, *xp;
 };
 
-// Residual LP of the best point (x~, y~): costs alpha_d * lambda~, bounds
-// alpha_p * ([l, u] - x~), in the scaled space; the correction restarts at 0.
+// Residual LP around this check's point (x~, y~): costs alpha_d * lambda~,
+// bounds alpha_p * ([l, u] - x~), row bounds alpha_p * ([rl, ru] - A x~), dual
+// offset alpha_d * y~, all in the scaled space; the correction restarts at 0.
 __global__ __launch_bounds__(kBlock) void k_enh_refine_cols(const Tile *__restrict__ tiles,
                                                             int ntiles, int chunk,
                                                             const EnhInst *__restrict__ st,
@@ -523,9 +516,9 @@ __global__ __launch_bounds__(kBlock) void k_enh_refine_cols(const Tile *__restri
     if (!s.refine) return;
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) {
-        const double xb = A.xbest[j], d = A.dc[j];
+        const double xb = A.xchk[j], d = A.dc[j];
         A.xbase[j] = xb;
-        A.c[j] = static_cast<float>(d * (s.alpha_d * A.lambest[j]));
+        A.c[j] = static_cast<float>(d * (s.alpha_d * A.lam[j]));
         const float lo = static_cast<float>(s.alpha_p * (A.l64[j] - xb) / d);
         const float hi = static_cast<float>(s.alpha_p * (A.u64[j] - xb) / d);
         A.l[j] = lo;
@@ -540,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_refine_cols(const Tile *__restri
 }
 
 struct RefineRowArgs {
-    const double *ybest, *axbest, *dr, *rl64, *ru64;
+    const double *ychk, *ax, *dr, *rl64, *ru64;
     double *ybase;
     float *rl, *ru, *ybar, *y, *y0, *yp;
 };
@@ -556,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_refine_rows(const Tile *__restri
     if (!s.refine) return;
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) {
-        const double yb = A.ybest[i], ab = A.axbest[i], d = A.dr[i];
+        const double yb = A.ychk[i], ab = A.ax[i], d = A.dr[i];
         A.ybase[i] = yb;
         A.rl[i] = static_cast<float>(s.alpha_p * (A.rl64[i] - ab) * d);
         A.ru[i] = static_cast<float>(s.alpha_p * (A.ru64[i] - ab) * d);
@@ -607,7 +600,7 @@ void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *s
                          a.pc};
     if (rts.count) hipLaunchKernelGGL(dev::k_enh_chk_rows, MPDHG_TILE_GRID(rts), st, R);
     if (cts.count) hipLaunchKernelGGL(dev::k_enh_chk_cols, MPDHG_TILE_GRID(cts), st, Cc);
-    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.patience, a.max_refinements};
+    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.max_refinements};
     hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, it);
     if (cts.count)
         hipLaunchKernelGGL(dev::k_enh_best_cols, MPDHG_TILE_GRID(cts), st, a.xchk, a.lam, a.xbest,
@@ -616,9 +609,9 @@ void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *s
         hipLaunchKernelGGL(dev::k_enh_best_rows, MPDHG_TILE_GRID(rts), st, a.ychk, a.ax, a.ybest,
                            a.axbest);
     if (a.max_refinements > 0) {
-        dev::RefineColArgs RC{a.xbest, a.lambest, a.dc, a.l64, a.u64, a.xbase,
+        dev::RefineColArgs RC{a.xchk, a.lam, a.dc, a.l64, a.u64, a.xbase,
                               a.cs, a.ls, a.us, a.x, a.x0, a.xp};
-        dev::RefineRowArgs RR{a.ybest, a.axbest, a.dr, a.rl64, a.ru64, a.ybase,
+        dev::RefineRowArgs RR{a.ychk, a.ax, a.dr, a.rl64, a.ru64, a.ybase,
                               a.rls, a.rus, a.ybar, a.y, a.y0, a.yp};
         if (cts.count) hipLaunchKernelGGL(dev::k_enh_refine_cols, MPDHG_TILE_GRID(cts), st, RC);
         if (rts.count) hipLaunchKernelGGL(dev::k_enh_refine_rows, MPDHG_TILE_GRID(rts), st, RR);

@@ -66,7 +66,7 @@ struct EnhCheck {
     double *pr, *pc;
     int32_t *status_out;
     double refine_ratio;
-    int patience, max_refinements;
+    int max_refinements;
     double *xbest, *lambest, *ybest, *axbest, *xbase, *ybase;
     float *cs, *ls, *us, *x, *x0, *xp, *rls, *rus, *ybar, *y, *y0, *yp;
 };

@@ -291,8 +291,7 @@ class GpuPdhgConfig:
     ruiz_iterations: int = 10
     profile: bool = False
     dual_sign: int = 0
-    refine_ratio: float = 1e-4
-    refine_patience: int = 8
+    refine_ratio: float = 1e-2
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,

@@ -35,7 +35,6 @@ class MpdhgConfig(C.Structure):
         ("profile", C.c_int32),
         ("dual_sign", C.c_int32),
         ("refine_ratio", C.c_double),
-        ("refine_patience", C.c_int32),
     ]
 
 

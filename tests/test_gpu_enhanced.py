@@ -71,7 +71,7 @@ def test_full_solve_tracks_cpu_port(need_gpu):
     p = S.gen_staircase(2, 400, 900, 1)
     sol = S.solve_gpu_pdhg(p, enh(tol=1e-6))
     ref = port(p, sol, tol=1e-6, max_iterations=100000, check_frequency=64, max_refinements=8,
-               refine_ratio=1e-4, patience=8)
+               refine_ratio=1e-2)
     assert ref["status"] == 0 and sol.status == S.SolutionStatus.Optimal
     assert abs(sol.primal_objective - ref["primal_obj"]) <= 1e-5 * (1 + abs(ref["primal_obj"]))
     assert 0.5 <= sol.simplex_iterations / ref["iterations"] <= 2.0
