@@ -101,14 +101,14 @@ def bench_sweep(args):
     wall = sum(walls) / len(walls)
     rf, spmv = roofline(sols[0])
     # plain: the same engine, one LP per launch sequence, on every 8th LP
-    sample = probs[::8]
+    sample = probs[::8] if not args.no_plain else []
     cfg.profile = False
     t_plain, ok_plain = 0.0, 0
     for p in sample:
         s1 = S.solve_gpu_pdhg(p, cfg)
         t_plain += s1.time_solver_sec
         ok_plain += int(s1.status == S.SolutionStatus.Optimal and s1.rel_kkt <= tol)
-    plain_rate = ok_plain / t_plain
+    plain_rate = ok_plain / t_plain if t_plain > 0 else float("nan")
     log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s; "
         f"batched {conv / dev:.1f} LPs/s ({conv / dev / plain_rate:.2f}x)")
     out = {
@@ -178,6 +178,7 @@ def main():
     ap.add_argument("--workload", default="sweep512", choices=["sweep512", "large-ref", "large-enh"])
     ap.add_argument("--cpu-lps", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-LP-per-call comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the "

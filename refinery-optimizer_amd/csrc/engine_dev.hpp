@@ -8,7 +8,10 @@ namespace mpdhg {
 // Workgroup shape of every tile kernel: 4 wave64s, one row per lane in the
 // epilogue, kTileNnz nonzeros staged through LDS per chunk.
 constexpr int kBlock = 256;
-constexpr int kTileNnz = 4096;
+#ifndef MPDHG_TILE_NNZ
+#define MPDHG_TILE_NNZ 4096
+#endif
+constexpr int kTileNnz = MPDHG_TILE_NNZ;
 constexpr int kPerThread = kTileNnz / kBlock;
 
 // A tile: rows [begin, end) of one instance's compressed structure

@@ -270,8 +270,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
         d_status.upload(run, st);
     }
-    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
-    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
+    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
+    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -326,21 +326,26 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             ctx.events.push_back(e);
         }
     }
-    const int stride = nsamp ? std::max(1, K / nsamp) : 1;
+    // Samples are the first non-check iterations after the first cycle, while
+    // every instance of the batch still runs, so that bytes_*_launch (computed
+    // for the whole batch) matches what each sampled launch streams.
     int used = 0;
     Pinned pin(2 * static_cast<size_t>(na) * sizeof(int32_t));
     int32_t *slots = static_cast<int32_t *>(pin.p);
     hipEvent_t poll_ev[2];
     MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
     MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
-    int it = 0, cycle = 0;
+    int it = 0, cycle = 0, covered = na;
     bool finished = false;
+    std::vector<DBuf<Tile>> keep_rt, keep_ct;  // compacted tile lists, alive until the end
+    std::vector<DBuf<int32_t>> keep_rng;
+    std::vector<std::vector<int32_t>> host_keep;
     try {
         while (it < K && !finished) {
             const int end = std::min(K, (it / F + 1) * F);
             for (; it < end; ++it) {
                 const int check = (it + 1 == end);
-                const bool samp = nsamp && used < nsamp && (it % stride == 0) && !check;
+                const bool samp = nsamp && used < nsamp && it >= F && covered == na && !check;
                 if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
                 launch::enh_primal(st, cts, d_inst.get(), P, it, check);
                 if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
@@ -359,8 +364,37 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             if (cycle > 0) {  // inspect the previous cycle while this one runs
                 const int prev = slot ^ 1;
                 MPDHG_HIP(hipEventSynchronize(poll_ev[prev]));
-                finished = std::none_of(slots + prev * na, slots + prev * na + na,
-                                        [](int32_t v) { return v == kRunning; });
+                const int32_t *sv = slots + prev * na;
+                const int running = static_cast<int>(std::count(sv, sv + na, kRunning));
+                finished = (running == 0);
+                if (!finished && 2 * running <= covered) {
+                    // Compact: later launches cover only the running instances'
+                    // tiles.  Stream order puts the range update after the cycle
+                    // already in flight, which still uses the old lists.
+                    std::vector<Tile> nrt, nct;
+                    std::vector<int32_t> rng(4 * static_cast<size_t>(na), 0);
+                    for (int a = 0; a < na; ++a) {
+                        if (sv[a] != kRunning) continue;
+                        const EnhInst &s0 = hh.inst[static_cast<size_t>(a)];
+                        rng[4 * static_cast<size_t>(a)] = static_cast<int32_t>(nrt.size());
+                        nrt.insert(nrt.end(), hh.rtiles.begin() + s0.row_tile_begin, hh.rtiles.begin() + s0.row_tile_end);
+                        rng[4 * static_cast<size_t>(a) + 1] = static_cast<int32_t>(nrt.size());
+                        rng[4 * static_cast<size_t>(a) + 2] = static_cast<int32_t>(nct.size());
+                        nct.insert(nct.end(), hh.ctiles.begin() + s0.col_tile_begin, hh.ctiles.begin() + s0.col_tile_end);
+                        rng[4 * static_cast<size_t>(a) + 3] = static_cast<int32_t>(nct.size());
+                    }
+                    keep_rt.emplace_back();
+                    keep_rt.back().upload(nrt, st);
+                    keep_ct.emplace_back();
+                    keep_ct.back().upload(nct, st);
+                    keep_rng.emplace_back();
+                    keep_rng.back().upload(rng, st);
+                    launch::enh_set_ranges(st, d_inst.get(), na, keep_rng.back().get());
+                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()));
+                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()));
+                    covered = running;
+                    host_keep.push_back(std::move(rng));
+                }
             }
             ++cycle;
         }

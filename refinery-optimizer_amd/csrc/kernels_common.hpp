@@ -49,21 +49,36 @@ constexpr int kStageBytes = kTileNnz * 8;
 // own row from LDS.  A row longer than a chunk is continued across chunks
 // by the same lane, so the summation order never changes.  `lds` must hold
 // kStageBytes.
+// The row-pointer reads of a tile: issued first, so that a kernel can put
+// its other independent loads (epilogue operands, instance state) in flight
+// before it consumes them.
+struct TileRows {
+    int k0, k1, lo, hi;
+};
+
+__device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restrict__ ptr) {
+    TileRows t;
+    const int r = tl.begin + static_cast<int>(threadIdx.x);
+    t.k0 = 0;
+    t.k1 = 0;
+    if (r < tl.end) {
+        t.k0 = ptr[r];
+        t.k1 = ptr[r + 1];
+    }
+    t.lo = ptr[tl.begin];
+    t.hi = ptr[tl.end];
+    return t;
+}
+
 template <typename V, typename X, typename A>
-__device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
-                                        const int *__restrict__ idx, const V *__restrict__ val,
-                                        const X *__restrict__ xs, void *lds_raw) {
+__device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
+                                           const V *__restrict__ val, const X *__restrict__ xs,
+                                           void *lds_raw) {
     constexpr int CH = kStageBytes / static_cast<int>(sizeof(Pair<V, X>));
     constexpr int PT = CH / kBlock;
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
-    const int r = tl.begin + tid;
-    int k0 = 0, k1 = 0;
-    if (r < tl.end) {
-        k0 = ptr[r];
-        k1 = ptr[r + 1];
-    }
-    const int lo = ptr[tl.begin], hi = ptr[tl.end];
+    const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi;
     A sum = A(0);
     for (int c0 = lo; c0 < hi; c0 += CH) {
         const int c1 = min(hi, c0 + CH);
@@ -97,6 +112,13 @@ __device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ 
         __syncthreads();
     }
     return sum;
+}
+
+template <typename V, typename X, typename A>
+__device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
+                                        const int *__restrict__ idx, const V *__restrict__ val,
+                                        const X *__restrict__ xs, void *lds_raw) {
+    return tile_dot_rows<V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
 }
 
 template <typename A>

@@ -105,28 +105,35 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
+    // Put every load that does not depend on the instance state in flight
+    // before the state is consumed: row pointers and epilogue operands.
+    const TileRows tr = tile_rows(tl, A.cptr);
+    const int j = tl.begin + static_cast<int>(threadIdx.x);
+    const bool mine = j < tl.end;
+    float xo = 0.0f, x0j = 0.0f, cj = 0.0f, lo = 0.0f, hi = 0.0f;
+    if (mine) {
+        xo = A.x[j];
+        x0j = A.x0[j];
+        cj = A.c[j];
+        lo = A.l[j];
+        hi = A.u[j];
+    }
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot<float>(tl, A.cptr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    const float g = tile_dot_rows<float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
-    const int j = tl.begin + static_cast<int>(threadIdx.x);
     double d2 = 0.0, e2 = 0.0;
-    if (j < tl.end) {
-        float xo, x0j;
-        if (restart) {
+    if (mine) {
+        if (restart) {  // z+ of the last check becomes iterate and anchor
             xo = A.xp[j];
             x0j = xo;
             A.x0[j] = x0j;
-        } else {
-            xo = A.x[j];
-            x0j = A.x0[j];
         }
-        const float tt = A.c[j] + g;
+        const float tt = cj + g;
         float v = __builtin_fmaf(-tau, tt, xo);
-        const float lo = A.l[j], hi = A.u[j];
         if (v < lo) v = lo;
         if (v > hi) v = hi;
         A.xbar[j] = 2.0f * v - xo;
@@ -172,30 +179,34 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
+    const TileRows tr = tile_rows(tl, A.rptr);
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    const bool mine = i < tl.end;
+    float yo = 0.0f, y0i = 0.0f, yb = 0.0f, lo = 0.0f, hi = 0.0f;
+    if (mine) {
+        yo = A.y[i];
+        y0i = A.y0[i];
+        yb = A.ybar[i];
+        lo = A.rl[i];
+        hi = A.ru[i];
+    }
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot<float>(tl, A.rptr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
-    const int i = tl.begin + static_cast<int>(threadIdx.x);
     double d2 = 0.0, e2 = 0.0;
-    if (i < tl.end) {
-        float yo, y0i;
+    if (mine) {
         if (restart) {
             yo = A.yp[i];
             y0i = yo;
             A.y0[i] = y0i;
-        } else {
-            yo = A.y[i];
-            y0i = A.y0[i];
         }
         // y+ = sigma (w - clamp(w, rl, ru)) - ybar, w = h + (y + ybar) / sigma:
         // y + sigma (h - bound) on an active bound, -ybar in between.
-        const float yb = A.ybar[i];
         const float w = __builtin_fmaf(yo + yb, inv_sigma, h);
-        const float lo = A.rl[i], hi = A.ru[i];
         float yn = -yb;
         if (w < lo) yn = __builtin_fmaf(sigma, h - lo, yo);
         else if (w > hi) yn = __builtin_fmaf(sigma, h - hi, yo);
@@ -560,10 +571,25 @@ __global__ __launch_bounds__(kBlock) void k_enh_refine_rows(const Tile *__restri
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_enh_set_ranges(EnhInst *__restrict__ st, int ninst,
+                                                           const int32_t *__restrict__ rng) {
+    const int q = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (q >= ninst) return;
+    st[q].row_tile_begin = rng[4 * q];
+    st[q].row_tile_end = rng[4 * q + 1];
+    st[q].col_tile_begin = rng[4 * q + 2];
+    st[q].col_tile_end = rng[4 * q + 3];
+}
+
 }  // namespace dev
 
 // ---- launchers --------------------------------------------------------------
 namespace launch {
+
+void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng) {
+    hipLaunchKernelGGL(dev::k_enh_set_ranges, dim3((ninst + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       s, st, ninst, rng);
+}
 
 #define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
 

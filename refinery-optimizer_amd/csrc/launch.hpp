@@ -81,6 +81,9 @@ void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhP
                 int check);
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
               int check);
+// Point instance q's partial-reduction ranges at rng[4q..4q+3] (row tile
+// begin/end, column tile begin/end) after a tile-list compaction.
+void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)
 // residual-LP construction, in that stream order.
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
