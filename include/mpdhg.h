@@ -145,6 +145,35 @@ int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
 int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count,
                       const mpdhg_config *cfg, mpdhg_result *res);
 
+/* Adjoint of the PDHG fixed point (extension; no reference counterpart).
+ * For L(x*) with upstream gradient w = dL/dx* at a solution (x, y) of `lp`:
+ * columns strictly inside their bounds form B, rows with nonzero duals form
+ * R, and the transposed fixed-point condition of the projected PDHG step on
+ * (B, R) reduces to A_RB^T mu = w_B, solved per instance on the GPU by CGLS
+ * (min-norm least squares in the scaled space).  Outputs:
+ *   dl_dc = 0 (the solution is piecewise constant in c on the active set),
+ *   dl_db[i] = mu_i: derivative w.r.t. a shift of row i's bounds (rl and ru
+ *              move together; only the active one matters),
+ *   dl_dl[j] / dl_du[j] = w_j - (A^T mu)_j for columns held at that bound.
+ * Any output pointer may be NULL. */
+typedef struct mpdhg_adjoint_io {
+    const double *x;   /* [n] forward primal solution */
+    const double *y;   /* [m] forward row duals, reference sign (c + A^T y) */
+    const double *w;   /* [n] dL/dx */
+    double *dl_dc;     /* [n] */
+    double *dl_db;     /* [m] */
+    double *dl_dl;     /* [n] */
+    double *dl_du;     /* [n] */
+    int32_t basic_cols, active_rows, iterations; /* out */
+    double residual;   /* out: final |K^T r| / |K^T w| of CGLS */
+} mpdhg_adjoint_io;
+
+/* active_tol: a column is basic when both bound distances exceed
+ * active_tol * (1 + |x_j|); a row is active when |y_i| > active_tol *
+ * (1 + max |y|).  tol: CGLS stopping tolerance; max_iter: CGLS iterations. */
+int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
+                        double active_tol, double tol, int max_iter);
+
 /* Basis identification of GpuSolver::solve_hybrid (gpu_solver.cpp:319-395):
  * the m most interior of the n columns and m row activities become Basic,
  * the rest sit at the nearer bound.  Ties keep index order (the reference's

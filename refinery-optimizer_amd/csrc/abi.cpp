@@ -165,6 +165,25 @@ int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdh
     return rc;
 }
 
+int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
+                        double active_tol, double tol, int max_iter) {
+    if (count < 0 || (count > 0 && (!lps || !io))) return mpdhg::fail(MPDHG_E_ARG, "bad batch");
+    mpdhg_ctx *tmp = nullptr;
+    int rc = MPDHG_OK;
+    if (!ctx) {
+        rc = make_ctx(0, &tmp);
+        if (rc) return rc;
+        ctx = tmp;
+    }
+    rc = guarded([&] {
+        MPDHG_HIP(hipSetDevice(ctx->c.device));
+        mpdhg::solve_adjoint(ctx->c, lps, count, io, active_tol, tol, max_iter);
+        return MPDHG_OK;
+    });
+    delete tmp;
+    return rc;
+}
+
 int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg, mpdhg_result *res) {
     if (!lp || !res) return mpdhg::fail(MPDHG_E_ARG, "NULL lp or result");
     return mpdhg_solve_batch(ctx, lp, 1, cfg, res);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -100,6 +101,15 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
                      mpdhg_result *res);
 void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
                     mpdhg_result *res);
+void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
+                   double active_tol, double tol, int max_iter);
+
+// Ruiz equilibration (infinity norm, `ruiz` passes) then one Pock-Chambolle
+// pass (alpha = 1): D_r A D_c has || . ||_2 <= 1.
+void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, double *dc);
+
+// Runs f(i) for i in [0, count) on up to hardware_concurrency threads.
+void parallel_for_each(int count, const std::function<void(int)> &f);
 
 // FP64 post-processing on the host for reference-mode results: A x,
 // c + A^T y and the relative KKT of (x, y).

@@ -89,4 +89,13 @@ struct EnhInst {
     double pow_norm2;          // power iteration: |A^T A v|^2
 };
 
+// Adjoint (CGLS on the active-set system) per-instance state.
+struct AdjInst {
+    int32_t done;
+    int32_t iters;
+    int32_t row_tile_begin, row_tile_end;
+    int32_t col_tile_begin, col_tile_end;
+    double gamma, gamma0, alpha, beta;
+};
+
 }  // namespace mpdhg

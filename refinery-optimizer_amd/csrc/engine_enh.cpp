@@ -29,8 +29,8 @@ double finite_or_zero(float a, float b) {
     return std::isfinite(a) ? static_cast<double>(a) : (std::isfinite(b) ? static_cast<double>(b) : 0.0);
 }
 
-// Ruiz equilibration (infinity norm) then one Pock-Chambolle pass (alpha = 1,
-// l1 norms); the scaled matrix is D_r A D_c with || . ||_2 <= 1.
+}  // namespace
+
 void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, double *dc) {
     const int m = h.m, n = h.n;
     std::vector<double> rs(static_cast<size_t>(m)), cs(static_cast<size_t>(n));
@@ -61,6 +61,8 @@ void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, d
     }
 }
 
+namespace {
+
 template <typename F>
 void parallel_for(int count, F &&f) {
     const int nt = std::max(1, std::min<int>(count, static_cast<int>(std::thread::hardware_concurrency())));
@@ -76,6 +78,12 @@ void parallel_for(int count, F &&f) {
         });
     for (auto &th : pool) th.join();
 }
+
+}  // namespace
+
+void parallel_for_each(int count, const std::function<void(int)> &f) { parallel_for(count, f); }
+
+namespace {
 
 struct Pinned {
     void *p = nullptr;

@@ -89,5 +89,29 @@ void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng);
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
                const EnhCheck &a, int it);
 
+// adjoint: CGLS on A_RB^T mu = w_B in the scaled space
+struct AdjColArgs {
+    const int *cptr, *cidx;
+    const double *cval;
+    const double *pt;    // gathered: mR * dr * p (or dr * mu for the final pass)
+    const double *mb, *dc;
+    double *q, *r, *rt;
+    double *pc;
+};
+struct AdjRowArgs {
+    const int *rptr, *ridx;
+    const double *rval;
+    const double *rt;    // gathered: mB * dc * r
+    const double *mr, *dr;
+    double *s, *mu, *p, *pt;
+    double *pr;
+};
+void adj_start(hipStream_t s, const TileSet &rts, AdjInst *st, int ninst, const AdjRowArgs &r,
+               double tol, int max_iter);
+void adj_step(hipStream_t s, const TileSet &rts, const TileSet &cts, AdjInst *st, int ninst,
+              const AdjColArgs &c, const AdjRowArgs &r, double tol, int max_iter);
+void adj_atmu(hipStream_t s, const TileSet &rts, const TileSet &cts, const AdjColArgs &c,
+              const AdjRowArgs &r);
+
 }  // namespace launch
 }  // namespace mpdhg

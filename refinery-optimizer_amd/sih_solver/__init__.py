@@ -426,6 +426,47 @@ class GpuSolver:
         return solve_batch(problems, config)
 
 
+@dataclass
+class AdjointResult:
+    """Gradients of L = w . x* at an LP solution (extension; see mpdhg_adjoint_batch)."""
+
+    dl_dc: np.ndarray
+    dl_db: np.ndarray   # w.r.t. a shift of each row's bounds (rl and ru together)
+    dl_dl: np.ndarray   # columns held at their lower bound
+    dl_du: np.ndarray   # columns held at their upper bound
+    basic_cols: int
+    active_rows: int
+    iterations: int
+    residual: float
+
+
+def adjoint_batch(problems: Sequence[Problem], xs, ys, ws, active_tol: float = 1e-7,
+                  tol: float = 1e-10, max_iter: int = 20000) -> List[AdjointResult]:
+    """Batched adjoint on the GPU: for each LP, the gradients of w . x* at the
+    solution (x, y) (row duals in the reference sign, as solve_* return them)."""
+    nb = len(problems)
+    lps = (MpdhgLp * max(nb, 1))()
+    io = (_abi.MpdhgAdjointIO * max(nb, 1))()
+    keep, outs = [], []
+    dp = C.POINTER(C.c_double)
+    for q, p in enumerate(problems):
+        lp, arrs = p._c_view()
+        lps[q] = lp
+        ins = [np.ascontiguousarray(a, np.float64) for a in (xs[q], ys[q], ws[q])]
+        n, m = p.num_cols(), p.num_rows()
+        if ins[0].size != n or ins[1].size != m or ins[2].size != n:
+            raise ValueError("x, y, w sizes do not match the problem")
+        o = (np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(n))
+        keep.append((arrs, ins))
+        outs.append(o)
+        io[q].x, io[q].y, io[q].w = (a.ctypes.data_as(dp) for a in ins)
+        io[q].dl_dc, io[q].dl_db, io[q].dl_dl, io[q].dl_du = (a.ctypes.data_as(dp) for a in o)
+    _abi.check(_abi.lib().mpdhg_adjoint_batch(_ctx()._p, lps, nb, io, active_tol, tol, max_iter),
+               "mpdhg_adjoint_batch")
+    return [AdjointResult(*outs[q], io[q].basic_cols, io[q].active_rows, io[q].iterations,
+                          io[q].residual) for q in range(nb)]
+
+
 def identify_basis(problem: Problem, x) -> tuple:
     """Basis statuses from a primal point (gpu_solver.cpp:319-395), as int8 arrays."""
     lp, keep = problem._c_view()

@@ -86,6 +86,22 @@ class MpdhgResult(C.Structure):
     ]
 
 
+class MpdhgAdjointIO(C.Structure):
+    _fields_ = [
+        ("x", C.POINTER(C.c_double)),
+        ("y", C.POINTER(C.c_double)),
+        ("w", C.POINTER(C.c_double)),
+        ("dl_dc", C.POINTER(C.c_double)),
+        ("dl_db", C.POINTER(C.c_double)),
+        ("dl_dl", C.POINTER(C.c_double)),
+        ("dl_du", C.POINTER(C.c_double)),
+        ("basic_cols", C.c_int32),
+        ("active_rows", C.c_int32),
+        ("iterations", C.c_int32),
+        ("residual", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); exactly the symbols include/mpdhg.h declares.
 _VP = C.c_void_p
 SIGNATURES = {
@@ -100,6 +116,8 @@ SIGNATURES = {
                               C.POINTER(MpdhgResult)]),
     "mpdhg_solve_batch": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgConfig),
                                     C.POINTER(MpdhgResult)]),
+    "mpdhg_adjoint_batch": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgAdjointIO),
+                                      C.c_double, C.c_double, C.c_int]),
     "mpdhg_identify_basis": (C.c_int, [C.POINTER(MpdhgLp), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
     "mpdhg_gen_random": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_uint64, C.POINTER(_VP)]),
