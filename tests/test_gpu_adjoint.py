@@ -71,10 +71,16 @@ def test_adjoint_closed_form_and_finite_differences(need_gpu):
             h = 1e-4 * max(1.0, np.abs(np.where(np.isfinite(p.row_upper), p.row_upper, 0)).max())
             fd = (L_highs(p, w, db=h * d) - L_highs(p, w, db=-h * d)) / (2 * h)
             assert abs(fd - g @ d) <= 1e-4 * max(abs(fd), np.linalg.norm(g) * 1e-3), (p.name, fd, g @ d)
-            dcv = rng.normal(size=p.num_cols())
+            # x* is locally constant in c only where no nonbasic reduced cost is ~0
+            # (the random-equality family zeroes 40% of them: dual degenerate)
+            lam = s.reduced_costs
+            at_bound = (s.x - p.col_lower <= 1e-7 * (1 + abs(s.x))) | (p.col_upper - s.x <= 1e-7 * (1 + abs(s.x)))
             hc = 1e-4 * max(1.0, np.abs(p.c).max())
-            fdc = (L_highs(p, w, dc=hc * dcv) - L_highs(p, w, dc=-hc * dcv)) / (2 * hc)
-            assert abs(fdc) <= 1e-6 * (1 + np.abs(w).sum()), (p.name, fdc)
+            if np.abs(lam[at_bound]).min(initial=np.inf) > 10 * hc * np.sqrt(p.num_cols()):
+                dcv = rng.normal(size=p.num_cols())
+                dcv /= np.linalg.norm(dcv)
+                fdc = (L_highs(p, w, dc=hc * dcv) - L_highs(p, w, dc=-hc * dcv)) / (2 * hc)
+                assert abs(fdc) <= 1e-6 * (1 + np.abs(w).sum()), (p.name, fdc)
         checked += 1
     assert checked >= 1
 
