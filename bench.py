@@ -15,16 +15,24 @@ Other workloads:
   large-ref  one random equality LP m=500,000 n=1,200,000 10 nnz/column,
              reference-mode PDHG; a step is one PDHG iteration.
   large-enh  the same LP, enhanced mode to 1e-4 relative KKT.
+  copy       a 256 MiB device-to-device elementwise copy (torch), the
+             calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
 
 The roofline object reports the dominant fused SpMV kernel: algorithmic
 bytes per launch / its average launch duration from HIP events recorded on
 the solve's stream inside the timed solve; peak = MI355X HBM3E 8 TB/s.
+`traffic` is the HBM bytes per launch of that kernel measured with the
+FETCH_SIZE and WRITE_SIZE counters (two rocprofv3 --pmc passes over
+`bench.py --max-iterations 128`, every instance active), scaled by what the
+counters read for the copy workload; the figures are read from
+profiles/pmc_traffic_sweep512.md, which records the commands.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -41,7 +49,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(sol, kernels=("k_enh_primal", "k_enh_dual")):
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_sweep512.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, or None.
+
+    The summary holds one line `traffic_bytes <kernel> <bytes>` per kernel
+    (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor, in bytes)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(rf"^traffic_bytes {re.escape(kernel)} (\d+)\s*$", TRAFFIC_FILE.read_text(),
+                  re.MULTILINE)
+    return float(m.group(1)) if m else None
+
+
+def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=True):
     n_l = max(sol.kernel_launches_csc, 1)
     csc_us = 1e3 * sol.kernel_ms_spmv_csc / n_l
     csr_us = 1e3 * sol.kernel_ms_spmv_csr / n_l
@@ -52,8 +75,8 @@ def roofline(sol, kernels=("k_enh_primal", "k_enh_dual")):
     else:
         name, us, b, g = kernels[1], csr_us, sol.bytes_csr_launch, gbs_csr
     return ({"bound": "hbm", "kernel": name, "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": g / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b,
-             "us_per_launch": us},
+             "frac": g / HBM_PEAK_GBS, "traffic": measured_traffic(name) if traffic else None,
+             "bytes_per_launch": b, "us_per_launch": us},
             {"csc_fused_gbs": gbs_csc, "csr_fused_gbs": gbs_csr, "csc_fused_us": csc_us,
              "csr_fused_us": csr_us})
 
@@ -83,7 +106,7 @@ def bench_sweep(args):
     probs = W.scenario_sweep(512)
     nnz = sum(p.num_nonzeros() for p in probs)
     log(f"generated 512 LPs ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
-    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=100000)
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=args.max_iterations)
     for w in range(args.warmup):
         S.solve_batch(probs, cfg)
         log(f"warmup step {w} done")
@@ -108,29 +131,35 @@ def bench_sweep(args):
         s1 = S.solve_gpu_pdhg(p, cfg)
         t_plain += s1.time_solver_sec
         ok_plain += int(s1.status == S.SolutionStatus.Optimal and s1.rel_kkt <= tol)
-    plain_rate = ok_plain / t_plain if t_plain > 0 else float("nan")
-    log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s; "
-        f"batched {conv / dev:.1f} LPs/s ({conv / dev / plain_rate:.2f}x)")
+    plain_rate = ok_plain / t_plain if t_plain > 0 and ok_plain else None
+    if plain_rate:
+        log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s; "
+            f"batched {conv / dev:.1f} LPs/s ({conv / dev / plain_rate:.2f}x)")
+    capped = args.max_iterations < 100000
     out = {
         "metric": METRIC, "value": conv / dev, "unit": "LPs/s", "ms_per_step": 1e3 * dev,
         "dtype": "fp32 iterates, fp64 KKT/refinement",
         "config": {"workload": "sweep512: 512 scenario LPs, m log-uniform [200, 20000], n=2.4m, "
-                               f"{nnz} nnz, FP64 rel. KKT <= 1e-6, step = one batched solve",
-                   "lps": 512, "tol": tol},
+                               f"{nnz} nnz, FP64 rel. KKT <= 1e-6, step = one batched solve"
+                               + (f" (profiling run capped at {args.max_iterations} iterations: "
+                                  "not a result)" if capped else ""),
+                   "lps": 512, "tol": tol, "max_iterations": args.max_iterations},
         "converged": conv,
         "wall_lps_per_s": conv / wall,
-        "plain": {"lps_per_s": plain_rate, "sample": f"every 8th LP ({len(sample)} LPs), one "
-                                                     "solve call each", "converged": ok_plain,
-                  "seconds": t_plain},
-        "batched_over_plain": (conv / dev) / plain_rate,
+        "plain": None if plain_rate is None else {
+            "lps_per_s": plain_rate, "sample": f"every 8th LP ({len(sample)} LPs), one solve "
+                                               "call each", "converged": ok_plain,
+            "seconds": t_plain},
+        "batched_over_plain": (conv / dev) / plain_rate if plain_rate else None,
         "roofline": rf, "spmv": spmv,
     }
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
-        cs = probs[::16][: args.cpu_lps]
+        stride = max(1, len(probs) // max(args.cpu_lps, 1))
+        cs = probs[::stride][: args.cpu_lps]
         ok, dt = cpu_port_sample(cs, tol, cores)
         out["cpu_baseline"] = {"value": ok / dt, "unit": "LPs/s", "cores": cores, "kind": "port",
-                               "sample": f"{len(cs)} LPs of the sweep (every 16th), CPU port of the "
+                               "sample": f"{len(cs)} LPs of the sweep (every {stride}th), CPU port of the "
                                          f"same algorithm, {cores} LPs in parallel, {dt:.1f}s, "
                                          f"{ok} converged"}
     return out
@@ -170,13 +199,42 @@ def bench_large(args, mode):
     return out
 
 
+def bench_copy(args):
+    """256 MiB float32 device copy: reads 256 MiB and writes 256 MiB per launch."""
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the copy workload needs a HIP device")
+    n = 64 << 20
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    gbs = 2 * 4 * n * args.steps / dt / 1e9
+    return {"metric": "device copy GB/s (counter calibration)", "value": gbs, "unit": "GB/s",
+            "ms_per_step": 1e3 * dt / args.steps, "dtype": "f32",
+            "config": {"workload": "copy: 256 MiB read + 256 MiB write per launch"},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": 2.0 * 4 * n}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sweep512", choices=["sweep512", "large-ref", "large-enh"])
-    ap.add_argument("--cpu-lps", type=int, default=32)
+    ap.add_argument("--workload", default="sweep512",
+                    choices=["sweep512", "large-ref", "large-enh", "copy"])
+    ap.add_argument("--max-iterations", type=int, default=100000,
+                    help="sweep512 iteration cap (a small cap makes a short profiling run in "
+                         "which every instance stays active; its value is then not a result)")
+    ap.add_argument("--cpu-lps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-LP-per-call comparison")
     args = ap.parse_args()
@@ -186,6 +244,8 @@ def main():
         sys.exit(2)
     if args.workload == "sweep512":
         out = bench_sweep(args)
+    elif args.workload == "copy":
+        out = bench_copy(args)
     else:
         out = bench_large(args, "ref" if args.workload == "large-ref" else "enh")
     out.setdefault("higher_is_better", True)
