@@ -77,6 +77,10 @@ typedef struct mpdhg_config {
     double refine_ratio;          /* enhanced: at a restart, re-centre the FP32 iterates on the
                                      restart point (one refinement round) once the FP64 KKT fell
                                      below ratio * its value at the last re-centring (1e-2) */
+    int32_t spmv_layout;          /* 0: auto -- each chunk of <= 4096 nonzeros stored sorted by
+                                     gathered index where the instance's vectors have < 2^20
+                                     entries; 1: storage order everywhere.  Results are
+                                     bit-identical either way (same per-row summation order). */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite

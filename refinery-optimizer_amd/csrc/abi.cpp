@@ -47,6 +47,8 @@ int check_config(const mpdhg_config *cfg) {
     if (cfg->mode != MPDHG_MODE_REFERENCE && cfg->mode != MPDHG_MODE_ENHANCED)
         return mpdhg::fail(MPDHG_E_ARG, "unknown mode");
     if (cfg->check_frequency <= 0) return mpdhg::fail(MPDHG_E_ARG, "check_frequency must be > 0");
+    if (cfg->spmv_layout != 0 && cfg->spmv_layout != 1)
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_layout must be 0 (auto) or 1 (storage order)");
     return MPDHG_OK;
 }
 
@@ -105,6 +107,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->profile = 0;
     cfg->dual_sign = 0;
     cfg->refine_ratio = 1e-2;
+    cfg->spmv_layout = 0;
 }
 
 int mpdhg_device_count(int *count) {

@@ -91,7 +91,7 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int 
             ++i;
         }
         out.push_back(Tile{static_cast<int32_t>(global_off + start),
-                           static_cast<int32_t>(global_off + i), inst, 0});
+                           static_cast<int32_t>(global_off + i), inst, -1});
     }
 }
 
@@ -292,6 +292,17 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
         plan_tiles(h.cptr, h.n, co, a, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
+    }
+    {
+        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
+        for (int a = 0; a < na; ++a) {
+            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+            rlen[static_cast<size_t>(a)] = h.m;
+            clen[static_cast<size_t>(a)] = h.n;
+        }
+        const Layout lay = static_cast<Layout>(cfg.spmv_layout);
+        gather_order(lay, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
+        gather_order(lay, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
     }
     // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
     hipStream_t st = ctx.stream;

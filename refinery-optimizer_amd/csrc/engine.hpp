@@ -3,10 +3,12 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine_dev.hpp"
@@ -110,6 +112,60 @@ void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, d
 
 // Runs f(i) for i in [0, count) on up to hardware_concurrency threads.
 void parallel_for_each(int count, const std::function<void(int)> &f);
+
+// SpMV layout choice for a solve (mpdhg_config::spmv_layout).
+enum class Layout { Auto = 0, Storage = 1 };
+
+namespace detail {
+template <typename V>
+void permute_chunk(std::vector<V> &vals, int c0, const std::vector<uint32_t> &key,
+                   std::vector<V> &tmp) {
+    const int nc = static_cast<int>(key.size());
+    tmp.assign(vals.begin() + c0, vals.begin() + c0 + nc);
+    for (int q = 0; q < nc; ++q) vals[static_cast<size_t>(c0 + q)] = tmp[key[static_cast<size_t>(q)] & kDestMask];
+}
+}  // namespace detail
+
+// Puts every chunk of every tile into gather order (see Tile) when the
+// tile's instance gathers from a vector shorter than kGatherLocalLimit and
+// the layout is Auto: the global indices in `idx` (storage order) become
+// packed (local << kDestBits | position) words sorted ascending, each value
+// array is permuted alike, and Tile::base is set to the vector's offset.
+// Other tiles keep storage order (base = -1).  vec_off / vec_len: offset and
+// length of each instance's gathered vector in the batch.
+template <typename... Vs>
+void gather_order(Layout layout, const std::vector<int> &ptr, std::vector<Tile> &tiles,
+                  const std::vector<int64_t> &vec_off, const std::vector<int64_t> &vec_len,
+                  std::vector<int> &idx, std::vector<Vs> &...vals) {
+    constexpr int kGroup = 64;  // tiles per work item
+    const int nt = static_cast<int>(tiles.size());
+    parallel_for_each((nt + kGroup - 1) / kGroup, [&](int g) {
+        std::vector<uint32_t> key;
+        std::tuple<std::vector<Vs>...> tmp;
+        for (int t = g * kGroup; t < std::min(nt, (g + 1) * kGroup); ++t) {
+            Tile &tl = tiles[static_cast<size_t>(t)];
+            const size_t a = static_cast<size_t>(tl.inst);
+            if (layout != Layout::Auto || vec_len[a] >= kGatherLocalLimit) {
+                tl.base = -1;
+                continue;
+            }
+            const int64_t off = vec_off[a];
+            tl.base = static_cast<int32_t>(off);
+            const int lo = ptr[static_cast<size_t>(tl.begin)], hi = ptr[static_cast<size_t>(tl.end)];
+            for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
+                const int nc = std::min(hi, c0 + kTileNnz) - c0;
+                key.resize(static_cast<size_t>(nc));
+                for (int q = 0; q < nc; ++q)
+                    key[static_cast<size_t>(q)] =
+                        (static_cast<uint32_t>(idx[static_cast<size_t>(c0 + q)] - off) << kDestBits) |
+                        static_cast<uint32_t>(q);
+                std::sort(key.begin(), key.end());
+                for (int q = 0; q < nc; ++q) idx[static_cast<size_t>(c0 + q)] = static_cast<int>(key[static_cast<size_t>(q)]);
+                std::apply([&](auto &...tv) { (detail::permute_chunk(vals, c0, key, tv), ...); }, tmp);
+            }
+        }
+    });
+}
 
 // FP64 post-processing on the host for reference-mode results: A x,
 // c + A^T y and the relative KKT of (x, y).

@@ -108,6 +108,16 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctiles);
         s.col_tile_end = static_cast<int32_t>(ctiles.size());
     }
+    {
+        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
+        for (int a = 0; a < na; ++a) {
+            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+            rlen[static_cast<size_t>(a)] = h.m;
+            clen[static_cast<size_t>(a)] = h.n;
+        }
+        gather_order(Layout::Auto, rptr, rtiles, L.col_off, clen, ridx, rval);
+        gather_order(Layout::Auto, cptr, ctiles, L.row_off, rlen, cidx, cval);
+    }
     hipStream_t st = ctx.stream;
     DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
     DBuf<double> d_cval, d_rval, d_dc, d_dr, d_mb, d_mr, d_r, d_rt;

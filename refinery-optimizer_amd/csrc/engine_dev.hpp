@@ -14,13 +14,34 @@ constexpr int kBlock = 256;
 constexpr int kTileNnz = MPDHG_TILE_NNZ;
 constexpr int kPerThread = kTileNnz / kBlock;
 
+constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
+static_assert((kTileNnz & (kTileNnz - 1)) == 0 && kTileNnz % kBlock == 0,
+              "kTileNnz must be a power of two and a multiple of kBlock");
+
+// Chunks: a tile's nonzeros [lo, hi) are processed in chunks
+// [lo + j kTileNnz, lo + (j+1) kTileNnz) (one chunk unless a single row is
+// longer than kTileNnz).
+//
+// Gather order: within each chunk the nonzeros may be stored sorted by the
+// index of the vector entry they gather (ties in storage order), so that
+// the 64 gathers of one wave instruction fall on few cache lines.  The
+// index word then packs (local vector index << kDestBits) | (storage
+// position in the chunk), where local = index - Tile::base.  The lane that
+// owns a row still reads that row's products back in storage order, so
+// every sum is formed in the same order either way.
+constexpr int kDestBits = ilog2(kTileNnz);
+constexpr uint32_t kDestMask = (1u << kDestBits) - 1u;
+constexpr int64_t kGatherLocalLimit = int64_t(1) << (32 - kDestBits);
+
 // A tile: rows [begin, end) of one instance's compressed structure
 // (CSR rows for A x, CSC columns for A^T y).  end - begin <= kBlock.
+// base >= 0: the tile's chunks are in gather order, gathered vector entries
+// at base + local; base == -1: storage order, index words are global.
 struct Tile {
     int32_t begin;
     int32_t end;
     int32_t inst;
-    int32_t pad;
+    int32_t base;
 };
 
 constexpr int32_t kRunning = -1;  // instance status while iterating

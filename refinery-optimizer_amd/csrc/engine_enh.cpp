@@ -241,6 +241,17 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
+    {
+        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
+        for (int a = 0; a < na; ++a) {
+            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+            rlen[static_cast<size_t>(a)] = h.m;
+            clen[static_cast<size_t>(a)] = h.n;
+        }
+        const Layout lay = static_cast<Layout>(cfg.spmv_layout);
+        gather_order(lay, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
+        gather_order(lay, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
+    }
 
     // ---- device buffers ----
     hipStream_t st = ctx.stream;

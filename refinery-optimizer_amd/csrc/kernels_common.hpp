@@ -39,21 +39,16 @@ struct Pair {
     X x;
 };
 
-// LDS bytes every tile kernel reserves for the staged (value, x) pairs.
+// LDS bytes every tile kernel reserves for the staged (value, x) pairs:
+// one chunk of 8-byte pairs (FP32 value and x); 16-byte pairs (FP64) are
+// staged in two passes over the same chunk.
 constexpr int kStageBytes = kTileNnz * 8;
 
-// Row sums of one tile: sum_k val[k] * xs[idx[k]] for every row of the tile,
-// in storage order (k ascending), one row per lane.  The tile's nonzeros
-// are streamed in chunks: coalesced loads of idx/val, the gathers issued
-// back to back, then (val, x) pairs parked in LDS; each lane then walks its
-// own row from LDS.  A row longer than a chunk is continued across chunks
-// by the same lane, so the summation order never changes.  `lds` must hold
-// kStageBytes.
 // The row-pointer reads of a tile: issued first, so that a kernel can put
 // its other independent loads (epilogue operands, instance state) in flight
 // before it consumes them.
 struct TileRows {
-    int k0, k1, lo, hi;
+    int k0, k1, lo, hi, base;
 };
 
 __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restrict__ ptr) {
@@ -67,21 +62,33 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
     }
     t.lo = ptr[tl.begin];
     t.hi = ptr[tl.end];
+    t.base = tl.base;
     return t;
 }
 
+// Row sums of one tile: sum_k val[k] * xs[col(k)] for every row of the
+// tile, in storage order (k ascending), one row per lane.  Each chunk is
+// streamed with coalesced loads of the index words and values, the gathers
+// are issued back to back, and the (value, x) pairs are parked in LDS at
+// their storage positions (the position is k - chunk start in storage
+// order, or the low kDestBits of the index word in gather order); each lane
+// then walks its own row from LDS.  A row longer than a chunk is continued
+// across chunks by the same lane, so the summation order never depends on
+// the layout.  `lds` must hold kStageBytes.
 template <typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw) {
-    constexpr int CH = kStageBytes / static_cast<int>(sizeof(Pair<V, X>));
-    constexpr int PT = CH / kBlock;
+    constexpr int S = kStageBytes / static_cast<int>(sizeof(Pair<V, X>));  // pairs per pass
+    constexpr int NP = kTileNnz / S;                                        // passes per chunk
+    static_assert(S * NP == kTileNnz, "staging must tile the chunk");
+    constexpr int PT = kTileNnz / kBlock;
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
-    const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi;
+    const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi, base = tr.base;
     A sum = A(0);
-    for (int c0 = lo; c0 < hi; c0 += CH) {
-        const int c1 = min(hi, c0 + CH);
+    for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
+        const int c1 = min(hi, c0 + kTileNnz);
         int id[PT];
         V v[PT];
 #pragma unroll
@@ -93,23 +100,42 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
             }
         }
         X g[PT];
+        int d[PT];
+        if (base >= 0) {
 #pragma unroll
-        for (int q = 0; q < PT; ++q) {
-            const int k = c0 + tid + q * kBlock;
-            if (k < c1) g[q] = xs[id[q]];
+            for (int q = 0; q < PT; ++q) {
+                const int k = c0 + tid + q * kBlock;
+                if (k < c1) {
+                    const uint32_t w = static_cast<uint32_t>(id[q]);
+                    g[q] = xs[base + static_cast<int>(w >> kDestBits)];
+                    d[q] = static_cast<int>(w & kDestMask);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const int k = c0 + tid + q * kBlock;
+                if (k < c1) {
+                    g[q] = xs[id[q]];
+                    d[q] = k - c0;
+                }
+            }
         }
 #pragma unroll
-        for (int q = 0; q < PT; ++q) {
-            const int k = c0 + tid + q * kBlock;
-            if (k < c1) lds[k - c0] = Pair<V, X>{v[q], g[q]};
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const int k = c0 + tid + q * kBlock;
+                if (k < c1 && (NP == 1 || d[q] / S == p)) lds[d[q] - p * S] = Pair<V, X>{v[q], g[q]};
+            }
+            __syncthreads();
+            const int a = max(k0, c0 + p * S), b = min(k1, min(c1, c0 + (p + 1) * S));
+            for (int k = a; k < b; ++k) {
+                const Pair<V, X> pr = lds[k - c0 - p * S];
+                sum = fma_acc<A, V>(pr.v, pr.x, sum);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        const int a = max(k0, c0), b = min(k1, c1);
-        for (int k = a; k < b; ++k) {
-            const Pair<V, X> p = lds[k - c0];
-            sum = fma_acc<A, V>(p.v, p.x, sum);
-        }
-        __syncthreads();
     }
     return sum;
 }

@@ -292,6 +292,7 @@ class GpuPdhgConfig:
     profile: bool = False
     dual_sign: int = 0
     refine_ratio: float = 1e-2
+    spmv_layout: int = 0  # 0: gather order where it applies, 1: storage order
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,

@@ -35,6 +35,7 @@ class MpdhgConfig(C.Structure):
         ("profile", C.c_int32),
         ("dual_sign", C.c_int32),
         ("refine_ratio", C.c_double),
+        ("spmv_layout", C.c_int32),
     ]
 
 

@@ -95,3 +95,16 @@ def test_iteration_limit_and_short_runs(need_gpu):
     e = S.Problem("empty")
     e.resize(3, 0)
     assert S.solve_gpu_pdhg(e, enh()).status == S.SolutionStatus.Optimal
+
+
+def test_gather_order_layout_is_bit_identical(need_gpu):
+    probs = [S.gen_random(300, 720, 8, 3), S.gen_staircase(2, 40, 90, 4), random_lp(60, 90, 0.1, 2)[0],
+             toy01()]
+    cfg = enh(tol=1e-6)
+    a = S.solve_batch(probs, cfg)
+    cfg.spmv_layout = 1
+    b = S.solve_batch(probs, cfg)
+    for p, s, t in zip(probs, a, b):
+        assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
+        assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
+        assert s.rel_kkt == t.rel_kkt, p.name

@@ -127,3 +127,16 @@ def test_dual_sign_option_and_kkt_fields(need_gpu):
     np.testing.assert_allclose(a.reduced_costs, p.c + p.A.mat_trans_vec(a.row_duals), rtol=1e-12,
                                atol=1e-12)
     np.testing.assert_allclose(a.slack, p.A.mat_vec(a.x), rtol=1e-12, atol=1e-12)
+
+
+def test_gather_order_layout_is_bit_identical(need_gpu):
+    # spmv_layout 0 stores each chunk sorted by gathered index; 1 keeps
+    # storage order.  The per-row summation order is the same, so are the bits.
+    probs = _mixed_batch()
+    c = cfg(max_iterations=300, check_frequency=25)
+    a = S.solve_batch(probs, c)
+    c.spmv_layout = 1
+    b = S.solve_batch(probs, c)
+    for p, s, t in zip(probs, a, b):
+        assert s.simplex_iterations == t.simplex_iterations, p.name
+        assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
