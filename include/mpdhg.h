@@ -81,6 +81,9 @@ typedef struct mpdhg_config {
                                      gathered index where the instance's vectors have < 2^20
                                      entries; 1: storage order everywhere.  Results are
                                      bit-identical either way (same per-row summation order). */
+    int32_t spmv_chunk;           /* nonzeros staged through LDS per workgroup pass: 0 = auto
+                                     (the smallest of 1024/2048/4096 holding 256 rows of average
+                                     length, per direction), or 1024, 2048, 4096 */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite

@@ -49,6 +49,8 @@ int check_config(const mpdhg_config *cfg) {
     if (cfg->check_frequency <= 0) return mpdhg::fail(MPDHG_E_ARG, "check_frequency must be > 0");
     if (cfg->spmv_layout != 0 && cfg->spmv_layout != 1)
         return mpdhg::fail(MPDHG_E_ARG, "spmv_layout must be 0 (auto) or 1 (storage order)");
+    if (cfg->spmv_chunk != 0 && !mpdhg::valid_chunk(cfg->spmv_chunk))
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_chunk must be 0 (auto), 1024, 2048 or 4096");
     return MPDHG_OK;
 }
 
@@ -108,6 +110,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->dual_sign = 0;
     cfg->refine_ratio = 1e-2;
     cfg->spmv_layout = 0;
+    cfg->spmv_chunk = 0;
 }
 
 int mpdhg_device_count(int *count) {

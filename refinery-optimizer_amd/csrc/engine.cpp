@@ -76,9 +76,9 @@ void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
 }
 
 // Rows [start, i) form one tile while the tile has at most kBlock rows and
-// at most kTileNnz nonzeros; a single longer row gets a tile of its own
-// (the kernel streams it in several chunks).
-void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst,
+// at most tn nonzeros; a single longer row gets a tile of its own (the
+// kernel streams it in several chunks).
+void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst, int tn,
                 std::vector<Tile> &out) {
     int i = 0;
     while (i < count) {
@@ -86,7 +86,7 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int 
         int nz = 0;
         while (i < count && i - start < kBlock) {
             const int rn = ptr[static_cast<size_t>(i) + 1] - ptr[static_cast<size_t>(i)];
-            if (i > start && nz + rn > kTileNnz) break;
+            if (i > start && nz + rn > tn) break;
             nz += rn;
             ++i;
         }
@@ -95,13 +95,23 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int 
     }
 }
 
-TileSet make_tileset(const DBuf<Tile> &d, int count) {
+TileSet make_tileset(const DBuf<Tile> &d, int count, int tn) {
     TileSet ts;
     ts.tiles = d.get();
     ts.count = count;
     ts.chunk = (count + 7) / 8;
     ts.grid = 8 * ts.chunk;
+    ts.tn = tn;
     return ts;
+}
+
+int choose_chunk(int64_t nnz, int64_t rows, int forced) {
+    if (forced) return forced;
+    int tn = kChunkMin;
+    while (tn < kTileNnz && static_cast<double>(tn) < static_cast<double>(kBlock) * nnz /
+                                                         static_cast<double>(std::max<int64_t>(rows, 1)))
+        tn *= 2;
+    return tn;
 }
 
 static inline bool finite_bound(double v) { return std::isfinite(v); }
@@ -226,6 +236,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
+    const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
+    const int ctn = choose_chunk(L.nnz, L.cols, cfg.spmv_chunk);
     hh.cptr.resize(C + 1);
     hh.cidx.resize(Z);
     hh.rptr.resize(R + 1);
@@ -287,10 +299,10 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         s.max_abs_c = mac;
         s.max_abs_b = mab;
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, ro, a, hh.rtiles);
+        plan_tiles(h.rptr, h.m, ro, a, rtn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, co, a, hh.ctiles);
+        plan_tiles(h.cptr, h.n, co, a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     {
@@ -301,8 +313,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        gather_order(lay, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
-        gather_order(lay, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
+        gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
+        gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
     }
     // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
     hipStream_t st = ctx.stream;
@@ -321,8 +333,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
     DBuf<float> d_prow(hh.rtiles.size()), d_pcol(hh.ctiles.size());
-    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
-    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
+    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn);
+    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn);
     MPDHG_HIP(hipStreamSynchronize(st));
 
     // ---- profiling samples: HIP events around the fused SpMV launches ----

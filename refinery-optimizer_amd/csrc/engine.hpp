@@ -87,10 +87,17 @@ struct BatchLayout {
     int64_t rows = 0, cols = 0, nnz = 0;
 };
 
-// Greedy tile plan over one instance's compressed structure.
-void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst,
+// Greedy tile plan over one instance's compressed structure: at most
+// kBlock rows and at most tn nonzeros per tile (a longer row alone).
+void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst, int tn,
                 std::vector<Tile> &out);
-TileSet make_tileset(const DBuf<Tile> &d, int count);
+TileSet make_tileset(const DBuf<Tile> &d, int count, int tn);
+
+// Chunk size of a tile set over `rows` rows holding `nnz` nonzeros: the
+// smallest of 1024, 2048, 4096 that holds kBlock rows of average length, so
+// that the LDS a workgroup reserves matches what a full tile stages.
+// `forced` (mpdhg_config::spmv_chunk) overrides it when nonzero.
+int choose_chunk(int64_t nnz, int64_t rows, int forced);
 
 struct Context {
     int device = 0;
@@ -118,26 +125,28 @@ enum class Layout { Auto = 0, Storage = 1 };
 
 namespace detail {
 template <typename V>
-void permute_chunk(std::vector<V> &vals, int c0, const std::vector<uint32_t> &key,
+void permute_chunk(std::vector<V> &vals, int c0, const std::vector<uint32_t> &key, uint32_t mask,
                    std::vector<V> &tmp) {
     const int nc = static_cast<int>(key.size());
     tmp.assign(vals.begin() + c0, vals.begin() + c0 + nc);
-    for (int q = 0; q < nc; ++q) vals[static_cast<size_t>(c0 + q)] = tmp[key[static_cast<size_t>(q)] & kDestMask];
+    for (int q = 0; q < nc; ++q) vals[static_cast<size_t>(c0 + q)] = tmp[key[static_cast<size_t>(q)] & mask];
 }
 }  // namespace detail
 
-// Puts every chunk of every tile into gather order (see Tile) when the
-// tile's instance gathers from a vector shorter than kGatherLocalLimit and
-// the layout is Auto: the global indices in `idx` (storage order) become
-// packed (local << kDestBits | position) words sorted ascending, each value
-// array is permuted alike, and Tile::base is set to the vector's offset.
-// Other tiles keep storage order (base = -1).  vec_off / vec_len: offset and
-// length of each instance's gathered vector in the batch.
+// Puts every chunk (tn nonzeros) of every tile into gather order (see Tile)
+// when the layout is Auto and the tile's instance gathers from a vector
+// shorter than gather_local_limit(tn): the global indices in `idx` (storage
+// order) become packed (local << log2 tn | position) words sorted
+// ascending, each value array is permuted alike, and Tile::base is set to
+// the vector's offset.  Other tiles keep storage order (base = -1).
+// vec_off / vec_len: offset and length of each instance's gathered vector.
 template <typename... Vs>
-void gather_order(Layout layout, const std::vector<int> &ptr, std::vector<Tile> &tiles,
+void gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vector<Tile> &tiles,
                   const std::vector<int64_t> &vec_off, const std::vector<int64_t> &vec_len,
                   std::vector<int> &idx, std::vector<Vs> &...vals) {
     constexpr int kGroup = 64;  // tiles per work item
+    const int bits = ilog2(tn);
+    const uint32_t mask = static_cast<uint32_t>(tn) - 1u;
     const int nt = static_cast<int>(tiles.size());
     parallel_for_each((nt + kGroup - 1) / kGroup, [&](int g) {
         std::vector<uint32_t> key;
@@ -145,23 +154,23 @@ void gather_order(Layout layout, const std::vector<int> &ptr, std::vector<Tile> 
         for (int t = g * kGroup; t < std::min(nt, (g + 1) * kGroup); ++t) {
             Tile &tl = tiles[static_cast<size_t>(t)];
             const size_t a = static_cast<size_t>(tl.inst);
-            if (layout != Layout::Auto || vec_len[a] >= kGatherLocalLimit) {
+            if (layout != Layout::Auto || vec_len[a] >= gather_local_limit(tn)) {
                 tl.base = -1;
                 continue;
             }
             const int64_t off = vec_off[a];
             tl.base = static_cast<int32_t>(off);
             const int lo = ptr[static_cast<size_t>(tl.begin)], hi = ptr[static_cast<size_t>(tl.end)];
-            for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
-                const int nc = std::min(hi, c0 + kTileNnz) - c0;
+            for (int c0 = lo; c0 < hi; c0 += tn) {
+                const int nc = std::min(hi, c0 + tn) - c0;
                 key.resize(static_cast<size_t>(nc));
                 for (int q = 0; q < nc; ++q)
                     key[static_cast<size_t>(q)] =
-                        (static_cast<uint32_t>(idx[static_cast<size_t>(c0 + q)] - off) << kDestBits) |
+                        (static_cast<uint32_t>(idx[static_cast<size_t>(c0 + q)] - off) << bits) |
                         static_cast<uint32_t>(q);
                 std::sort(key.begin(), key.end());
                 for (int q = 0; q < nc; ++q) idx[static_cast<size_t>(c0 + q)] = static_cast<int>(key[static_cast<size_t>(q)]);
-                std::apply([&](auto &...tv) { (detail::permute_chunk(vals, c0, key, tv), ...); }, tmp);
+                std::apply([&](auto &...tv) { (detail::permute_chunk(vals, c0, key, mask, tv), ...); }, tmp);
             }
         }
     });

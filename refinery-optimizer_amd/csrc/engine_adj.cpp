@@ -50,6 +50,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
+    const int rtn = choose_chunk(L.nnz, L.rows, 0);
+    const int ctn = choose_chunk(L.nnz, L.cols, 0);
     std::vector<int> cptr(C + 1), cidx(Z), rptr(R + 1), ridx(Z);
     std::vector<double> cval(Z), rval(Z), dc(C), dr(R), mb(C), mr(R), r0(C), rt0(C);
     std::vector<AdjInst> inst(static_cast<size_t>(na));
@@ -102,10 +104,10 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
         AdjInst &s = inst[static_cast<size_t>(a)];
         s.row_tile_begin = static_cast<int32_t>(rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, rtiles);
+        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, rtn, rtiles);
         s.row_tile_end = static_cast<int32_t>(rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctiles);
+        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, ctiles);
         s.col_tile_end = static_cast<int32_t>(ctiles.size());
     }
     {
@@ -115,8 +117,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
             rlen[static_cast<size_t>(a)] = h.m;
             clen[static_cast<size_t>(a)] = h.n;
         }
-        gather_order(Layout::Auto, rptr, rtiles, L.col_off, clen, ridx, rval);
-        gather_order(Layout::Auto, cptr, ctiles, L.row_off, rlen, cidx, cval);
+        gather_order(Layout::Auto, rtn, rptr, rtiles, L.col_off, clen, ridx, rval);
+        gather_order(Layout::Auto, ctn, cptr, ctiles, L.row_off, rlen, cidx, cval);
     }
     hipStream_t st = ctx.stream;
     DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
@@ -132,8 +134,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(double), st));
     MPDHG_HIP(hipMemsetAsync(d_q.get(), 0, C * sizeof(double), st));
     DBuf<double> d_pc(ctiles.size() * kPcCount), d_pr(rtiles.size() * kPrCount);
-    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()));
-    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()));
+    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rtn);
+    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), ctn);
     launch::AdjColArgs CA{d_cptr.get(), d_cidx.get(), d_cval.get(), d_pt.get(), d_mb.get(), d_dc.get(),
                           d_q.get(), d_r.get(), d_rt.get(), d_pc.get()};
     launch::AdjRowArgs RA{d_rptr.get(), d_ridx.get(), d_rval.get(), d_rt.get(), d_mr.get(), d_dr.get(),

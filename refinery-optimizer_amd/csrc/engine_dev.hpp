@@ -6,32 +6,30 @@
 namespace mpdhg {
 
 // Workgroup shape of every tile kernel: 4 wave64s, one row per lane in the
-// epilogue, kTileNnz nonzeros staged through LDS per chunk.
+// epilogue.
 constexpr int kBlock = 256;
-#ifndef MPDHG_TILE_NNZ
-#define MPDHG_TILE_NNZ 4096
-#endif
-constexpr int kTileNnz = MPDHG_TILE_NNZ;
-constexpr int kPerThread = kTileNnz / kBlock;
+
+// Chunks: a tile set has a chunk size tn in {1024, 2048, 4096} (kTileNnz is
+// the largest); a tile holds at most kBlock rows and at most tn nonzeros
+// unless it is a single longer row, and its nonzeros [lo, hi) are staged
+// through LDS in chunks [lo + j tn, lo + (j+1) tn).  The LDS a workgroup
+// reserves is sized by tn, so sparse structures (short rows) use small
+// chunks and more workgroups fit on a CU.
+constexpr int kTileNnz = 4096;
+constexpr int kChunkMin = 1024;
 
 constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
-static_assert((kTileNnz & (kTileNnz - 1)) == 0 && kTileNnz % kBlock == 0,
-              "kTileNnz must be a power of two and a multiple of kBlock");
+constexpr bool valid_chunk(int tn) { return tn == 1024 || tn == 2048 || tn == 4096; }
 
-// Chunks: a tile's nonzeros [lo, hi) are processed in chunks
-// [lo + j kTileNnz, lo + (j+1) kTileNnz) (one chunk unless a single row is
-// longer than kTileNnz).
-//
 // Gather order: within each chunk the nonzeros may be stored sorted by the
 // index of the vector entry they gather (ties in storage order), so that
 // the 64 gathers of one wave instruction fall on few cache lines.  The
-// index word then packs (local vector index << kDestBits) | (storage
-// position in the chunk), where local = index - Tile::base.  The lane that
-// owns a row still reads that row's products back in storage order, so
-// every sum is formed in the same order either way.
-constexpr int kDestBits = ilog2(kTileNnz);
-constexpr uint32_t kDestMask = (1u << kDestBits) - 1u;
-constexpr int64_t kGatherLocalLimit = int64_t(1) << (32 - kDestBits);
+// index word then packs (local vector index << log2 tn) | (storage
+// position in the chunk), where local = index - Tile::base, which needs
+// local < gather_local_limit(tn).  The lane that owns a row still reads that
+// row's products back in storage order, so every sum is formed in the same
+// order either way.
+constexpr int64_t gather_local_limit(int tn) { return int64_t(1) << (32 - ilog2(tn)); }
 
 // A tile: rows [begin, end) of one instance's compressed structure
 // (CSR rows for A x, CSC columns for A^T y).  end - begin <= kBlock.

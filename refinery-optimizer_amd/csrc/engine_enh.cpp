@@ -149,6 +149,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
+    const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
+    const int ctn = choose_chunk(L.nnz, L.cols, cfg.spmv_chunk);
     EnhHost hh;
     hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.ridx.resize(Z);
     hh.cval.resize(Z); hh.rval.resize(Z); hh.cval64.resize(Z); hh.rval64.resize(Z);
@@ -235,10 +237,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         const HostLp &h = H[static_cast<size_t>(q)];
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, hh.rtiles);
+        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, rtn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, hh.ctiles);
+        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     {
@@ -249,8 +251,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        gather_order(lay, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
-        gather_order(lay, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
+        gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
+        gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
     }
 
     // ---- device buffers ----
@@ -289,8 +291,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
         d_status.upload(run, st);
     }
-    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()));
-    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()));
+    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn);
+    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn);
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -409,8 +411,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     keep_rng.emplace_back();
                     keep_rng.back().upload(rng, st);
                     launch::enh_set_ranges(st, d_inst.get(), na, keep_rng.back().get());
-                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()));
-                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()));
+                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn);
+                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn);
                     covered = running;
                     host_keep.push_back(std::move(rng));
                 }

@@ -24,16 +24,17 @@ using launch::AdjRowArgs;
 
 // q_j = mB_j dc_j sum_i A_ij pt_i (pt = masked, D_r-scaled row vector);
 // also rt_j for the next row SpMV after the residual update r -= alpha q.
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_adj_cols(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjColArgs A) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].done) return;
-    const double s = tile_dot_t<double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_t<TN, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q2 = 0.0;
     if (j < tl.end) {
@@ -78,17 +79,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_upd_cols(const Tile *__restrict_
 }
 
 // s_i = mR_i dr_i sum_j A_ij rt_j ; mu += alpha p ; partial |s|^2.
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_adj_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjRowArgs A) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     const AdjInst &s = st[tl.inst];
     if (s.done) return;
-    const double v = tile_dot_t<double, double, double>(tl, A.rptr, A.ridx, A.rval, A.rt, lds);
+    const double v = tile_dot_t<TN, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.rt, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double s2 = 0.0;
     if (i < tl.end) {
@@ -151,13 +153,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_unscale(const Tile *__restrict__
 
 // Final column pass in the original space: q_j = sum_i A_ij pt_i = (A^T mu)_j
 // with the unscaled multipliers, for the bound gradients.
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_adj_atmu(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, AdjColArgs A) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const double s = tile_dot_t<double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_t<TN, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) A.q[j] = s;
 }
@@ -166,14 +169,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_atmu(const Tile *__restrict__ ti
 
 namespace launch {
 
-#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
 
 void adj_step(hipStream_t s, const TileSet &rts, const TileSet &cts, AdjInst *st, int ninst,
               const AdjColArgs &c, const AdjRowArgs &r, double tol, int max_iter) {
-    if (cts.count) hipLaunchKernelGGL(dev::k_adj_cols, MPDHG_TILE_GRID(cts), st, c);
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_adj_cols, st, c);
     hipLaunchKernelGGL(dev::k_adj_alpha, dim3(ninst), dim3(kBlock), 0, s, st, ninst, c.pc);
     if (cts.count) hipLaunchKernelGGL(dev::k_adj_upd_cols, MPDHG_TILE_GRID(cts), st, c);
-    if (rts.count) hipLaunchKernelGGL(dev::k_adj_rows, MPDHG_TILE_GRID(rts), st, r);
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_adj_rows, st, r);
     hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kBlock), 0, s, st, ninst, r.pr, tol, max_iter);
     if (rts.count) hipLaunchKernelGGL(dev::k_adj_upd_rows, MPDHG_TILE_GRID(rts), st, r);
 }
@@ -181,7 +183,7 @@ void adj_step(hipStream_t s, const TileSet &rts, const TileSet &cts, AdjInst *st
 void adj_start(hipStream_t s, const TileSet &rts, AdjInst *st, int ninst, const AdjRowArgs &r,
                double tol, int max_iter) {
     // s = K^T r0 with alpha = 0 (mu unchanged), gamma0 = |s|^2, p = s
-    if (rts.count) hipLaunchKernelGGL(dev::k_adj_rows, MPDHG_TILE_GRID(rts), st, r);
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_adj_rows, st, r);
     hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kBlock), 0, s, st, ninst, r.pr, tol, max_iter);
     if (rts.count) hipLaunchKernelGGL(dev::k_adj_upd_rows, MPDHG_TILE_GRID(rts), st, r);
 }
@@ -189,10 +191,9 @@ void adj_start(hipStream_t s, const TileSet &rts, AdjInst *st, int ninst, const 
 void adj_atmu(hipStream_t s, const TileSet &rts, const TileSet &cts, const AdjColArgs &c,
               const AdjRowArgs &r) {
     if (rts.count) hipLaunchKernelGGL(dev::k_adj_unscale, MPDHG_TILE_GRID(rts), r);
-    if (cts.count) hipLaunchKernelGGL(dev::k_adj_atmu, MPDHG_TILE_GRID(cts), c);
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_adj_atmu, c);
 }
 
-#undef MPDHG_TILE_GRID
 
 }  // namespace launch
 }  // namespace mpdhg

@@ -39,10 +39,10 @@ struct Pair {
     X x;
 };
 
-// LDS bytes every tile kernel reserves for the staged (value, x) pairs:
-// one chunk of 8-byte pairs (FP32 value and x); 16-byte pairs (FP64) are
-// staged in two passes over the same chunk.
-constexpr int kStageBytes = kTileNnz * 8;
+// LDS bytes a tile kernel with chunk size TN reserves for the staged
+// (value, x) pairs: one chunk of 8-byte pairs (FP32 value and x); 16-byte
+// pairs (FP64) are staged in two passes over the same chunk.
+constexpr int stage_bytes(int tn) { return tn * 8; }
 
 // The row-pointer reads of a tile: issued first, so that a kernel can put
 // its other independent loads (epilogue operands, instance state) in flight
@@ -67,28 +67,31 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
 }
 
 // Row sums of one tile: sum_k val[k] * xs[col(k)] for every row of the
-// tile, in storage order (k ascending), one row per lane.  Each chunk is
-// streamed with coalesced loads of the index words and values, the gathers
-// are issued back to back, and the (value, x) pairs are parked in LDS at
-// their storage positions (the position is k - chunk start in storage
-// order, or the low kDestBits of the index word in gather order); each lane
-// then walks its own row from LDS.  A row longer than a chunk is continued
-// across chunks by the same lane, so the summation order never depends on
-// the layout.  `lds` must hold kStageBytes.
-template <typename V, typename X, typename A>
+// tile, in storage order (k ascending), one row per lane.  Each chunk of TN
+// nonzeros is streamed with coalesced loads of the index words and values,
+// the gathers are issued back to back, and the (value, x) pairs are parked
+// in LDS at their storage positions (the position is k - chunk start in
+// storage order, or the low log2 TN bits of the index word in gather
+// order); each lane then walks its own row from LDS.  A row longer than a
+// chunk is continued across chunks by the same lane, so the summation order
+// never depends on the layout or on TN.  `lds` must hold stage_bytes(TN).
+template <int TN, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw) {
-    constexpr int S = kStageBytes / static_cast<int>(sizeof(Pair<V, X>));  // pairs per pass
-    constexpr int NP = kTileNnz / S;                                        // passes per chunk
-    static_assert(S * NP == kTileNnz, "staging must tile the chunk");
-    constexpr int PT = kTileNnz / kBlock;
+    static_assert(valid_chunk(TN), "chunk size");
+    constexpr int kBits = ilog2(TN);
+    constexpr uint32_t kMask = static_cast<uint32_t>(TN) - 1u;
+    constexpr int S = stage_bytes(TN) / static_cast<int>(sizeof(Pair<V, X>));  // pairs per pass
+    constexpr int NP = TN / S;                                                 // passes per chunk
+    static_assert(S * NP == TN, "staging must tile the chunk");
+    constexpr int PT = TN / kBlock;
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
     const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi, base = tr.base;
     A sum = A(0);
-    for (int c0 = lo; c0 < hi; c0 += kTileNnz) {
-        const int c1 = min(hi, c0 + kTileNnz);
+    for (int c0 = lo; c0 < hi; c0 += TN) {
+        const int c1 = min(hi, c0 + TN);
         int id[PT];
         V v[PT];
 #pragma unroll
@@ -107,8 +110,8 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
                 const int k = c0 + tid + q * kBlock;
                 if (k < c1) {
                     const uint32_t w = static_cast<uint32_t>(id[q]);
-                    g[q] = xs[base + static_cast<int>(w >> kDestBits)];
-                    d[q] = static_cast<int>(w & kDestMask);
+                    g[q] = xs[base + static_cast<int>(w >> kBits)];
+                    d[q] = static_cast<int>(w & kMask);
                 }
             }
         } else {
@@ -140,18 +143,18 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
     return sum;
 }
 
-template <typename V, typename X, typename A>
+template <int TN, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
                                         const int *__restrict__ idx, const V *__restrict__ val,
                                         const X *__restrict__ xs, void *lds_raw) {
-    return tile_dot_rows<V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
+    return tile_dot_rows<TN, V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
 }
 
-template <typename A>
+template <int TN, typename A>
 __device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
                                       const int *__restrict__ idx, const float *__restrict__ val,
                                       const float *__restrict__ xs, void *lds) {
-    return tile_dot_t<float, float, A>(tl, ptr, idx, val, xs, lds);
+    return tile_dot_t<TN, float, float, A>(tl, ptr, idx, val, xs, lds);
 }
 
 // Block-wide max of non-negative floats (NaN operands are dropped, as
@@ -198,4 +201,20 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
 }
 
 }  // namespace dev
+
+// Launch-argument prefix of a tile kernel over a tile set (in launcher
+// bodies, with the stream in `s`).
+#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
+
+// Launches KERNEL<tn> for the tile set's chunk size.
+#define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                     \
+    do {                                                                                     \
+        if ((ts).tn == 1024)                                                                 \
+            hipLaunchKernelGGL(KERNEL<1024>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+        else if ((ts).tn == 2048)                                                            \
+            hipLaunchKernelGGL(KERNEL<2048>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+        else                                                                                 \
+            hipLaunchKernelGGL(KERNEL<4096>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+    } while (0)
+
 }  // namespace mpdhg

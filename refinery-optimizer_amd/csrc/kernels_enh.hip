@@ -33,21 +33,23 @@ __global__ __launch_bounds__(kBlock) void k_pow_scale(const Tile *__restrict__ t
     if (j < tl.end) v[j] = z[j] * inv;
 }
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ rptr,
                                                      const int *__restrict__ ridx,
                                                      const float *__restrict__ rval,
                                                      const float *__restrict__ v,
                                                      float *__restrict__ w) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<float>(tl, rptr, ridx, rval, v, lds);
+    const float s = tile_dot<TN, float>(tl, rptr, ridx, rval, v, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) w[i] = s;
 }
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ cptr,
                                                      const int *__restrict__ cidx,
@@ -55,12 +57,12 @@ __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ ti
                                                      const float *__restrict__ w,
                                                      float *__restrict__ z,
                                                      double *__restrict__ pc) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<float>(tl, cptr, cidx, cval, w, lds);
+    const float s = tile_dot<TN, float>(tl, cptr, cidx, cval, w, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q = 0.0;
     if (j < tl.end) {
@@ -97,10 +99,11 @@ struct PrimalArgs {
     float rho;
 };
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
                                                        int chunk, const EnhInst *__restrict__ st,
                                                        PrimalArgs A, int it, int check) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot_rows<float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    const float g = tile_dot_rows<TN, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -171,10 +174,11 @@ struct DualArgs {
     float rho;
 };
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const EnhInst *__restrict__ st,
                                                      DualArgs A, int it, int check) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<TN, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -243,16 +247,17 @@ struct CheckRowArgs {
     double *pr;
 };
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckRowArgs A) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const double axi = tile_dot_t<double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
+    const double axi = tile_dot_t<TN, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double viol = 0.0, sviol = 0.0, dobj = 0.0;
     if (i < tl.end) {
@@ -288,16 +293,17 @@ struct CheckColArgs {
     double *pc;
 };
 
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckColArgs A) {
-    __shared__ __align__(16) unsigned char lds[kStageBytes];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const double aty = tile_dot_t<double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
+    const double aty = tile_dot_t<TN, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
     if (j < tl.end) {
@@ -591,18 +597,17 @@ void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng) {
                        s, st, ninst, rng);
 }
 
-#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
 
 void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v) {
     if (cts.count) hipLaunchKernelGGL(dev::k_pow_scale, MPDHG_TILE_GRID(cts), st, z, v);
 }
 void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
               const float *rval, const float *v, float *w) {
-    if (rts.count) hipLaunchKernelGGL(dev::k_pow_rows, MPDHG_TILE_GRID(rts), rptr, ridx, rval, v, w);
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_pow_rows, rptr, ridx, rval, v, w);
 }
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
               const float *cval, const float *w, float *z, double *pc) {
-    if (cts.count) hipLaunchKernelGGL(dev::k_pow_cols, MPDHG_TILE_GRID(cts), cptr, cidx, cval, w, z, pc);
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_pow_cols, cptr, cidx, cval, w, z, pc);
 }
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc) {
     hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kBlock), 0, s, st, ninst, pc);
@@ -611,21 +616,21 @@ void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhP
                 int check) {
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
                       a.xbar, a.xbase, a.dc, a.xchk, a.pc, a.rho};
-    if (cts.count) hipLaunchKernelGGL(dev::k_enh_primal, MPDHG_TILE_GRID(cts), st, A, it, check);
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, it, check);
 }
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
               int check) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
                     a.ybase, a.dr, a.ychk, a.pr, a.rho};
-    if (rts.count) hipLaunchKernelGGL(dev::k_enh_dual, MPDHG_TILE_GRID(rts), st, A, it, check);
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, it, check);
 }
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
                const EnhCheck &a, int it) {
     dev::CheckRowArgs R{a.rptr, a.ridx, a.rval64, a.xchk, a.ychk, a.rl64, a.ru64, a.ax, a.pr};
     dev::CheckColArgs Cc{a.cptr, a.cidx, a.cval64, a.ychk, a.xchk, a.c64, a.l64, a.u64, a.lam,
                          a.pc};
-    if (rts.count) hipLaunchKernelGGL(dev::k_enh_chk_rows, MPDHG_TILE_GRID(rts), st, R);
-    if (cts.count) hipLaunchKernelGGL(dev::k_enh_chk_cols, MPDHG_TILE_GRID(cts), st, Cc);
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_chk_rows, st, R);
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_chk_cols, st, Cc);
     dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.max_refinements};
     hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, it);
     if (cts.count)
@@ -644,7 +649,6 @@ void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *s
     }
 }
 
-#undef MPDHG_TILE_GRID
 
 }  // namespace launch
 }  // namespace mpdhg

@@ -15,18 +15,19 @@ namespace mpdhg {
 namespace dev {
 
 // cuda_pdhg.cu:33-49 (A^T y) fused with :54-77 (projected primal step).
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_ref_primal(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
     const float *__restrict__ y, const float *__restrict__ c, const float *__restrict__ l,
     const float *__restrict__ u, const float *__restrict__ tau, float *__restrict__ x,
     float *__restrict__ xbar) {
-    __shared__ float2 lds[kTileNnz];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<TN, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) {
         const float xo = x[j];
@@ -41,17 +42,18 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
 }
 
 // cuda_pdhg.cu:14-30 (A xbar) fused with :82-103 (Moreau dual step).
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_ref_dual(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
     const float *__restrict__ xbar, const float *__restrict__ rl, const float *__restrict__ ru,
     const float *__restrict__ sigma, float *__restrict__ y) {
-    __shared__ float2 lds[kTileNnz];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float h = tile_dot<float>(tl, rptr, ridx, rval, xbar, lds);
+    const float h = tile_dot<TN, float>(tl, rptr, ridx, rval, xbar, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) {
         const float sg = sigma[i];
@@ -66,18 +68,19 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
 
 // cuda_pdhg.cu:114-129 row residuals of A x, max-reduced per tile (the
 // reference copies m floats to the host and reduces there, :249-255).
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_ref_rowres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
     const float *__restrict__ x, const float *__restrict__ rl, const float *__restrict__ ru,
     float *__restrict__ part) {
-    __shared__ float2 lds[kTileNnz];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ float red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float ax = tile_dot<float>(tl, rptr, ridx, rval, x, lds);
+    const float ax = tile_dot<TN, float>(tl, rptr, ridx, rval, x, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (i < tl.end) {
@@ -90,19 +93,20 @@ __global__ __launch_bounds__(kBlock) void k_ref_rowres(
 }
 
 // cuda_pdhg.cu:131-151 column fixed-point residuals from A^T y, max-reduced per tile.
+template <int TN>
 __global__ __launch_bounds__(kBlock) void k_ref_colres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
     const float *__restrict__ y, const float *__restrict__ x, const float *__restrict__ c,
     const float *__restrict__ l, const float *__restrict__ u, const float *__restrict__ tau,
     float *__restrict__ part) {
-    __shared__ float2 lds[kTileNnz];
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ float red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<TN, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (j < tl.end) {
@@ -183,29 +187,25 @@ namespace launch {
 void ref_primal(hipStream_t s, const TileSet &ts, const RefInst *st, const int *cptr,
                 const int *cidx, const float *cval, const float *y, const float *c, const float *l,
                 const float *u, const float *tau, float *x, float *xbar) {
-    hipLaunchKernelGGL(dev::k_ref_primal, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles, ts.count,
-                       ts.chunk, st, cptr, cidx, cval, y, c, l, u, tau, x, xbar);
+    MPDHG_LAUNCH_TN(ts, dev::k_ref_primal, st, cptr, cidx, cval, y, c, l, u, tau, x, xbar);
 }
 
 void ref_dual(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
               const int *ridx, const float *rval, const float *xbar, const float *rl,
               const float *ru, const float *sigma, float *y) {
-    hipLaunchKernelGGL(dev::k_ref_dual, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles, ts.count,
-                       ts.chunk, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
+    MPDHG_LAUNCH_TN(ts, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
 }
 
 void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
                 const int *ridx, const float *rval, const float *x, const float *rl,
                 const float *ru, float *part) {
-    hipLaunchKernelGGL(dev::k_ref_rowres, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles, ts.count,
-                       ts.chunk, st, rptr, ridx, rval, x, rl, ru, part);
+    MPDHG_LAUNCH_TN(ts, dev::k_ref_rowres, st, rptr, ridx, rval, x, rl, ru, part);
 }
 
 void ref_colres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *cptr,
                 const int *cidx, const float *cval, const float *y, const float *x,
                 const float *c, const float *l, const float *u, const float *tau, float *part) {
-    hipLaunchKernelGGL(dev::k_ref_colres, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles, ts.count,
-                       ts.chunk, st, cptr, cidx, cval, y, x, c, l, u, tau, part);
+    MPDHG_LAUNCH_TN(ts, dev::k_ref_colres, st, cptr, cidx, cval, y, x, c, l, u, tau, part);
 }
 
 void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, const float *pcol,

@@ -11,8 +11,9 @@ namespace mpdhg {
 struct TileSet {
     const Tile *tiles = nullptr;
     int count = 0;
-    int chunk = 0;  // ceil(count / 8)
-    int grid = 0;   // 8 * chunk
+    int chunk = 0;        // ceil(count / 8)
+    int grid = 0;         // 8 * chunk
+    int tn = kTileNnz;    // nonzeros per LDS chunk (1024, 2048 or 4096)
 };
 
 namespace launch {

@@ -36,6 +36,7 @@ class MpdhgConfig(C.Structure):
         ("dual_sign", C.c_int32),
         ("refine_ratio", C.c_double),
         ("spmv_layout", C.c_int32),
+        ("spmv_chunk", C.c_int32),
     ]
 
 

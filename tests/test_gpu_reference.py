@@ -129,14 +129,17 @@ def test_dual_sign_option_and_kkt_fields(need_gpu):
     np.testing.assert_allclose(a.slack, p.A.mat_vec(a.x), rtol=1e-12, atol=1e-12)
 
 
-def test_gather_order_layout_is_bit_identical(need_gpu):
-    # spmv_layout 0 stores each chunk sorted by gathered index; 1 keeps
-    # storage order.  The per-row summation order is the same, so are the bits.
+def test_layouts_and_chunk_sizes_are_bit_identical(need_gpu):
+    # spmv_layout 0 stores each chunk sorted by gathered index, 1 keeps
+    # storage order; spmv_chunk sets the nonzeros staged per workgroup pass
+    # (and so the tiling).  The per-row summation order is the same in every
+    # case and the reference check reduces with max, so every result equals
+    # the oracle's bit for bit.
     probs = _mixed_batch()
     c = cfg(max_iterations=300, check_frequency=25)
-    a = S.solve_batch(probs, c)
-    c.spmv_layout = 1
-    b = S.solve_batch(probs, c)
-    for p, s, t in zip(probs, a, b):
-        assert s.simplex_iterations == t.simplex_iterations, p.name
-        assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
+    refs = [oracle_of(p, c) for p in probs]
+    for layout in (0, 1):
+        for chunk in (0, 1024, 2048, 4096):
+            c.spmv_layout, c.spmv_chunk = layout, chunk
+            for p, s, r in zip(probs, S.solve_batch(probs, c), refs):
+                assert_bitexact(s, r, f"{p.name} layout={layout} chunk={chunk}")
