@@ -183,6 +183,16 @@ def bench_large(args, mode):
                "ms_per_step": 1e3 * sol.time_solver_sec / args.steps, "dtype": "fp32",
                "config": {"workload": "large-ref: random equality LP m=500000 n=1200000 10 nnz/col, "
                                       "reference-mode PDHG, step = one iteration"}}
+        # the same iterations with rocSPARSE products (bench/cusparse_pdhg_baseline.cu counterpart)
+        from sih_solver import vendor
+
+        out["vendor_rocsparse"] = {}
+        for name, mode_v in (("transpose_op", vendor.TRANSPOSE_OP), ("explicit_csc", vendor.EXPLICIT_CSC)):
+            v = vendor.run(p, args.steps, mode_v)
+            out["vendor_rocsparse"][name] = {"us_per_iter": v.us_per_iter, "us_spmv_t": v.us_spmv_t,
+                                             "us_spmv": v.us_spmv}
+            log(f"rocSPARSE {name}: {v.us_per_iter:.1f} us/iter (A^T y {v.us_spmv_t:.1f} us, "
+                f"A x {v.us_spmv:.1f} us); engine {1e6 * sol.time_solver_sec / args.steps:.1f} us/iter")
     else:
         cfg = S.GpuPdhgConfig.enhanced(tol=1e-4, max_iterations=args.steps, profile=True)
         sol = S.solve_gpu_pdhg(p, cfg)

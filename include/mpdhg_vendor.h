@@ -1,0 +1,44 @@
+/*
+ * mpdhg_vendor.h -- bench-only vendor baseline (libmpdhg_vendor.so).
+ *
+ * The reference's bench/cusparse_pdhg_baseline.cu:60-255 runs a fixed number
+ * of reference PDHG iterations (diagonal steps 0.95 / max(1, ||a||_1), the
+ * updates of cuda_pdhg.cu:54-77 and :82-103) with the vendor sparse library
+ * doing both products, and prints microseconds per iteration and the final
+ * objective.  This library is the ROCm counterpart with rocSPARSE: a
+ * comparison point for the engine's fused kernels, not part of the product
+ * (libmpdhg.so does not link it).  Its summation order is rocSPARSE's, so
+ * its iterates equal the engine's only up to FP32 rounding.
+ */
+#ifndef MPDHG_VENDOR_H
+#define MPDHG_VENDOR_H
+
+#include "mpdhg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDHG_VENDOR_TRANSPOSE_OP 0  /* A^T y as the transposed product of the CSR (the reference's way) */
+#define MPDHG_VENDOR_EXPLICIT_CSC 1  /* A^T y as a product with an explicit CSR of A^T */
+
+typedef struct mpdhg_vendor_result {
+    double us_per_iter;   /* wall time of the iteration loop / iterations (HIP events) */
+    double us_spmv_t;     /* A^T y alone, mean over 20 launches */
+    double us_spmv;       /* A xbar alone, mean over 20 launches */
+    double objective;     /* sum_j c_j x_j of the final x */
+} mpdhg_vendor_result;
+
+/* bench/cusparse_pdhg_baseline.cu:60-255 main(): `iterations` reference PDHG
+ * iterations from x = clamp(0, l, u), y = 0 on device 0.  x (optional, [n])
+ * receives the final primal iterate.  Returns 0 or a negative MPDHG_E_* code
+ * (message in mpdhg_vendor_last_error()). */
+int mpdhg_vendor_pdhg(const mpdhg_lp *lp, int iterations, int transpose_mode, double *x,
+                      mpdhg_vendor_result *out);
+const char *mpdhg_vendor_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPDHG_VENDOR_H */

@@ -1,0 +1,72 @@
+"""ctypes binding of the bench-only rocSPARSE baseline (include/mpdhg_vendor.h).
+
+bench/cusparse_pdhg_baseline.cu:60-255 counterpart: a fixed number of
+reference PDHG iterations with rocSPARSE products, for comparison with the
+engine's fused kernels.  Not used by any solve path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import Problem
+from ._abi import MpdhgLp
+
+LIB_PATH = Path(os.environ.get("MPDHG_VENDOR_LIB",
+                               Path(__file__).resolve().parent.parent / "lib" / "libmpdhg_vendor.so"))
+TRANSPOSE_OP = 0   # A^T y as the transposed product of the CSR (the reference's way)
+EXPLICIT_CSC = 1   # A^T y with an explicit CSR of A^T
+
+
+class MpdhgVendorResult(C.Structure):
+    _fields_ = [("us_per_iter", C.c_double), ("us_spmv_t", C.c_double), ("us_spmv", C.c_double),
+                ("objective", C.c_double)]
+
+
+SIGNATURES = {
+    "mpdhg_vendor_pdhg": (C.c_int, [C.POINTER(MpdhgLp), C.c_int, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(MpdhgVendorResult)]),
+    "mpdhg_vendor_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libmpdhg_vendor.so not found at {LIB_PATH}; build with "
+                               "make -C refinery-optimizer_amd/csrc")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            getattr(h, name).restype = res
+            getattr(h, name).argtypes = args
+        _lib = h
+    return _lib
+
+
+@dataclass
+class VendorRun:
+    us_per_iter: float
+    us_spmv_t: float
+    us_spmv: float
+    objective: float
+    x: np.ndarray
+
+
+def run(problem: Problem, iterations: int, mode: int = TRANSPOSE_OP) -> VendorRun:
+    lp, keep = problem._c_view()
+    x = np.zeros(problem.num_cols())
+    r = MpdhgVendorResult()
+    rc = lib().mpdhg_vendor_pdhg(C.byref(lp), iterations, mode,
+                                 x.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r))
+    if rc != 0:
+        raise RuntimeError(f"mpdhg_vendor_pdhg failed (code {rc}): "
+                           f"{lib().mpdhg_vendor_last_error().decode(errors='replace')}")
+    del keep
+    return VendorRun(r.us_per_iter, r.us_spmv_t, r.us_spmv, r.objective, x)

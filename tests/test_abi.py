@@ -1,0 +1,262 @@
+"""The C ABI of libmpdhg.so without a GPU: exports, struct layouts, host logic.
+
+* every function include/*.h declares is exported by the built library, and
+  the ctypes binding declares exactly the functions of include/mpdhg.h;
+* the ctypes structures have the size and field offsets the C compiler gives
+  the header's structures (gcc on a generated probe);
+* configuration defaults are the reference's (gpu_solver.hpp:12-22) and
+  invalid extension values are rejected before any device work;
+* without a device the solve entry points fail loudly (no CPU fallback);
+* the seeded generators equal an independent Python restatement of their
+  specification (SplitMix64-seeded xoshiro256**, Floyd sampling) bit for bit.
+"""
+import ctypes as C
+import math
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import sih_solver as S
+from sih_solver import _abi
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+DECL = re.compile(r"^[A-Za-z_][\w\s\*]*?\b(mpdhg_\w+)\s*\(", re.MULTILINE)
+
+
+def declared(path):
+    text = re.sub(r"/\*.*?\*/", "", path.read_text(), flags=re.DOTALL)
+    return sorted(set(DECL.findall(text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = C.CDLL(str(_abi.LIB_PATH))
+    names = declared(ROOT / "include" / "mpdhg.h")
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_abi.SIGNATURES) == names
+
+
+def test_other_headers_are_exported_by_their_libraries():
+    libs = {"mpdhg_vendor.h": ROOT / "refinery-optimizer_amd" / "lib" / "libmpdhg_vendor.so"}
+    for h in HEADERS:
+        if h.name == "mpdhg.h":
+            continue
+        assert h.name in libs, f"no library recorded for {h.name}"
+        lib = C.CDLL(str(libs[h.name]))
+        for n in declared(h):
+            assert hasattr(lib, n), (h.name, n)
+    from sih_solver import vendor
+
+    assert sorted(vendor.SIGNATURES) == declared(ROOT / "include" / "mpdhg_vendor.h")
+
+
+def _structs():
+    from sih_solver import vendor
+
+    return {"mpdhg_config": _abi.MpdhgConfig, "mpdhg_lp": _abi.MpdhgLp,
+            "mpdhg_result": _abi.MpdhgResult, "mpdhg_adjoint_io": _abi.MpdhgAdjointIO,
+            "mpdhg_vendor_result": vendor.MpdhgVendorResult}
+
+
+def test_ctypes_layouts_match_the_header(tmp_path):
+    STRUCTS = _structs()
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "mpdhg.h"', '#include "mpdhg_vendor.h"',
+             "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if line:
+            a, b, v = line.split()
+            got[(a, b)] = int(v)
+    for cname, cls in STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
+def test_config_defaults_are_the_reference_defaults():
+    c = _abi.MpdhgConfig()
+    _abi.lib().mpdhg_config_default(C.byref(c))
+    # GpuPdhgConfig{} (gpu_solver.hpp:12-22)
+    assert c.max_iterations == 4000 and c.check_frequency == 25
+    assert np.float32(c.tol_primal) == np.float32(1e-4) == np.float32(c.tol_dual) == np.float32(c.tol_gap)
+    assert c.step_tau == 0.0 and c.step_sigma == 0.0
+    assert c.enable_restart == 1 and c.enable_hybrid_polish == 1
+    assert c.mode == _abi.MPDHG_MODE_REFERENCE and c.spmv_layout == 0 and c.spmv_chunk == 0
+    py = S.GpuPdhgConfig()
+    for name, _ in _abi.MpdhgConfig._fields_:
+        assert math.isclose(float(getattr(py, name)), float(getattr(c, name)), rel_tol=1e-6), name
+
+
+@pytest.mark.parametrize("field,value", [("spmv_chunk", 3000), ("spmv_layout", 2), ("mode", 7),
+                                         ("check_frequency", 0)])
+def test_invalid_configuration_is_rejected(field, value):
+    # mpdhg_solve with a NULL context validates the configuration before it
+    # creates a temporary context, so this holds with or without a device.
+    p = S.gen_random(20, 50, 4, 1)
+    c = S.GpuPdhgConfig()
+    setattr(c, field, value)
+    lp, keep = p._c_view()
+    res = _abi.MpdhgResult()
+    rc = _abi.lib().mpdhg_solve(None, C.byref(lp), C.byref(c._c()), C.byref(res))
+    assert rc == -1  # MPDHG_E_ARG
+    assert field in _abi.lib().mpdhg_last_error().decode() or "mode" in _abi.lib().mpdhg_last_error().decode()
+
+
+def test_no_device_fails_loudly():
+    if S.GpuSolver.is_cuda_available():
+        pytest.skip("a HIP device is visible")
+    p = S.gen_random(20, 50, 4, 1)
+    with pytest.raises(_abi.MpdhgError):
+        S.solve_gpu_pdhg(p)
+    with pytest.raises(_abi.MpdhgError):
+        S.solve_batch([p], S.GpuPdhgConfig.enhanced())
+    assert S.GpuSolver.get_vram_info().total_bytes == 0
+
+
+# ---- generator restatement ---------------------------------------------------
+M64 = (1 << 64) - 1
+
+
+class Xoshiro:
+    """xoshiro256** seeded through SplitMix64 (Blackman & Vigna)."""
+
+    def __init__(self, seed):
+        st = seed & M64
+        self.s = []
+        for _ in range(4):
+            st = (st + 0x9E3779B97F4A7C15) & M64
+            z = st
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+            self.s.append(z ^ (z >> 31))
+
+    @staticmethod
+    def rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & M64
+
+    def next(self):
+        s = self.s
+        r = (self.rotl((s[1] * 5) & M64, 7) * 9) & M64
+        t = (s[1] << 17) & M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self.rotl(s[3], 45)
+        return r
+
+    def u01(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+    def uni(self, a, b):
+        return a + (b - a) * self.u01()
+
+    def below(self, k):
+        return self.next() % k
+
+    def normal(self):
+        u1 = self.u01()
+        u2 = self.u01()
+        u1 = max(u1, 1e-300)
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(6.283185307179586 * u2)
+
+    def sample(self, n, k):  # Floyd: k distinct values of [0, n), ascending
+        out = []
+        for t in range(n - k, n):
+            r = self.below(t + 1)
+            out.append(t if r in out else r)
+        return sorted(out)
+
+
+def random_lp_spec(m, n, k, seed):
+    """cfg0 generator: 8 (k) distinct rows per column, U[-1,1] values, 0 <= x <= 10,
+    b = A x0 with x0 ~ U[0,10], c = A^T y0 + s, y0 ~ N(0,1), s ~ U[0,1] with
+    floor(0.4 n) entries zeroed."""
+    g = Xoshiro(seed)
+    cp, ri, v = [0], [], []
+    for _ in range(n):
+        for r in g.sample(m, k):
+            ri.append(r)
+            v.append(g.uni(-1.0, 1.0))
+        cp.append(len(ri))
+    x0 = [g.uni(0.0, 10.0) for _ in range(n)]
+    b = [0.0] * m
+    for j in range(n):
+        for p in range(cp[j], cp[j + 1]):
+            b[ri[p]] += v[p] * x0[j]
+    y0 = [g.normal() for _ in range(m)]
+    s = [g.u01() for _ in range(n)]
+    for j in g.sample(n, (n * 4) // 10):
+        s[j] = 0.0
+    c = []
+    for j in range(n):
+        acc = 0.0
+        for p in range(cp[j], cp[j + 1]):
+            acc += v[p] * y0[ri[p]]
+        c.append(acc + s[j])
+    return np.array(cp), np.array(ri), np.array(v), np.array(b), np.array(c)
+
+
+@pytest.mark.parametrize("m,n,k,seed", [(12, 30, 4, 0), (40, 96, 8, 7), (5, 9, 8, 3)])
+def test_random_generator_matches_its_specification(m, n, k, seed):
+    p = S.gen_random(m, n, k, seed)
+    cp, ri, v, b, c = random_lp_spec(m, n, min(k, m), seed)
+    A = p.A
+    assert np.array_equal(A.csc_col_ptr(), cp) and np.array_equal(A.csc_row_ind(), ri)
+    assert np.array_equal(A.csc_values(), v)
+    assert np.array_equal(p.row_lower, b) and np.array_equal(p.row_upper, b)
+    assert np.array_equal(p.c, c)
+    assert np.all(p.col_lower == 0.0) and np.all(p.col_upper == 10.0)
+
+
+def test_staircase_generator_structure():
+    T, R, Cc = 3, 40, 96
+    p = S.gen_staircase(T, R, Cc, 5)
+    P, Q = R // 2, (3 * R) // 10
+    NC = Cc - 2 * P
+    assert (p.num_rows(), p.num_cols()) == (T * R, T * Cc)
+    A = p.A
+    cp, ri, v = A.csc_col_ptr(), A.csc_row_ind(), A.csc_values()
+    for t in range(T):
+        r0, c0 = t * R, t * Cc
+        assert np.all(p.row_lower[r0:r0 + P] == 0) and np.all(p.row_upper[r0:r0 + P] == 0)
+        assert np.all(np.isneginf(p.row_lower[r0 + P:r0 + R]))
+        for j in range(c0, c0 + NC):  # crude: yields <= 1 in total, one unit row
+            rows, vals = ri[cp[j]:cp[j + 1]], v[cp[j]:cp[j + 1]]
+            y = vals[rows < r0 + P]
+            assert 4 <= len(y) <= 8 and y.sum() <= 1.0 + 1e-12 and np.all(y > 0)
+            assert np.sum(rows >= r0 + P + Q) == 1
+            assert 40 <= p.c[j] <= 90
+        for j in range(c0 + NC, c0 + NC + P):  # sales: revenue, -1 in the balance
+            assert -120 <= p.c[j] <= -60 and v[cp[j]] == -1.0
+        for j in range(c0 + NC + P, c0 + Cc):  # inventory: -1 now, +1 next block
+            rows, vals = ri[cp[j]:cp[j + 1]], v[cp[j]:cp[j + 1]]
+            assert vals[0] == -1.0 and rows[0] == r0 + (j - c0 - NC - P)
+            assert (len(rows) == 2) == (t + 1 < T)
+    assert np.all(p.col_lower == 0) and np.all((0 <= p.col_upper) & (p.col_upper <= 1000))
+
+
+def test_scenario_generator_shocks_costs_and_zeroes_one_capacity():
+    base = S.gen_staircase(2, 40, 96, 1)
+    sc = S.gen_scenario("staircase", (2, 40, 96), 1, 9, 0.3, True)
+    ratio = sc.c / base.c
+    assert np.all(ratio > 0) and not np.allclose(ratio, 1.0)
+    changed = np.flatnonzero(sc.row_upper != base.row_upper)
+    assert len(changed) == 1 and sc.row_upper[changed[0]] == 0.0
+    assert np.array_equal(sc.A.csc_values(), base.A.csc_values())
