@@ -84,6 +84,10 @@ typedef struct mpdhg_config {
     int32_t spmv_chunk;           /* nonzeros staged through LDS per workgroup pass: 0 = auto
                                      (the smallest of 1024/2048/4096 holding 256 rows of average
                                      length, per direction), or 1024, 2048, 4096 */
+    int32_t launch_graph;         /* enhanced: 1 = record one check cycle (the fused launch pairs,
+                                     the FP64 check, the decisions) as a HIP graph and replay it
+                                     for every full cycle; 0 = launch kernel by kernel.  Same
+                                     results either way. */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite

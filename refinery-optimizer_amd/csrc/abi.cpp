@@ -51,6 +51,8 @@ int check_config(const mpdhg_config *cfg) {
         return mpdhg::fail(MPDHG_E_ARG, "spmv_layout must be 0 (auto) or 1 (storage order)");
     if (cfg->spmv_chunk != 0 && !mpdhg::valid_chunk(cfg->spmv_chunk))
         return mpdhg::fail(MPDHG_E_ARG, "spmv_chunk must be 0 (auto), 1024, 2048 or 4096");
+    if (cfg->launch_graph != 0 && cfg->launch_graph != 1)
+        return mpdhg::fail(MPDHG_E_ARG, "launch_graph must be 0 or 1");
     return MPDHG_OK;
 }
 
@@ -111,6 +113,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->refine_ratio = 1e-2;
     cfg->spmv_layout = 0;
     cfg->spmv_chunk = 0;
+    cfg->launch_graph = 1;
 }
 
 int mpdhg_device_count(int *count) {

@@ -345,8 +345,20 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     int used = 0;
 
     // ---- hot loop (gpu_solver.cpp:227-271) ----
-    const double t_loop = now_sec();
+    // The host reads each check's status snapshot one check late, while the
+    // next F iterations are already queued: the kernels skip instances that
+    // have stopped, so iterations queued after the last instance stopped
+    // change nothing, and the device never drains between checks.
     std::vector<RefInst> hs(static_cast<size_t>(na));
+    Pinned snap(2 * hs.size() * sizeof(RefInst));
+    RefInst *const snaps[2] = {static_cast<RefInst *>(snap.p), static_cast<RefInst *>(snap.p) + na};
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (auto &v : ev.e) MPDHG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    int nchk = 0;
+    const double t_loop = now_sec();
     int iter;
     for (iter = 1; iter <= K; ++iter) {
         const bool samp = nsamp && (iter % stride == 0) && used < nsamp;
@@ -371,12 +383,16 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
                                  cfg.tol_primal, cfg.tol_dual);
             launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
             MPDHG_HIP(hipGetLastError());
-            MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(RefInst),
+            const int cur = nchk & 1;
+            MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(RefInst),
                                      hipMemcpyDeviceToHost, st));
-            MPDHG_HIP(hipStreamSynchronize(st));
-            bool any = false;
-            for (const auto &s : hs) any = any || (s.status == kRunning);
-            if (!any) break;
+            MPDHG_HIP(hipEventRecord(ev.e[cur], st));
+            if (nchk++ > 0) {
+                MPDHG_HIP(hipEventSynchronize(ev.e[cur ^ 1]));
+                bool any = false;
+                for (int a = 0; a < na; ++a) any = any || (snaps[cur ^ 1][a].status == kRunning);
+                if (!any) break;
+            }
         }
     }
     MPDHG_HIP(hipStreamSynchronize(st));

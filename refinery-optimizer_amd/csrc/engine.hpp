@@ -72,6 +72,15 @@ class DBuf {
     size_t n_ = 0;
 };
 
+// Pinned host memory (hipHostMalloc), for asynchronous device-to-host polls.
+struct Pinned {
+    void *p = nullptr;
+    explicit Pinned(size_t bytes) { MPDHG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
+    ~Pinned() { if (p) (void)hipHostFree(p); }
+    Pinned(const Pinned &) = delete;
+    Pinned &operator=(const Pinned &) = delete;
+};
+
 // One LP validated and converted to int32 CSC (the layout every kernel
 // reads), with its CSR built by counting sort (gpu_solver.cpp:92-109).
 struct HostLp {

@@ -83,18 +83,6 @@ void parallel_for(int count, F &&f) {
 
 void parallel_for_each(int count, const std::function<void(int)> &f) { parallel_for(count, f); }
 
-namespace {
-
-struct Pinned {
-    void *p = nullptr;
-    explicit Pinned(size_t bytes) { MPDHG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
-    ~Pinned() { if (p) (void)hipHostFree(p); }
-    Pinned(const Pinned &) = delete;
-    Pinned &operator=(const Pinned &) = delete;
-};
-
-}  // namespace
-
 void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
                     mpdhg_result *res) {
     const double t_call = now_sec();
@@ -324,13 +312,18 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     MPDHG_HIP(hipMemsetAsync(d_yp.get(), 0, R * sizeof(float), st));
 
     // ---- iteration loop ----
+    // The first iteration of the current check cycle lives on the device
+    // (advanced by the last launch of each cycle), so one recorded cycle
+    // replays unchanged for every full cycle.
+    DBuf<int> d_itb(1);
+    MPDHG_HIP(hipMemsetAsync(d_itb.get(), 0, sizeof(int), st));
     const float rho = cfg.reflection;
     launch::EnhPrimal P{d_cptr.get(), d_cidx.get(), d_cval.get(), d_y.get(), d_yp.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_xbar.get(),
-                        d_xbase.get(), d_dc.get(), d_xchk.get(), d_pc.get(), rho};
+                        d_xbase.get(), d_dc.get(), d_xchk.get(), d_pc.get(), d_itb.get(), rho};
     launch::EnhDual D{d_rptr.get(), d_ridx.get(), d_rval.get(), d_xbar.get(), d_rls.get(), d_rus.get(),
                       d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_ybase.get(), d_dr.get(),
-                      d_ychk.get(), d_pr.get(), rho};
+                      d_ychk.get(), d_pr.get(), d_itb.get(), rho};
     launch::EnhCheck CK{d_rptr.get(), d_ridx.get(), d_cptr.get(), d_cidx.get(), d_rval64.get(),
                         d_cval64.get(), d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
                         d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
@@ -338,7 +331,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                         cfg.max_refinements, d_xbest.get(), d_lambest.get(),
                         d_ybest.get(), d_axbest.get(), d_xbase.get(), d_ybase.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_rls.get(),
-                        d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get()};
+                        d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_itb.get()};
     const int nsamp = cfg.profile ? 256 : 0;
     if (nsamp) {
         while (ctx.events.size() < static_cast<size_t>(3 * nsamp + 4)) {
@@ -361,22 +354,71 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     std::vector<DBuf<Tile>> keep_rt, keep_ct;  // compacted tile lists, alive until the end
     std::vector<DBuf<int32_t>> keep_rng;
     std::vector<std::vector<int32_t>> host_keep;
+    // One check cycle of len iterations in stream order: F fused launch pairs,
+    // the FP64 check and decisions, the iteration advance.
+    auto issue_cycle = [&](int len) {
+        for (int i = 0; i < len; ++i) {
+            const int check = (i + 1 == len);
+            launch::enh_primal(st, cts, d_inst.get(), P, i, check);
+            launch::enh_dual(st, rts, d_inst.get(), D, i, check);
+        }
+        launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
+        launch::enh_advance(st, d_itb.get(), len);
+    };
+    // Full cycles replay a HIP graph recorded from issue_cycle(F); it is
+    // recorded again after each tile-list compaction (new lists and grids).
+    struct CycleGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        void reset() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            e = nullptr;
+            g = nullptr;
+        }
+        ~CycleGraph() { reset(); }
+    } graph;
+    int lists = 0, graph_lists = -1;  // tile-list version, and the one the graph was recorded for
     try {
         while (it < K && !finished) {
-            const int end = std::min(K, (it / F + 1) * F);
-            for (; it < end; ++it) {
-                const int check = (it + 1 == end);
-                const bool samp = nsamp && used < nsamp && it >= F && covered == na && !check;
-                if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
-                launch::enh_primal(st, cts, d_inst.get(), P, it, check);
-                if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
-                launch::enh_dual(st, rts, d_inst.get(), D, it, check);
-                if (samp) {
-                    MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
-                    ++used;
+            const int len = std::min(F, K - it);
+            if (nsamp && used < nsamp && it >= F && covered == na) {
+                // a profiled cycle: eager launches with events around the fused pairs
+                for (int i = 0; i < len; ++i) {
+                    const int check = (i + 1 == len);
+                    const bool samp = used < nsamp && !check;
+                    if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
+                    launch::enh_primal(st, cts, d_inst.get(), P, i, check);
+                    if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
+                    launch::enh_dual(st, rts, d_inst.get(), D, i, check);
+                    if (samp) {
+                        MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
+                        ++used;
+                    }
                 }
+                launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
+                launch::enh_advance(st, d_itb.get(), len);
+            } else if (cfg.launch_graph && len == F) {
+                if (graph_lists != lists) {
+                    graph.reset();
+                    MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    try {
+                        issue_cycle(F);
+                    } catch (...) {
+                        hipGraph_t partial = nullptr;
+                        (void)hipStreamEndCapture(st, &partial);
+                        if (partial) (void)hipGraphDestroy(partial);
+                        throw;
+                    }
+                    MPDHG_HIP(hipStreamEndCapture(st, &graph.g));
+                    MPDHG_HIP(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
+                    graph_lists = lists;
+                }
+                MPDHG_HIP(hipGraphLaunch(graph.e, st));
+            } else {
+                issue_cycle(len);
             }
-            launch::enh_check(st, rts, cts, d_inst.get(), na, CK, end - 1);
+            it += len;
             MPDHG_HIP(hipGetLastError());
             const int slot = cycle & 1;
             MPDHG_HIP(hipMemcpyAsync(slots + slot * na, d_status.get(), static_cast<size_t>(na) * sizeof(int32_t),
@@ -414,6 +456,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn);
                     cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn);
                     covered = running;
+                    ++lists;
                     host_keep.push_back(std::move(rng));
                 }
             }

@@ -96,13 +96,16 @@ struct PrimalArgs {
     const double *xbase, *dc;
     double *xchk;
     double *pc;
+    const int *itb;  // first iteration of the current check cycle (device)
     float rho;
 };
 
+// Iteration it = *A.itb + off: the iteration number lives on the device, so
+// one recorded check cycle (a HIP graph) replays for every cycle.
 template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
                                                        int chunk, const EnhInst *__restrict__ st,
-                                                       PrimalArgs A, int it, int check) {
+                                                       PrimalArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
@@ -123,6 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
     }
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
+    const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
     const float g = tile_dot_rows<TN, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
     float a, b;
@@ -171,13 +175,14 @@ struct DualArgs {
     const double *ybase, *dr;
     double *ychk;
     double *pr;
+    const int *itb;
     float rho;
 };
 
 template <int TN>
 __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const EnhInst *__restrict__ st,
-                                                     DualArgs A, int it, int check) {
+                                                     DualArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     __shared__ double red[kBlock / 64];
     const int t = xcd_tile(blockIdx.x, chunk);
@@ -196,6 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
     }
     const EnhInst &s = st[tl.inst];
     if (s.status != kRunning) return;
+    const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
     const float h = tile_dot_rows<TN, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
     float a, b;
@@ -339,6 +345,7 @@ struct FinalizeArgs {
     int32_t *status_out;
     double refine_ratio;
     int max_refinements;
+    const int *itb;
 };
 
 __device__ __forceinline__ void set_steps(EnhInst &s) {
@@ -348,7 +355,7 @@ __device__ __forceinline__ void set_steps(EnhInst &s) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ st, int ninst,
-                                                         FinalizeArgs A, int it) {
+                                                         FinalizeArgs A, int off) {
     __shared__ double red[kBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ s
     dobjr = block_sum_d(dobjr, red);
     if (threadIdx.x != 0) return;
 
-    const int done = it + 1;
+    const int done = *A.itb + off + 1;
     const double dobj = dobjr + dobjc;
     const double ep = fmax(rv, cv) / (1.0 + s.bnorm);
     const double ed = fmax(dv, sv) / (1.0 + s.cnorm);
@@ -587,6 +594,11 @@ __global__ __launch_bounds__(kBlock) void k_enh_set_ranges(EnhInst *__restrict__
     st[q].col_tile_end = rng[4 * q + 3];
 }
 
+// End of a check cycle of `len` iterations: the next cycle starts at *itb.
+__global__ void k_enh_advance(int *__restrict__ itb, int len) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *itb += len;
+}
+
 }  // namespace dev
 
 // ---- launchers --------------------------------------------------------------
@@ -612,27 +624,30 @@ void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cid
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc) {
     hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kBlock), 0, s, st, ninst, pc);
 }
-void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int it,
+void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
                 int check) {
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
-                      a.xbar, a.xbase, a.dc, a.xchk, a.pc, a.rho};
-    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, it, check);
+                      a.xbar, a.xbase, a.dc, a.xchk, a.pc, a.itb, a.rho};
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, off, check);
 }
-void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
+void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
               int check) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
-                    a.ybase, a.dr, a.ychk, a.pr, a.rho};
-    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, it, check);
+                    a.ybase, a.dr, a.ychk, a.pr, a.itb, a.rho};
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
+}
+void enh_advance(hipStream_t s, int *itb, int len) {
+    hipLaunchKernelGGL(dev::k_enh_advance, dim3(1), dim3(64), 0, s, itb, len);
 }
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
-               const EnhCheck &a, int it) {
+               const EnhCheck &a, int off) {
     dev::CheckRowArgs R{a.rptr, a.ridx, a.rval64, a.xchk, a.ychk, a.rl64, a.ru64, a.ax, a.pr};
     dev::CheckColArgs Cc{a.cptr, a.cidx, a.cval64, a.ychk, a.xchk, a.c64, a.l64, a.u64, a.lam,
                          a.pc};
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_chk_rows, st, R);
     if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_chk_cols, st, Cc);
-    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.max_refinements};
-    hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, it);
+    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.max_refinements, a.itb};
+    hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, off);
     if (cts.count)
         hipLaunchKernelGGL(dev::k_enh_best_cols, MPDHG_TILE_GRID(cts), st, a.xchk, a.lam, a.xbest,
                            a.lambest);

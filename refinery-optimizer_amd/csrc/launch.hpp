@@ -46,6 +46,7 @@ struct EnhPrimal {
     const double *xbase, *dc;
     double *xchk;
     double *pc;
+    const int *itb;  // device: first iteration of the current check cycle
     float rho;
 };
 struct EnhDual {
@@ -57,6 +58,7 @@ struct EnhDual {
     const double *ybase, *dr;
     double *ychk;
     double *pr;
+    const int *itb;
     float rho;
 };
 struct EnhCheck {
@@ -70,6 +72,7 @@ struct EnhCheck {
     int max_refinements;
     double *xbest, *lambest, *ybest, *axbest, *xbase, *ybase;
     float *cs, *ls, *us, *x, *x0, *xp, *rls, *rus, *ybar, *y, *y0, *yp;
+    const int *itb;
 };
 
 void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v);
@@ -78,17 +81,20 @@ void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *rid
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
               const float *cval, const float *w, float *z, double *pc);
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc);
-void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int it,
+// Iteration *a.itb + off of the current check cycle.
+void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
                 int check);
-void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int it,
+void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
               int check);
+// *itb += len (end of a check cycle of len iterations).
+void enh_advance(hipStream_t s, int *itb, int len);
 // Point instance q's partial-reduction ranges at rng[4q..4q+3] (row tile
 // begin/end, column tile begin/end) after a tile-list compaction.
 void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)
 // residual-LP construction, in that stream order.
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
-               const EnhCheck &a, int it);
+               const EnhCheck &a, int off);
 
 // adjoint: CGLS on A_RB^T mu = w_B in the scaled space
 struct AdjColArgs {

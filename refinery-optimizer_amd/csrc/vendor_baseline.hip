@@ -157,7 +157,14 @@ int run_baseline(const mpdhg_lp *lp, int iters, int mode, double *xout, mpdhg_ve
     hip_ok(hipMemsetAsync(dy.p, 0, static_cast<size_t>(m) * sizeof(float), st), "memset");
 
     rocsparse_handle hd;
-    sp_ok(rocsparse_create_handle(&hd), "handle");
+    {
+        const hipError_t pending = hipGetLastError();
+        const rocsparse_status hs = rocsparse_create_handle(&hd);
+        if (hs != rocsparse_status_success)
+            throw Fail{MPDHG_E_HIP, std::string("rocsparse_create_handle: status ") +
+                                        std::to_string(static_cast<int>(hs)) + " (HIP error pending before: " +
+                                        hipGetErrorName(pending) + ", after: " + hipGetErrorName(hipGetLastError()) + ")"};
+    }
     struct HandleGuard { rocsparse_handle h; ~HandleGuard() { (void)rocsparse_destroy_handle(h); } } hg{hd};
     sp_ok(rocsparse_set_stream(hd, st), "set_stream");
     rocsparse_spmat_descr A, At = nullptr;

@@ -294,6 +294,7 @@ class GpuPdhgConfig:
     refine_ratio: float = 1e-2
     spmv_layout: int = 0  # 0: gather order where it applies, 1: storage order
     spmv_chunk: int = 0  # nonzeros per LDS pass: 0 = auto, or 1024 / 2048 / 4096
+    launch_graph: int = 1  # enhanced: replay each check cycle as a recorded HIP graph
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,

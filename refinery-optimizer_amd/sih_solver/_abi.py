@@ -37,6 +37,7 @@ class MpdhgConfig(C.Structure):
         ("refine_ratio", C.c_double),
         ("spmv_layout", C.c_int32),
         ("spmv_chunk", C.c_int32),
+        ("launch_graph", C.c_int32),
     ]
 
 

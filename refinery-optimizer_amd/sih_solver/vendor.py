@@ -37,8 +37,17 @@ _lib = None
 
 
 def lib() -> C.CDLL:
+    """Load libmpdhg_vendor.so.  Refuses a process that has imported torch:
+    torch bundles its own HIP runtime and rocSPARSE, the dynamic linker then
+    binds this library's rocSPARSE to torch's copy, and rocSPARSE fails on
+    the other runtime's device state."""
     global _lib
     if _lib is None:
+        import sys
+
+        if "torch" in sys.modules:
+            raise RuntimeError("the rocSPARSE baseline must run in a process that has not "
+                               "imported torch (torch bundles its own ROCm libraries)")
         if not LIB_PATH.exists():
             raise RuntimeError(f"libmpdhg_vendor.so not found at {LIB_PATH}; build with "
                                "make -C refinery-optimizer_amd/csrc")

@@ -103,7 +103,7 @@ def test_config_defaults_are_the_reference_defaults():
 
 
 @pytest.mark.parametrize("field,value", [("spmv_chunk", 3000), ("spmv_layout", 2), ("mode", 7),
-                                         ("check_frequency", 0)])
+                                         ("check_frequency", 0), ("launch_graph", 2)])
 def test_invalid_configuration_is_rejected(field, value):
     # mpdhg_solve with a NULL context validates the configuration before it
     # creates a temporary context, so this holds with or without a device.

@@ -108,3 +108,21 @@ def test_gather_order_layout_is_bit_identical(need_gpu):
         assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
         assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
         assert s.rel_kkt == t.rel_kkt, p.name
+
+
+def test_graph_replay_equals_eager_launches(need_gpu):
+    # Full check cycles replay a recorded HIP graph, re-recorded after each
+    # tile-list compaction; eager work (status copies, compactions) runs
+    # between replays.  Every result must equal kernel-by-kernel launches.
+    probs = [S.gen_random(200, 480, 8, 31), S.gen_staircase(1, 200, 480, 32), toy01(),
+             random_lp(80, 120, 0.08, 33)[0], S.gen_random(600, 1440, 8, 34),
+             S.gen_staircase(2, 100, 240, 35)]
+    cfg = enh(tol=1e-6)
+    g = S.solve_batch(probs, cfg)
+    cfg.launch_graph = 0
+    e = S.solve_batch(probs, cfg)
+    assert len({s.simplex_iterations for s in g}) > 2  # instances stop in different cycles
+    for p, s, t in zip(probs, g, e):
+        assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
+        assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
+        assert s.rel_kkt == t.rel_kkt and s.status == t.status, p.name
