@@ -95,13 +95,14 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int 
     }
 }
 
-TileSet make_tileset(const DBuf<Tile> &d, int count, int tn) {
+TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather) {
     TileSet ts;
     ts.tiles = d.get();
     ts.count = count;
     ts.chunk = (count + 7) / 8;
     ts.grid = 8 * ts.chunk;
     ts.tn = tn;
+    ts.gather = gather;
     return ts;
 }
 
@@ -305,6 +306,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         plan_tiles(h.cptr, h.n, co, a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
+    bool rgo = false, cgo = false;
     {
         std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
         for (int a = 0; a < na; ++a) {
@@ -313,8 +315,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
-        gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
+        rgo = gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
+        cgo = gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
     }
     // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
     hipStream_t st = ctx.stream;
@@ -333,8 +335,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
     DBuf<float> d_prow(hh.rtiles.size()), d_pcol(hh.ctiles.size());
-    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn);
-    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn);
+    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn, rgo);
+    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn, cgo);
     MPDHG_HIP(hipStreamSynchronize(st));
 
     // ---- profiling samples: HIP events around the fused SpMV launches ----

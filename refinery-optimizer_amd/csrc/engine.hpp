@@ -100,7 +100,7 @@ struct BatchLayout {
 // kBlock rows and at most tn nonzeros per tile (a longer row alone).
 void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst, int tn,
                 std::vector<Tile> &out);
-TileSet make_tileset(const DBuf<Tile> &d, int count, int tn);
+TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather);
 
 // Chunk size of a tile set over `rows` rows holding `nnz` nonzeros: the
 // smallest of 1024, 2048, 4096 that holds kBlock rows of average length, so
@@ -142,17 +142,24 @@ void permute_chunk(std::vector<V> &vals, int c0, const std::vector<uint32_t> &ke
 }
 }  // namespace detail
 
-// Puts every chunk (tn nonzeros) of every tile into gather order (see Tile)
-// when the layout is Auto and the tile's instance gathers from a vector
-// shorter than gather_local_limit(tn): the global indices in `idx` (storage
-// order) become packed (local << log2 tn | position) words sorted
+// Puts every chunk (tn nonzeros) of every tile of a tile set into gather
+// order (see Tile) when the layout is Auto and every instance gathers from a
+// vector shorter than gather_local_limit(tn): the global indices in `idx`
+// (storage order) become packed (local << log2 tn | position) words sorted
 // ascending, each value array is permuted alike, and Tile::base is set to
-// the vector's offset.  Other tiles keep storage order (base = -1).
-// vec_off / vec_len: offset and length of each instance's gathered vector.
+// the vector's offset.  Otherwise the whole set keeps storage order
+// (base = -1).  Returns whether the set is in gather order.  vec_off /
+// vec_len: offset and length of each instance's gathered vector.
 template <typename... Vs>
-void gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vector<Tile> &tiles,
+bool gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vector<Tile> &tiles,
                   const std::vector<int64_t> &vec_off, const std::vector<int64_t> &vec_len,
                   std::vector<int> &idx, std::vector<Vs> &...vals) {
+    bool all = (layout == Layout::Auto);
+    for (int64_t len : vec_len) all = all && (len < gather_local_limit(tn));
+    if (!all) {
+        for (Tile &tl : tiles) tl.base = -1;
+        return false;
+    }
     constexpr int kGroup = 64;  // tiles per work item
     const int bits = ilog2(tn);
     const uint32_t mask = static_cast<uint32_t>(tn) - 1u;
@@ -163,10 +170,6 @@ void gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vecto
         for (int t = g * kGroup; t < std::min(nt, (g + 1) * kGroup); ++t) {
             Tile &tl = tiles[static_cast<size_t>(t)];
             const size_t a = static_cast<size_t>(tl.inst);
-            if (layout != Layout::Auto || vec_len[a] >= gather_local_limit(tn)) {
-                tl.base = -1;
-                continue;
-            }
             const int64_t off = vec_off[a];
             tl.base = static_cast<int32_t>(off);
             const int lo = ptr[static_cast<size_t>(tl.begin)], hi = ptr[static_cast<size_t>(tl.end)];
@@ -183,6 +186,7 @@ void gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vecto
             }
         }
     });
+    return true;
 }
 
 // FP64 post-processing on the host for reference-mode results: A x,

@@ -110,6 +110,7 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, ctiles);
         s.col_tile_end = static_cast<int32_t>(ctiles.size());
     }
+    bool rgo = false, cgo = false;
     {
         std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
         for (int a = 0; a < na; ++a) {
@@ -117,8 +118,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
             rlen[static_cast<size_t>(a)] = h.m;
             clen[static_cast<size_t>(a)] = h.n;
         }
-        gather_order(Layout::Auto, rtn, rptr, rtiles, L.col_off, clen, ridx, rval);
-        gather_order(Layout::Auto, ctn, cptr, ctiles, L.row_off, rlen, cidx, cval);
+        rgo = gather_order(Layout::Auto, rtn, rptr, rtiles, L.col_off, clen, ridx, rval);
+        cgo = gather_order(Layout::Auto, ctn, cptr, ctiles, L.row_off, rlen, cidx, cval);
     }
     hipStream_t st = ctx.stream;
     DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
@@ -134,8 +135,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(double), st));
     MPDHG_HIP(hipMemsetAsync(d_q.get(), 0, C * sizeof(double), st));
     DBuf<double> d_pc(ctiles.size() * kPcCount), d_pr(rtiles.size() * kPrCount);
-    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rtn);
-    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), ctn);
+    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rtn, rgo);
+    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), ctn, cgo);
     launch::AdjColArgs CA{d_cptr.get(), d_cidx.get(), d_cval.get(), d_pt.get(), d_mb.get(), d_dc.get(),
                           d_q.get(), d_r.get(), d_rt.get(), d_pc.get()};
     launch::AdjRowArgs RA{d_rptr.get(), d_ridx.get(), d_rval.get(), d_rt.get(), d_mr.get(), d_dr.get(),

@@ -231,6 +231,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
+    bool rgo = false, cgo = false;
     {
         std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
         for (int a = 0; a < na; ++a) {
@@ -239,8 +240,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
-        gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
+        rgo = gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
+        cgo = gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
     }
 
     // ---- device buffers ----
@@ -279,8 +280,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
         d_status.upload(run, st);
     }
-    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn);
-    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn);
+    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn, rgo);
+    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn, cgo);
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -453,8 +454,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     keep_rng.emplace_back();
                     keep_rng.back().upload(rng, st);
                     launch::enh_set_ranges(st, d_inst.get(), na, keep_rng.back().get());
-                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn);
-                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn);
+                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn, rgo);
+                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn, cgo);
                     covered = running;
                     ++lists;
                     host_keep.push_back(std::move(rng));

@@ -71,11 +71,14 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
 // nonzeros is streamed with coalesced loads of the index words and values,
 // the gathers are issued back to back, and the (value, x) pairs are parked
 // in LDS at their storage positions (the position is k - chunk start in
-// storage order, or the low log2 TN bits of the index word in gather
-// order); each lane then walks its own row from LDS.  A row longer than a
-// chunk is continued across chunks by the same lane, so the summation order
-// never depends on the layout or on TN.  `lds` must hold stage_bytes(TN).
-template <int TN, typename V, typename X, typename A>
+// storage order, G = false, or the low log2 TN bits of the index word in
+// gather order, G = true); each lane then walks its own row from LDS.  A
+// row longer than a chunk is continued across chunks by the same lane, so
+// the summation order never depends on the layout or on TN.  `lds` must
+// hold stage_bytes(TN).  The layout is a template parameter (one per tile
+// set) so that each instantiation carries one addressing path: two paths in
+// one kernel cost a third more registers and occupancy with it.
+template <int TN, bool G, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw) {
@@ -88,48 +91,41 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
     constexpr int PT = TN / kBlock;
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
-    const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi, base = tr.base;
+    const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi;
     A sum = A(0);
     for (int c0 = lo; c0 < hi; c0 += TN) {
         const int c1 = min(hi, c0 + TN);
+        // Lanes past the chunk end re-read its last entry and rewrite the same
+        // pair to the same LDS slot: no per-entry branches, every load issued.
+        // Addresses as a uniform base plus an unsigned 32-bit offset, so that
+        // every load needs one offset register rather than a 64-bit address.
+        const int *__restrict__ ib = idx + c0;
+        const V *__restrict__ vb = val + c0;
+        const uint32_t last = static_cast<uint32_t>(c1 - 1 - c0);
         int id[PT];
         V v[PT];
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
-            const int k = c0 + tid + q * kBlock;
-            if (k < c1) {
-                id[q] = idx[k];
-                v[q] = val[k];
-            }
+            const uint32_t o = min(static_cast<uint32_t>(tid + q * kBlock), last);
+            id[q] = ib[o];
+            v[q] = vb[o];
         }
         X g[PT];
-        int d[PT];
-        if (base >= 0) {
+        if (G) {
+            const X *__restrict__ xb = xs + tr.base;
 #pragma unroll
-            for (int q = 0; q < PT; ++q) {
-                const int k = c0 + tid + q * kBlock;
-                if (k < c1) {
-                    const uint32_t w = static_cast<uint32_t>(id[q]);
-                    g[q] = xs[base + static_cast<int>(w >> kBits)];
-                    d[q] = static_cast<int>(w & kMask);
-                }
-            }
+            for (int q = 0; q < PT; ++q) g[q] = xb[static_cast<uint32_t>(id[q]) >> kBits];
         } else {
 #pragma unroll
-            for (int q = 0; q < PT; ++q) {
-                const int k = c0 + tid + q * kBlock;
-                if (k < c1) {
-                    g[q] = xs[id[q]];
-                    d[q] = k - c0;
-                }
-            }
+            for (int q = 0; q < PT; ++q) g[q] = xs[static_cast<uint32_t>(id[q])];
         }
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
 #pragma unroll
             for (int q = 0; q < PT; ++q) {
-                const int k = c0 + tid + q * kBlock;
-                if (k < c1 && (NP == 1 || d[q] / S == p)) lds[d[q] - p * S] = Pair<V, X>{v[q], g[q]};
+                const int d = G ? static_cast<int>(static_cast<uint32_t>(id[q]) & kMask)
+                                : static_cast<int>(min(static_cast<uint32_t>(tid + q * kBlock), last));
+                if (NP == 1 || d / S == p) lds[d - p * S] = Pair<V, X>{v[q], g[q]};
             }
             __syncthreads();
             const int a = max(k0, c0 + p * S), b = min(k1, min(c1, c0 + (p + 1) * S));
@@ -143,18 +139,18 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
     return sum;
 }
 
-template <int TN, typename V, typename X, typename A>
+template <int TN, bool G, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
                                         const int *__restrict__ idx, const V *__restrict__ val,
                                         const X *__restrict__ xs, void *lds_raw) {
-    return tile_dot_rows<TN, V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
+    return tile_dot_rows<TN, G, V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
 }
 
-template <int TN, typename A>
+template <int TN, bool G, typename A>
 __device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
                                       const int *__restrict__ idx, const float *__restrict__ val,
                                       const float *__restrict__ xs, void *lds) {
-    return tile_dot_t<TN, float, float, A>(tl, ptr, idx, val, xs, lds);
+    return tile_dot_t<TN, G, float, float, A>(tl, ptr, idx, val, xs, lds);
 }
 
 // Block-wide max of non-negative floats (NaN operands are dropped, as
@@ -206,15 +202,22 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
 // bodies, with the stream in `s`).
 #define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
 
-// Launches KERNEL<tn> for the tile set's chunk size.
-#define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                     \
+// Launches KERNEL<tn, gather> for the tile set's chunk size and layout.
+#define MPDHG_LAUNCH_TN1(ts, G, KERNEL, ...)                                                 \
     do {                                                                                     \
         if ((ts).tn == 1024)                                                                 \
-            hipLaunchKernelGGL(KERNEL<1024>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<1024, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
         else if ((ts).tn == 2048)                                                            \
-            hipLaunchKernelGGL(KERNEL<2048>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<2048, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
         else                                                                                 \
-            hipLaunchKernelGGL(KERNEL<4096>, MPDHG_TILE_GRID(ts), __VA_ARGS__);              \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<4096, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
+    } while (0)
+#define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                     \
+    do {                                                                                     \
+        if ((ts).gather)                                                                     \
+            MPDHG_LAUNCH_TN1(ts, true, KERNEL, __VA_ARGS__);                                  \
+        else                                                                                 \
+            MPDHG_LAUNCH_TN1(ts, false, KERNEL, __VA_ARGS__);                                 \
     } while (0)
 
 }  // namespace mpdhg

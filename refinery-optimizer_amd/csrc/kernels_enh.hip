@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_pow_scale(const Tile *__restrict__ t
     if (j < tl.end) v[j] = z[j] * inv;
 }
 
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ rptr,
                                                      const int *__restrict__ ridx,
@@ -44,12 +44,12 @@ __global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ ti
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<TN, float>(tl, rptr, ridx, rval, v, lds);
+    const float s = tile_dot<TN, G, float>(tl, rptr, ridx, rval, v, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) w[i] = s;
 }
 
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ cptr,
                                                      const int *__restrict__ cidx,
@@ -58,11 +58,12 @@ __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ ti
                                                      float *__restrict__ z,
                                                      double *__restrict__ pc) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ double red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<TN, float>(tl, cptr, cidx, cval, w, lds);
+    const float s = tile_dot<TN, G, float>(tl, cptr, cidx, cval, w, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q = 0.0;
     if (j < tl.end) {
@@ -102,12 +103,13 @@ struct PrimalArgs {
 
 // Iteration it = *A.itb + off: the iteration number lives on the device, so
 // one recorded check cycle (a HIP graph) replays for every cycle.
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
                                                        int chunk, const EnhInst *__restrict__ st,
                                                        PrimalArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ double red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
     if (s.status != kRunning) return;
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot_rows<TN, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    const float g = tile_dot_rows<TN, G, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -179,12 +181,13 @@ struct DualArgs {
     float rho;
 };
 
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const EnhInst *__restrict__ st,
                                                      DualArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ double red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
     if (s.status != kRunning) return;
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<TN, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<TN, G, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -253,17 +256,18 @@ struct CheckRowArgs {
     double *pr;
 };
 
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckRowArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ double red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const double axi = tile_dot_t<TN, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
+    const double axi = tile_dot_t<TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double viol = 0.0, sviol = 0.0, dobj = 0.0;
     if (i < tl.end) {
@@ -299,17 +303,18 @@ struct CheckColArgs {
     double *pc;
 };
 
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ double red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const double aty = tile_dot_t<TN, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
+    const double aty = tile_dot_t<TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
     if (j < tl.end) {

@@ -15,7 +15,7 @@ namespace mpdhg {
 namespace dev {
 
 // cuda_pdhg.cu:33-49 (A^T y) fused with :54-77 (projected primal step).
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_ref_primal(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<TN, float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) {
         const float xo = x[j];
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
 }
 
 // cuda_pdhg.cu:14-30 (A xbar) fused with :82-103 (Moreau dual step).
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_ref_dual(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float h = tile_dot<TN, float>(tl, rptr, ridx, rval, xbar, lds);
+    const float h = tile_dot<TN, G, float>(tl, rptr, ridx, rval, xbar, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) {
         const float sg = sigma[i];
@@ -68,19 +68,20 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
 
 // cuda_pdhg.cu:114-129 row residuals of A x, max-reduced per tile (the
 // reference copies m floats to the host and reduces there, :249-255).
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_ref_rowres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
     const float *__restrict__ x, const float *__restrict__ rl, const float *__restrict__ ru,
     float *__restrict__ part) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ float red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    float *const red = reinterpret_cast<float *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float ax = tile_dot<TN, float>(tl, rptr, ridx, rval, x, lds);
+    const float ax = tile_dot<TN, G, float>(tl, rptr, ridx, rval, x, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (i < tl.end) {
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_rowres(
 }
 
 // cuda_pdhg.cu:131-151 column fixed-point residuals from A^T y, max-reduced per tile.
-template <int TN>
+template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_ref_colres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
@@ -101,12 +102,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_colres(
     const float *__restrict__ l, const float *__restrict__ u, const float *__restrict__ tau,
     float *__restrict__ part) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    __shared__ float red[kBlock / 64];
+    // block reductions run after the row sums, in the then idle staging area
+    float *const red = reinterpret_cast<float *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<TN, float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (j < tl.end) {

@@ -14,6 +14,7 @@ struct TileSet {
     int chunk = 0;        // ceil(count / 8)
     int grid = 0;         // 8 * chunk
     int tn = kTileNnz;    // nonzeros per LDS chunk (1024, 2048 or 4096)
+    bool gather = false;  // every tile's chunks in gather order (Tile::base >= 0)
 };
 
 namespace launch {
