@@ -129,10 +129,17 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
             }
             __syncthreads();
             const int a = max(k0, c0 + p * S), b = min(k1, min(c1, c0 + (p + 1) * S));
-            for (int k = a; k < b; ++k) {
-                const Pair<V, X> pr = lds[k - c0 - p * S];
-                sum = fma_acc<A, V>(pr.v, pr.x, sum);
+            // four LDS reads in flight ahead of the (sequential) FMA chain
+            const Pair<V, X> *row = lds - (c0 + p * S);
+            int k = a;
+            for (; k + 4 <= b; k += 4) {
+                const Pair<V, X> p0 = row[k], p1 = row[k + 1], p2 = row[k + 2], p3 = row[k + 3];
+                sum = fma_acc<A, V>(p0.v, p0.x, sum);
+                sum = fma_acc<A, V>(p1.v, p1.x, sum);
+                sum = fma_acc<A, V>(p2.v, p2.x, sum);
+                sum = fma_acc<A, V>(p3.v, p3.x, sum);
             }
+            for (; k < b; ++k) sum = fma_acc<A, V>(row[k].v, row[k].x, sum);
             __syncthreads();
         }
     }
