@@ -106,7 +106,8 @@ def bench_sweep(args):
     probs = W.scenario_sweep(512)
     nnz = sum(p.num_nonzeros() for p in probs)
     log(f"generated 512 LPs ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
-    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=args.max_iterations)
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=args.max_iterations,
+                                   launch_graph=0 if args.no_graph else 1)
     for w in range(args.warmup):
         S.solve_batch(probs, cfg)
         log(f"warmup step {w} done")
@@ -155,11 +156,12 @@ def bench_sweep(args):
     }
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
-        stride = max(1, len(probs) // max(args.cpu_lps, 1))
-        cs = probs[::stride][: args.cpu_lps]
+        # every 16th LP of the sweep with m <= 4000 (bounded CPU time: the
+        # largest LPs alone would take minutes on the host cores)
+        cs = [p for p in probs[::16] if p.num_rows() <= 4000][: args.cpu_lps]
         ok, dt = cpu_port_sample(cs, tol, cores)
         out["cpu_baseline"] = {"value": ok / dt, "unit": "LPs/s", "cores": cores, "kind": "port",
-                               "sample": f"{len(cs)} LPs of the sweep (every {stride}th), CPU port of the "
+                               "sample": f"{len(cs)} LPs of the sweep (every 16th with m <= 4000), CPU port of the "
                                          f"same algorithm, {cores} LPs in parallel, {dt:.1f}s, "
                                          f"{ok} converged"}
     return out
@@ -244,9 +246,13 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
-    ap.add_argument("--cpu-lps", type=int, default=16)
+    ap.add_argument("--cpu-lps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-LP-per-call comparison")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernel by kernel instead of replaying recorded check cycles "
+                         "(the kernels are the same; rocprofv3's kernel tracing faults on graph "
+                         "replays in this ROCm release, so profiling runs use this)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the "
