@@ -15,6 +15,11 @@ Other workloads:
   large-ref  one random equality LP m=500,000 n=1,200,000 10 nnz/column,
              reference-mode PDHG; a step is one PDHG iteration.
   large-enh  the same LP, enhanced mode to 1e-4 relative KKT.
+  staircase96  one refinery staircase LP (96 blocks of 400 x 900) to 1e-8,
+             objective error against the committed FP64 HiGHS value.
+  adjoint64  64 staircase LPs (48 blocks) forward to 1e-8 and backward
+             through the torch.autograd layer, with central finite
+             differences on 4 of them.
   copy       a 256 MiB device-to-device elementwise copy (torch), the
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
 
@@ -211,6 +216,166 @@ def bench_large(args, mode):
     return out
 
 
+HIGHS_FIXTURES = ROOT / "tests" / "golden" / "highs_staircase.json"
+
+
+def highs_objective(T, R, Cb, seed):
+    """FP64 objective of a staircase LP from the committed HiGHS fixtures, or None."""
+    if not HIGHS_FIXTURES.exists():
+        return None
+    for c in json.loads(HIGHS_FIXTURES.read_text()):
+        if (c["T"], c["R"], c["C"], c["seed"]) == (T, R, Cb, seed):
+            return c["objective"]
+    return None
+
+
+def bench_staircase(args):
+    """Refinery staircase LP, 96 time blocks of 400 rows x 900 columns, to 1e-8."""
+    import sih_solver as S
+
+    T, R, Cb, seed, tol = 96, 400, 900, 1, 1e-8
+    p = S.gen_staircase(T, R, Cb, seed)
+    log(f"staircase T={T}: m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()}")
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=400000,
+                                   launch_graph=0 if args.no_graph else 1)
+    for _ in range(args.warmup):
+        S.solve_gpu_pdhg(p, cfg)
+    times = []
+    for s in range(args.steps):
+        cfg.profile = (s == args.steps - 1)
+        sol = S.solve_gpu_pdhg(p, cfg)
+        times.append(sol.time_solver_sec)
+        log(f"step {s}: status {int(sol.status)} kkt {sol.rel_kkt:.2e} in {sol.simplex_iterations} "
+            f"iterations, {times[-1]:.3f}s")
+    ref = highs_objective(T, R, Cb, seed)
+    rf, spmv = roofline(sol, traffic=False)
+    dev = sum(times) / len(times)
+    return {"metric": "seconds to 1e-8 rel KKT (staircase LP, FP32 iterates + FP64 refinement)",
+            "value": dev, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * dev,
+            "dtype": "fp32 iterates, fp64 KKT/refinement",
+            "config": {"workload": f"staircase{T}: T={T} blocks of {R}x{Cb}, m={p.num_rows()} "
+                                   f"n={p.num_cols()} nnz={p.num_nonzeros()}, seed {seed}, to {tol:g}"},
+            "status": int(sol.status), "iterations": sol.simplex_iterations,
+            "restarts": sol.restarts, "refinements": sol.refinements, "rel_kkt": sol.rel_kkt,
+            "objective": sol.primal_objective, "highs_objective": ref,
+            "rel_objective_error_vs_highs": (abs(sol.primal_objective - ref) / max(1.0, abs(ref))
+                                             if ref is not None else None),
+            "roofline": rf, "spmv": spmv}
+
+
+def bench_adjoint(args):
+    """64 staircase LPs (48 blocks each): batched forward to 1e-8 through the
+    torch.autograd layer, the adjoint backward, and central finite differences
+    of L = w . x*(b) along random unit directions on 4 of the LPs."""
+    import numpy as np
+    import torch
+
+    import sih_solver as S
+    from sih_solver.autograd import _shifted, lp_layer
+
+    K, T, R, Cb, tol = 64, 48, 400, 900, 1e-8
+    t = time.perf_counter()
+    probs = [S.gen_staircase(T, R, Cb, 100 + k) for k in range(K)]
+    nnz = sum(p.num_nonzeros() for p in probs)
+    log(f"generated {K} staircase LPs T={T} ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=1000000)
+    rng = np.random.default_rng(7)
+    c = torch.tensor(np.concatenate([p.c for p in probs]), dtype=torch.float64, requires_grad=True)
+    b = torch.zeros(sum(p.num_rows() for p in probs), dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(rng.normal(size=c.numel()))
+    t0 = time.perf_counter()
+    x = lp_layer(probs, c, b, cfg)
+    t1 = time.perf_counter()
+    (w * x).sum().backward()
+    t2 = time.perf_counter()
+    log(f"forward {t1 - t0:.2f}s, backward {t2 - t1:.2f}s")
+    # Central differences of L along shifts of b (forward solves to 1e-9).
+    # x*(b) is a differentiable function only where the solution is a unique
+    # vertex: the basic columns B must be determined by the active rows R
+    # (A_RB of full column rank, so A_RB^T mu = w_B is consistent) and every
+    # nonbasic column must have a reduced cost away from zero.  LPs failing
+    # either test (PDHG then returns a point inside an optimal face) are
+    # counted, not differenced.  On the others the direction d = A_RB v keeps
+    # the active set (x_B moves by h v), and h also keeps every basic column
+    # inside its bounds and every inactive row feasible.
+    import scipy.sparse as sp
+
+    co = np.concatenate([[0], np.cumsum([p.num_cols() for p in probs])])
+    ro = np.concatenate([[0], np.cumsum([p.num_rows() for p in probs])])
+    wn, gb, xs = w.numpy(), b.grad.numpy(), x.detach().numpy()
+    fsols = x.grad_fn.sols  # the forward solutions the layer kept
+    fd_probs, meta, screened = [], [], []
+    for k in range(min(16, K)):
+        if len(meta) == 4:
+            break
+        p, sk = probs[k], fsols[k]
+        xk, yk, lam = xs[co[k]:co[k + 1]], sk.row_duals, sk.reduced_costs
+        wk, gk = wn[co[k]:co[k + 1]], gb[ro[k]:ro[k + 1]]
+        A = sp.csc_matrix((p.A.csc_values(), p.A.csc_row_ind(), p.A.csc_col_ptr()),
+                          shape=(p.num_rows(), p.num_cols()))
+        ax = A @ xk
+        marg = 1e-7 * (1.0 + np.abs(xk))
+        isb = (xk - p.col_lower > marg) & (p.col_upper - xk > marg)
+        B = np.flatnonzero(isb)
+        fin = np.concatenate([p.row_lower[np.isfinite(p.row_lower)], p.row_upper[np.isfinite(p.row_upper)]])
+        tl = 1e-7 * (1.0 + np.abs(fin).max(initial=0.0))
+        isr = ((np.abs(yk) > 1e-7 * (1.0 + np.abs(yk).max())) |
+               (np.abs(ax - p.row_lower) <= tl) | (np.abs(ax - p.row_upper) <= tl))
+        R = np.flatnonzero(isr)
+        AB = A[R][:, B]
+        sys_res = np.linalg.norm(AB.T @ gk[R] - wk[B]) / np.linalg.norm(wk[B])
+        # reduced costs are accurate to about the KKT target times (1 + |c|)
+        lam_min = float(np.min(np.abs(lam[~isb]), initial=np.inf)) / (1.0 + np.abs(p.c).max())
+        dual_deg = int(np.sum(np.abs(lam[~isb]) <= 1e-7 * (1.0 + np.abs(p.c).max())))
+        log(f"LP {k}: min |reduced cost| of nonbasic columns / (1 + |c|) = {lam_min:.2e}")
+        if sys_res > 1e-6 or dual_deg > 0:
+            screened.append(dict(lp=k, system_residual=sys_res, zero_reduced_costs=dual_deg))
+            log(f"LP {k}: not a unique vertex (||A_RB^T mu - w_B||/||w_B|| = {sys_res:.1e}, "
+                f"{dual_deg} nonbasic columns with zero reduced cost): no finite difference")
+            continue
+        v = rng.normal(size=B.size)
+        d = np.zeros(p.num_rows())
+        d[R] = AB @ v
+        scale = np.linalg.norm(d)
+        d /= scale
+        v /= scale  # a unit shift h d moves x_B by h v
+        room = np.minimum(xk[B] - p.col_lower[B], p.col_upper[B] - xk[B])
+        hs = [1e-4 * max(1.0, np.abs(fin).max(initial=0.0)),
+              0.5 * float(np.min(room / np.maximum(np.abs(v), 1e-300)))]
+        dv = A[~isr][:, B] @ v  # activity change of the inactive rows per unit step
+        lo_gap = (ax - p.row_lower)[~isr]
+        hi_gap = (p.row_upper - ax)[~isr]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            lim = np.where(dv < 0, lo_gap / -dv, np.where(dv > 0, hi_gap / dv, np.inf))
+        hs.append(0.5 * float(np.min(lim, initial=np.inf)))
+        h = min(hs)
+        for sgn in (1.0, -1.0):
+            fd_probs.append(_shifted(p, p.c, sgn * h * d))
+        meta.append((k, d, h, B.size, R.size, sys_res))
+    sols = S.solve_batch(fd_probs, S.GpuPdhgConfig.enhanced(tol=1e-9, max_iterations=2000000))
+    fd_ok = all(s.status == S.SolutionStatus.Optimal for s in sols)
+    errs, sys_res_max, steps = [], 0.0, []
+    for i, (k, d, h, nb, nr, sres) in enumerate(meta):
+        wk = wn[co[k]:co[k + 1]]
+        fd = (wk @ sols[2 * i].x - wk @ sols[2 * i + 1].x) / (2 * h)
+        an = gb[ro[k]:ro[k + 1]] @ d
+        errs.append(abs(fd - an) / max(abs(fd), abs(an), 1e-12))
+        sys_res_max = max(sys_res_max, sres)
+        steps.append(h)
+        log(f"LP {k}: |B|={nb} |R|={nr} h={h:.3g} FD {fd:.6e} adjoint {an:.6e} "
+            f"rel err {errs[-1]:.2e}; ||A_RB^T mu - w_B||/||w_B|| = {sres:.2e}")
+    return {"metric": "batched forward + adjoint seconds (64 staircase LPs to 1e-8)",
+            "value": t2 - t0, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * (t2 - t0),
+            "dtype": "fp32 iterates, fp64 KKT/refinement and adjoint",
+            "config": {"workload": f"adjoint64: {K} staircase LPs T={T} ({R}x{Cb} blocks), {nnz} nnz, "
+                                   "L = w.x*, w ~ N(0,1), forward to 1e-8"},
+            "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "dl_dc_all_zero": bool(torch.all(c.grad == 0)),
+            "fd_max_rel_error": max(errs) if errs else None, "fd_checks": len(errs),
+            "fd_solves_converged": fd_ok, "fd_tol": 1e-9, "fd_steps": steps,
+            "adjoint_system_rel_residual_max": sys_res_max, "not_unique_vertex": screened}
+
+
 def bench_copy(args):
     """256 MiB float32 device copy: reads 256 MiB and writes 256 MiB per launch."""
     import torch
@@ -242,7 +407,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
-                    choices=["sweep512", "large-ref", "large-enh", "copy"])
+                    choices=["sweep512", "large-ref", "large-enh", "staircase96", "adjoint64",
+                             "copy"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
@@ -262,6 +428,10 @@ def main():
         out = bench_sweep(args)
     elif args.workload == "copy":
         out = bench_copy(args)
+    elif args.workload == "staircase96":
+        out = bench_staircase(args)
+    elif args.workload == "adjoint64":
+        out = bench_adjoint(args)
     else:
         out = bench_large(args, "ref" if args.workload == "large-ref" else "enh")
     out.setdefault("higher_is_better", True)

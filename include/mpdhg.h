@@ -158,8 +158,8 @@ int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count,
 
 /* Adjoint of the PDHG fixed point (extension; no reference counterpart).
  * For L(x*) with upstream gradient w = dL/dx* at a solution (x, y) of `lp`:
- * columns strictly inside their bounds form B, rows with nonzero duals form
- * R, and the transposed fixed-point condition of the projected PDHG step on
+ * columns strictly inside their bounds form B, rows with nonzero duals or
+ * tight at x form R, and the transposed fixed-point condition of the projected PDHG step on
  * (B, R) reduces to A_RB^T mu = w_B, solved per instance on the GPU by CGLS
  * (min-norm least squares in the scaled space).  Outputs:
  *   dl_dc = 0 (the solution is piecewise constant in c on the active set),
@@ -181,7 +181,9 @@ typedef struct mpdhg_adjoint_io {
 
 /* active_tol: a column is basic when both bound distances exceed
  * active_tol * (1 + |x_j|); a row is active when |y_i| > active_tol *
- * (1 + max |y|).  tol: CGLS stopping tolerance; max_iter: CGLS iterations. */
+ * (1 + max |y|) or (A x)_i is within active_tol * (1 + max finite |bound|) of
+ * a finite bound (a tight row with a zero dual at a degenerate vertex).
+ * tol: CGLS stopping tolerance; max_iter: CGLS iterations. */
 int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
                         double active_tol, double tol, int max_iter);
 

@@ -87,8 +87,24 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
             r0[g] = mb[g] * pdc[j] * o.w[j];  // scaled right-hand side D_c w_B
             rt0[g] = mb[g] * pdc[j] * r0[g];
         }
+        // Active rows: nonzero duals, and rows tight at x.  At a degenerate
+        // vertex a tight row can carry a zero dual; leaving it out makes
+        // A_RB lose column rank, and A_RB^T mu = w_B inconsistent.
+        std::vector<double> ax(static_cast<size_t>(h.m), 0.0);
+        for (int j = 0; j < h.n; ++j)
+            for (int k = h.cptr[static_cast<size_t>(j)]; k < h.cptr[static_cast<size_t>(j) + 1]; ++k)
+                ax[static_cast<size_t>(h.cidx[static_cast<size_t>(k)])] += lp.vals[k] * o.x[j];
+        double bnorm = 0.0;
         for (int i = 0; i < h.m; ++i) {
-            const bool active = std::fabs(o.y[i]) > active_tol * (1.0 + ymax);
+            if (std::isfinite(lp.row_lower[i])) bnorm = std::max(bnorm, std::fabs(lp.row_lower[i]));
+            if (std::isfinite(lp.row_upper[i])) bnorm = std::max(bnorm, std::fabs(lp.row_upper[i]));
+        }
+        const double tight = active_tol * (1.0 + bnorm);
+        for (int i = 0; i < h.m; ++i) {
+            const double a = ax[static_cast<size_t>(i)];
+            const bool active = std::fabs(o.y[i]) > active_tol * (1.0 + ymax) ||
+                                (std::isfinite(lp.row_lower[i]) && std::fabs(a - lp.row_lower[i]) <= tight) ||
+                                (std::isfinite(lp.row_upper[i]) && std::fabs(a - lp.row_upper[i]) <= tight);
             mr[static_cast<size_t>(ro + i)] = active ? 1.0 : 0.0;
             nr += active;
         }
@@ -143,12 +159,46 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
                           d_s.get(), d_mu.get(), d_p.get(), d_pt.get(), d_pr.get()};
     launch::adj_start(st, rts, d_inst.get(), na, RA, tol, max_iter);
     std::vector<AdjInst> hs(static_cast<size_t>(na));
-    for (int k = 0; k < max_iter; k += 32) {
-        for (int u = 0; u < 32; ++u) launch::adj_step(st, rts, cts, d_inst.get(), na, CA, RA, tol, max_iter);
-        MPDHG_HIP(hipGetLastError());
-        MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(AdjInst), hipMemcpyDeviceToHost, st));
-        MPDHG_HIP(hipStreamSynchronize(st));
-        if (std::all_of(hs.begin(), hs.end(), [](const AdjInst &s) { return s.done != 0; })) break;
+    // Blocks of 32 CGLS steps, recorded once as a HIP graph and replayed; the
+    // host reads each block's done flags while the next block runs (steps of
+    // finished instances are no-ops).
+    constexpr int kSteps = 32;
+    struct StepGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        ~StepGraph() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } graph;
+    MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int u = 0; u < kSteps; ++u) launch::adj_step(st, rts, cts, d_inst.get(), na, CA, RA, tol, max_iter);
+    } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(st, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+    }
+    MPDHG_HIP(hipStreamEndCapture(st, &graph.g));
+    MPDHG_HIP(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
+    Pinned snap(2 * hs.size() * sizeof(AdjInst));
+    AdjInst *const snaps[2] = {static_cast<AdjInst *>(snap.p), static_cast<AdjInst *>(snap.p) + na};
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (auto &v : ev.e) MPDHG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    for (int k = 0, blk = 0; k < max_iter; k += kSteps, ++blk) {
+        MPDHG_HIP(hipGraphLaunch(graph.e, st));
+        const int cur = blk & 1;
+        MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(AdjInst), hipMemcpyDeviceToHost, st));
+        MPDHG_HIP(hipEventRecord(ev.e[cur], st));
+        if (blk > 0) {
+            MPDHG_HIP(hipEventSynchronize(ev.e[cur ^ 1]));
+            const AdjInst *sv = snaps[cur ^ 1];
+            if (std::all_of(sv, sv + na, [](const AdjInst &s) { return s.done != 0; })) break;
+        }
     }
     launch::adj_atmu(st, rts, cts, CA, RA);  // pt = D_r mu, q = A^T pt
     MPDHG_HIP(hipGetLastError());

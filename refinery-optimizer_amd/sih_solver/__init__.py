@@ -444,9 +444,11 @@ class AdjointResult:
 
 
 def adjoint_batch(problems: Sequence[Problem], xs, ys, ws, active_tol: float = 1e-7,
-                  tol: float = 1e-10, max_iter: int = 20000) -> List[AdjointResult]:
+                  tol: float = 1e-10, max_iter: int = 200000) -> List[AdjointResult]:
     """Batched adjoint on the GPU: for each LP, the gradients of w . x* at the
-    solution (x, y) (row duals in the reference sign, as solve_* return them)."""
+    solution (x, y) (row duals in the reference sign, as solve_* return them).
+    CGLS needs a number of steps that grows with the coupling length of the
+    active set (about 900 per time block of a staircase LP); max_iter bounds it."""
     nb = len(problems)
     lps = (MpdhgLp * max(nb, 1))()
     io = (_abi.MpdhgAdjointIO * max(nb, 1))()
