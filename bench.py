@@ -321,9 +321,9 @@ def bench_adjoint(args):
         tl = 1e-7 * (1.0 + np.abs(fin).max(initial=0.0))
         isr = ((np.abs(yk) > 1e-7 * (1.0 + np.abs(yk).max())) |
                (np.abs(ax - p.row_lower) <= tl) | (np.abs(ax - p.row_upper) <= tl))
-        R = np.flatnonzero(isr)
-        AB = A[R][:, B]
-        sys_res = np.linalg.norm(AB.T @ gk[R] - wk[B]) / np.linalg.norm(wk[B])
+        act = np.flatnonzero(isr)
+        AB = A[act][:, B]
+        sys_res = np.linalg.norm(AB.T @ gk[act] - wk[B]) / np.linalg.norm(wk[B])
         # reduced costs are accurate to about the KKT target times (1 + |c|)
         lam_min = float(np.min(np.abs(lam[~isb]), initial=np.inf)) / (1.0 + np.abs(p.c).max())
         dual_deg = int(np.sum(np.abs(lam[~isb]) <= 1e-7 * (1.0 + np.abs(p.c).max())))
@@ -335,7 +335,7 @@ def bench_adjoint(args):
             continue
         v = rng.normal(size=B.size)
         d = np.zeros(p.num_rows())
-        d[R] = AB @ v
+        d[act] = AB @ v
         scale = np.linalg.norm(d)
         d /= scale
         v /= scale  # a unit shift h d moves x_B by h v
@@ -351,7 +351,7 @@ def bench_adjoint(args):
         h = min(hs)
         for sgn in (1.0, -1.0):
             fd_probs.append(_shifted(p, p.c, sgn * h * d))
-        meta.append((k, d, h, B.size, R.size, sys_res))
+        meta.append((k, d, h, B.size, act.size, sys_res))
     sols = S.solve_batch(fd_probs, S.GpuPdhgConfig.enhanced(tol=1e-9, max_iterations=2000000))
     fd_ok = all(s.status == S.SolutionStatus.Optimal for s in sols)
     errs, sys_res_max, steps = [], 0.0, []

@@ -140,3 +140,39 @@ def test_autograd_layer_and_batching(need_gpu):
         assert np.allclose(one.dl_db, b.grad.numpy()[roff:roff + m], rtol=1e-9, atol=1e-12)
         off += n
         roff += m
+
+
+@pytest.mark.parametrize("seed", [100, 104])
+def test_adjoint_on_staircase_vertex_matches_finite_differences(need_gpu, seed):
+    # The refinery staircase structure (4 blocks of 400 x 900), degenerate:
+    # more tight rows than basic columns and zero reduced costs.  The GPU
+    # adjoint is evaluated at HiGHS's optimal vertex; central differences of
+    # L = w . x*(b) (FP64 HiGHS solves) along a shift that keeps the active
+    # set, d = A_RB v, must agree to 1e-6 relative.
+    p = S.gen_staircase(4, 400, 900, seed)
+    lp = O.LpArrays.of(p)
+    s, y = _highs(lp)
+    x = s.x
+    rng = np.random.default_rng(seed)
+    w = rng.normal(size=lp.n)
+    r = S.adjoint_batch([p], [x], [y], [w])[0]
+    A = sp.csc_matrix((lp.v, lp.ri, lp.cp), shape=(lp.m, lp.n))
+    ax = A @ x
+    isb = (x - lp.l > 1e-7 * (1 + abs(x))) & (lp.u - x > 1e-7 * (1 + abs(x)))
+    fin = np.concatenate([lp.rl[np.isfinite(lp.rl)], lp.ru[np.isfinite(lp.ru)]])
+    tl = 1e-7 * (1 + np.abs(fin).max())
+    isr = (np.abs(y) > 1e-7 * (1 + np.abs(y).max())) | (np.abs(ax - lp.rl) <= tl) | (np.abs(ax - lp.ru) <= tl)
+    assert (r.basic_cols, r.active_rows) == (int(isb.sum()), int(isr.sum()))
+    B, R = np.flatnonzero(isb), np.flatnonzero(isr)
+    assert r.active_rows > r.basic_cols  # degenerate vertex
+    v = rng.normal(size=B.size)
+    d = np.zeros(lp.m)
+    d[R] = A[R][:, B] @ v
+    d /= np.linalg.norm(d)
+    h = 1e-4 * max(1.0, np.abs(fin).max())
+    Ls = []
+    for sg in (1.0, -1.0):
+        q = O.LpArrays(lp.m, lp.n, lp.cp, lp.ri, lp.v, lp.c, lp.l, lp.u, lp.rl + sg * h * d, lp.ru + sg * h * d)
+        Ls.append(w @ _highs(q)[0].x)
+    fd = (Ls[0] - Ls[1]) / (2 * h)
+    assert abs(fd - r.dl_db @ d) <= 1e-6 * max(abs(fd), 1e-9), (fd, r.dl_db @ d)
