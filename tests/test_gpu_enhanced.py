@@ -67,6 +67,22 @@ def test_converges_and_matches_highs(need_gpu, tol):
         assert abs(s.primal_objective - h.fun) <= 10 * tol * (1 + abs(h.fun)), (p.name, s.primal_objective, h.fun)
 
 
+def test_staircase_12_blocks_to_1e8_matches_highs_fixture(need_gpu):
+    # The refinery staircase (12 blocks of 400 x 900, m = 4800) to 1e-8; its
+    # FP64 HiGHS objective is a committed fixture (tests/golden/make_highs_fixtures.py).
+    import json
+    from pathlib import Path
+
+    fx = json.loads((Path(__file__).parent / "golden" / "highs_staircase.json").read_text())
+    ref = next(c for c in fx if (c["T"], c["R"], c["C"], c["seed"]) == (12, 400, 900, 1))
+    p = S.gen_staircase(12, 400, 900, 1)
+    assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (ref["m"], ref["n"], ref["nnz"])
+    s = S.solve_gpu_pdhg(p, enh(tol=1e-8, max_iterations=400000))
+    assert s.status == S.SolutionStatus.Optimal and s.rel_kkt <= 1e-8
+    assert O.kkt(O.LpArrays.of(p), s.x, s.row_duals)["kkt"] <= 1e-8 * 1.0001
+    assert abs(s.primal_objective - ref["objective"]) <= 1e-7 * abs(ref["objective"])
+
+
 def test_full_solve_tracks_cpu_port(need_gpu):
     p = S.gen_staircase(2, 400, 900, 1)
     sol = S.solve_gpu_pdhg(p, enh(tol=1e-6))
