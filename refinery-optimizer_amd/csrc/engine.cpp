@@ -76,23 +76,54 @@ void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
 }
 
 // Rows [start, i) form one tile while the tile has at most kBlock rows and
-// at most tn nonzeros; a single longer row gets a tile of its own (the
-// kernel streams it in several chunks).
-void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst, int tn,
-                std::vector<Tile> &out) {
+// at most tn nonzeros, growing by whole kGroup-row groups; a single denser
+// group gets a tile of its own (the kernel streams it in several chunks).
+void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int64_t nz_off,
+                int64_t grp_off, int inst, int tn, std::vector<Tile> &out) {
     int i = 0;
     while (i < count) {
         const int start = i;
         int nz = 0;
         while (i < count && i - start < kBlock) {
-            const int rn = ptr[static_cast<size_t>(i) + 1] - ptr[static_cast<size_t>(i)];
-            if (i > start && nz + rn > tn) break;
-            nz += rn;
-            ++i;
+            const int ge = std::min(count, i + kGroup);
+            const int gn = ptr[static_cast<size_t>(ge)] - ptr[static_cast<size_t>(i)];
+            if (i > start && nz + gn > tn) break;
+            nz += gn;
+            i = ge;
         }
         out.push_back(Tile{static_cast<int32_t>(global_off + start),
-                           static_cast<int32_t>(global_off + i), inst, -1});
+                           static_cast<int32_t>(global_off + i), inst, -1,
+                           static_cast<int32_t>(nz_off + ptr[static_cast<size_t>(start)]),
+                           static_cast<int32_t>(nz_off + ptr[static_cast<size_t>(i)]), 1,
+                           static_cast<int32_t>(grp_off + start / kGroup)});
     }
+}
+
+BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &act) {
+    const size_t na = act.size();
+    BatchLayout L;
+    L.row_off.resize(na);
+    L.col_off.resize(na);
+    L.nnz_off.resize(na);
+    L.rgrp_off.resize(na);
+    L.cgrp_off.resize(na);
+    for (size_t a = 0; a < na; ++a) {
+        const HostLp &h = H[static_cast<size_t>(act[a])];
+        L.row_off[a] = L.rows;
+        L.col_off[a] = L.cols;
+        L.nnz_off[a] = L.nnz;
+        L.rgrp_off[a] = L.rgroups;
+        L.cgrp_off[a] = L.cgroups;
+        L.rows += h.m;
+        L.cols += h.n;
+        L.nnz += h.nnz;
+        L.rgroups += (h.m + kGroup - 1) / kGroup;
+        L.cgroups += (h.n + kGroup - 1) / kGroup;
+    }
+    const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
+    if (L.rows > lim || L.cols > lim || L.nnz > lim)
+        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    return L;
 }
 
 TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather) {
@@ -218,23 +249,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     }
     // ---- host preparation (gpu_solver.cpp:82-151, 192-196) ----
     RefHost hh;
-    BatchLayout L;
-    L.row_off.resize(static_cast<size_t>(na));
-    L.col_off.resize(static_cast<size_t>(na));
-    L.nnz_off.resize(static_cast<size_t>(na));
-    for (int a = 0; a < na; ++a) {
-        const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-        L.row_off[static_cast<size_t>(a)] = L.rows;
-        L.col_off[static_cast<size_t>(a)] = L.cols;
-        L.nnz_off[static_cast<size_t>(a)] = L.nnz;
-        L.rows += h.m;
-        L.cols += h.n;
-        L.nnz += h.nnz;
-    }
-    if (L.rows > std::numeric_limits<int32_t>::max() - kTileNnz ||
-        L.cols > std::numeric_limits<int32_t>::max() - kTileNnz ||
-        L.nnz > std::numeric_limits<int32_t>::max() - kTileNnz)
-        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
     const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
@@ -300,10 +315,10 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         s.max_abs_c = mac;
         s.max_abs_b = mab;
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, ro, a, rtn, hh.rtiles);
+        plan_tiles(h.rptr, h.m, ro, zo, L.rgrp_off[static_cast<size_t>(a)], a, rtn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, co, a, ctn, hh.ctiles);
+        plan_tiles(h.cptr, h.n, co, zo, L.cgrp_off[static_cast<size_t>(a)], a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     bool rgo = false, cgo = false;
@@ -384,6 +399,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             launch::ref_finalize(st, d_inst.get(), na, d_prow.get(), d_pcol.get(), iter, K,
                                  cfg.tol_primal, cfg.tol_dual);
             launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
+            launch::tile_run(st, rts, d_inst.get());
+            launch::tile_run(st, cts, d_inst.get());
             MPDHG_HIP(hipGetLastError());
             const int cur = nchk & 1;
             MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(RefInst),

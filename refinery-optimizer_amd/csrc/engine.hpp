@@ -90,16 +90,22 @@ struct HostLp {
 };
 void prepare_structure(const mpdhg_lp &lp, HostLp &h);
 
-// Concatenation offsets of a batch.
+// Concatenation offsets of a batch: rows, columns, nonzeros, and the
+// kGroup-row groups of the rows and of the columns.
 struct BatchLayout {
-    std::vector<int64_t> row_off, col_off, nnz_off;
-    int64_t rows = 0, cols = 0, nnz = 0;
+    std::vector<int64_t> row_off, col_off, nnz_off, rgrp_off, cgrp_off;
+    int64_t rows = 0, cols = 0, nnz = 0, rgroups = 0, cgroups = 0;
 };
+// Layout of the instances H[act[a]]; throws if the batch overflows int32 indexing.
+BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &act);
 
-// Greedy tile plan over one instance's compressed structure: at most
-// kBlock rows and at most tn nonzeros per tile (a longer row alone).
-void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int inst, int tn,
-                std::vector<Tile> &out);
+// Greedy tile plan over one instance's compressed structure: whole
+// kGroup-row groups, at most kBlock rows and at most tn nonzeros per tile (a
+// denser group alone, streamed in several chunks).  global_off / nz_off /
+// grp_off: the instance's first row, first nonzero and first group in the
+// batch.
+void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int64_t nz_off,
+                int64_t grp_off, int inst, int tn, std::vector<Tile> &out);
 TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather);
 
 // Chunk size of a tile set over `rows` rows holding `nnz` nonzeros: the

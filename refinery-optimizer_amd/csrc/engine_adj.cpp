@@ -32,22 +32,7 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     }
     const int na = static_cast<int>(act.size());
     if (na == 0) return;
-    BatchLayout L;
-    L.row_off.resize(static_cast<size_t>(na));
-    L.col_off.resize(static_cast<size_t>(na));
-    L.nnz_off.resize(static_cast<size_t>(na));
-    for (int a = 0; a < na; ++a) {
-        const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-        L.row_off[static_cast<size_t>(a)] = L.rows;
-        L.col_off[static_cast<size_t>(a)] = L.cols;
-        L.nnz_off[static_cast<size_t>(a)] = L.nnz;
-        L.rows += h.m;
-        L.cols += h.n;
-        L.nnz += h.nnz;
-    }
-    const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
-    if (L.rows > lim || L.cols > lim || L.nnz > lim)
-        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
     const int rtn = choose_chunk(L.nnz, L.rows, 0);
@@ -119,12 +104,13 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     for (int a = 0; a < na; ++a) {
         const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
         AdjInst &s = inst[static_cast<size_t>(a)];
-        s.row_tile_begin = static_cast<int32_t>(rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, rtn, rtiles);
-        s.row_tile_end = static_cast<int32_t>(rtiles.size());
-        s.col_tile_begin = static_cast<int32_t>(ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, ctiles);
-        s.col_tile_end = static_cast<int32_t>(ctiles.size());
+        const size_t ua = static_cast<size_t>(a);
+        s.rgrp_begin = static_cast<int32_t>(L.rgrp_off[ua]);
+        s.rgrp_end = static_cast<int32_t>(L.rgrp_off[ua] + (h.m + kGroup - 1) / kGroup);
+        s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
+        s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
+        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rtn, rtiles);
+        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, ctn, ctiles);
     }
     bool rgo = false, cgo = false;
     {
@@ -150,13 +136,14 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     for (double *p : {d_s.get(), d_mu.get(), d_p.get(), d_pt.get()})
         MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(double), st));
     MPDHG_HIP(hipMemsetAsync(d_q.get(), 0, C * sizeof(double), st));
-    DBuf<double> d_pc(ctiles.size() * kPcCount), d_pr(rtiles.size() * kPrCount);
+    // per-group partials of |q|^2 (columns) and |s|^2 (rows)
+    DBuf<double> d_gq(static_cast<size_t>(L.cgroups)), d_gs(static_cast<size_t>(L.rgroups));
     const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rtn, rgo);
     const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), ctn, cgo);
     launch::AdjColArgs CA{d_cptr.get(), d_cidx.get(), d_cval.get(), d_pt.get(), d_mb.get(), d_dc.get(),
-                          d_q.get(), d_r.get(), d_rt.get(), d_pc.get()};
+                          d_q.get(), d_r.get(), d_rt.get(), d_gq.get()};
     launch::AdjRowArgs RA{d_rptr.get(), d_ridx.get(), d_rval.get(), d_rt.get(), d_mr.get(), d_dr.get(),
-                          d_s.get(), d_mu.get(), d_p.get(), d_pt.get(), d_pr.get()};
+                          d_s.get(), d_mu.get(), d_p.get(), d_pt.get(), d_gs.get()};
     launch::adj_start(st, rts, d_inst.get(), na, RA, tol, max_iter);
     std::vector<AdjInst> hs(static_cast<size_t>(na));
     // Blocks of 32 CGLS steps, recorded once as a HIP graph and replayed; the

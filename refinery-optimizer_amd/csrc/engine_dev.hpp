@@ -5,21 +5,28 @@
 
 namespace mpdhg {
 
-// Workgroup shape of every tile kernel: 4 wave64s, one row per lane in the
-// epilogue.
-constexpr int kBlock = 256;
+#ifndef MPDHG_BLOCK
+#define MPDHG_BLOCK 256
+#endif
 
-// Chunks: a tile set has a chunk size tn in {1024, 2048, 4096} (kTileNnz is
-// the largest); a tile holds at most kBlock rows and at most tn nonzeros
+// Workgroup shape of every tile kernel: kBlock / 64 wave64s, one row per
+// lane in the epilogue.
+constexpr int kBlock = MPDHG_BLOCK;
+static_assert(kBlock == 256 || kBlock == 512, "tile workgroups of 4 or 8 waves");
+
+// Chunks: a tile set has a chunk size tn, a power of two in [kChunkMin,
+// kTileNnz]; a tile holds at most kBlock rows and at most tn nonzeros
 // unless it is a single longer row, and its nonzeros [lo, hi) are staged
 // through LDS in chunks [lo + j tn, lo + (j+1) tn).  The LDS a workgroup
 // reserves is sized by tn, so sparse structures (short rows) use small
 // chunks and more workgroups fit on a CU.
-constexpr int kTileNnz = 4096;
+constexpr int kTileNnz = 16 * kBlock;
 constexpr int kChunkMin = 1024;
 
 constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
-constexpr bool valid_chunk(int tn) { return tn == 1024 || tn == 2048 || tn == 4096; }
+constexpr bool valid_chunk(int tn) {
+    return tn >= kChunkMin && tn <= kTileNnz && (tn & (tn - 1)) == 0;
+}
 
 // Gather order: within each chunk the nonzeros may be stored sorted by the
 // index of the vector entry they gather (ties in storage order), so that
@@ -31,16 +38,38 @@ constexpr bool valid_chunk(int tn) { return tn == 1024 || tn == 2048 || tn == 40
 // order either way.
 constexpr int64_t gather_local_limit(int tn) { return int64_t(1) << (32 - ilog2(tn)); }
 
+// Row groups: each instance's rows (and columns) are cut into groups of
+// kGroup = 16 consecutive rows from the instance's first row, the rows of
+// 16 adjacent lanes.  Tiles start on group boundaries, so every FP64 sum
+// that feeds a decision is formed per group by its lanes in a fixed order
+// and then per instance over its groups in a fixed order (canon_sum): no
+// result depends on the tile schedule.  Small groups keep tiles packed
+// close to their nonzero budget.
+constexpr int kGroup = 16;
+static_assert(kGroup <= 64 && (kGroup & (kGroup - 1)) == 0, "groups within a wave");
+
 // A tile: rows [begin, end) of one instance's compressed structure
-// (CSR rows for A x, CSC columns for A^T y).  end - begin <= kBlock.
+// (CSR rows for A x, CSC columns for A^T y).  end - begin <= kBlock, and
+// begin is the first row of a group (grp: that group's index in the batch).
 // base >= 0: the tile's chunks are in gather order, gathered vector entries
 // at base + local; base == -1: storage order, index words are global.
+// [lo, hi) is the tile's nonzero range (ptr[begin], ptr[end]), held in the
+// record so that a kernel can start streaming the nonzeros as soon as the
+// record arrives.  run is a hint refreshed after every status change
+// (1 while the instance iterates): a tile with run == 0 belongs to a
+// stopped instance and its workgroup exits at once.  The instance status
+// stays authoritative: kernels store nothing for a stopped instance.
 struct Tile {
     int32_t begin;
     int32_t end;
     int32_t inst;
     int32_t base;
+    int32_t lo;
+    int32_t hi;
+    int32_t run;
+    int32_t grp;
 };
+static_assert(sizeof(Tile) == 32, "one 32-byte scalar load per tile record");
 
 constexpr int32_t kRunning = -1;  // instance status while iterating
 
@@ -58,27 +87,32 @@ struct RefInst {
     int32_t pad;
 };
 
-// Per-tile partial reductions written by the enhanced-mode kernels.
-// Column (CSC) tiles:
+// Per-group partials of the enhanced mode, quantity-major: quantity q of
+// group g at part[q * groups + g].
+// Column (CSC) groups:
 enum : int {
-    kPcDx2 = 0,     // sum (x - x+)^2            (primal step, check iterations)
-    kPcDxa2 = 1,    // sum (x+ - x0)^2           (distance to the anchor)
-    kPcDualViol = 2,  // max |lambda - proj(lambda)|
-    kPcColViol = 3,   // max bound violation of x
-    kPcPobj = 4,      // sum c_j x_j
-    kPcDobj = 5,      // sum l_j lambda_j^+ - u_j lambda_j^-
-    kPcPow = 6,       // power iteration: sum z_j^2
-    kPcCount = 8
+    kGcDx2 = 0,       // sum (x - x+)^2            (fused primal step, check iterations)
+    kGcDxa2 = 1,      // sum (x+ - x0)^2           (distance to the anchor)
+    kGcPobj = 2,      // sum c_j x_j               (FP64 check)
+    kGcDobj = 3,      // sum l_j lambda_j^+ - u_j lambda_j^-
+    kGcDualViol = 4,  // max |lambda - proj(lambda)|
+    kGcColViol = 5,   // max bound violation of x
+    kGcPow = 6,       // power iteration: sum z_j^2
+    kGcCount = 7
 };
-// Row (CSR) tiles:
+// Row (CSR) groups:
 enum : int {
-    kPrDy2 = 0,      // sum (y - y+)^2
-    kPrDya2 = 1,     // sum (y+ - y0)^2
-    kPrRowViol = 2,  // max row-activity violation
-    kPrSignViol = 3, // max row-dual sign violation
-    kPrDobj = 4,     // - sum (ru_i y_i^+ - rl_i y_i^-)
-    kPrCount = 8
+    kGrDy2 = 0,       // sum (y - y+)^2
+    kGrDya2 = 1,      // sum (y+ - y0)^2
+    kGrDobj = 2,      // - sum (ru_i y_i^+ - rl_i y_i^-)
+    kGrRowViol = 3,   // max row-activity violation
+    kGrSignViol = 4,  // max row-dual sign violation
+    kGrCount = 5
 };
+
+// Threads of the per-instance kernels (finalize, power-iteration and CGLS
+// scalars): fixed, so that their reduction trees never change.
+constexpr int kInstBlock = 256;
 
 // Enhanced-mode per-instance state (device resident; written by the
 // per-instance finalize kernel, read by every tile of the instance).
@@ -95,6 +129,8 @@ struct EnhInst {
     int32_t max_iter;
     int32_t row_tile_begin, row_tile_end;
     int32_t col_tile_begin, col_tile_end;
+    int32_t rgrp_begin, rgrp_end;  // the instance's row / column groups
+    int32_t cgrp_begin, cgrp_end;
     float tau, sigma, inv_sigma, pad0;
     double eta, omega;
     double r_anchor, r_prev;   // fixed-point residuals (restart criteria)
@@ -112,8 +148,8 @@ struct EnhInst {
 struct AdjInst {
     int32_t done;
     int32_t iters;
-    int32_t row_tile_begin, row_tile_end;
-    int32_t col_tile_begin, col_tile_end;
+    int32_t rgrp_begin, rgrp_end;  // the instance's row / column groups
+    int32_t cgrp_begin, cgrp_end;
     double gamma, gamma0, alpha, beta;
 };
 

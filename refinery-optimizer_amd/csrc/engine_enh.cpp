@@ -119,22 +119,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;
         return;
     }
-    BatchLayout L;
-    L.row_off.resize(static_cast<size_t>(na));
-    L.col_off.resize(static_cast<size_t>(na));
-    L.nnz_off.resize(static_cast<size_t>(na));
-    for (int a = 0; a < na; ++a) {
-        const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-        L.row_off[static_cast<size_t>(a)] = L.rows;
-        L.col_off[static_cast<size_t>(a)] = L.cols;
-        L.nnz_off[static_cast<size_t>(a)] = L.nnz;
-        L.rows += h.m;
-        L.cols += h.n;
-        L.nnz += h.nnz;
-    }
-    const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
-    if (L.rows > lim || L.cols > lim || L.nnz > lim)
-        throw Error(MPDHG_E_ARG, "batch too large for int32 indexing; split it");
+    const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
     const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
@@ -224,11 +209,16 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         const int q = act[static_cast<size_t>(a)];
         const HostLp &h = H[static_cast<size_t>(q)];
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
+        const size_t ua = static_cast<size_t>(a);
+        s.rgrp_begin = static_cast<int32_t>(L.rgrp_off[ua]);
+        s.rgrp_end = static_cast<int32_t>(L.rgrp_off[ua] + (h.m + kGroup - 1) / kGroup);
+        s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
+        s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[static_cast<size_t>(a)], a, rtn, hh.rtiles);
+        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rtn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[static_cast<size_t>(a)], a, ctn, hh.ctiles);
+        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, ctn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     bool rgo = false, cgo = false;
@@ -265,7 +255,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     d_rt.upload(hh.rtiles, st);
     d_ct.upload(hh.ctiles, st);
     d_inst.upload(hh.inst, st);
-    DBuf<double> d_pc(hh.ctiles.size() * kPcCount), d_pr(hh.rtiles.size() * kPrCount);
+    // per-group partials of the decision sums and maxima (quantity-major)
+    const int gcn = static_cast<int>(L.cgroups), grn = static_cast<int>(L.rgroups);
+    DBuf<double> d_gc(static_cast<size_t>(kGcCount) * L.cgroups), d_gr(static_cast<size_t>(kGrCount) * L.rgroups);
     DBuf<int32_t> d_status(static_cast<size_t>(na));
     MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_x0.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
@@ -292,8 +284,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         launch::pow_scale(st, cts, d_inst.get(), d_xbar.get(), d_xp.get());
         launch::pow_rows(st, rts, d_rptr.get(), d_ridx.get(), d_rval.get(), d_xp.get(), d_yp.get());
         launch::pow_cols(st, cts, d_cptr.get(), d_cidx.get(), d_cval.get(), d_yp.get(), d_xbar.get(),
-                         d_pc.get());
-        launch::pow_fin(st, d_inst.get(), na, d_pc.get());
+                         d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
+        launch::pow_fin(st, d_inst.get(), na, d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
     }
     MPDHG_HIP(hipGetLastError());
     std::vector<EnhInst> hs(static_cast<size_t>(na));
@@ -321,14 +313,14 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const float rho = cfg.reflection;
     launch::EnhPrimal P{d_cptr.get(), d_cidx.get(), d_cval.get(), d_y.get(), d_yp.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_xbar.get(),
-                        d_xbase.get(), d_dc.get(), d_xchk.get(), d_pc.get(), d_itb.get(), rho};
+                        d_xbase.get(), d_dc.get(), d_xchk.get(), d_gc.get(), gcn, d_itb.get(), rho};
     launch::EnhDual D{d_rptr.get(), d_ridx.get(), d_rval.get(), d_xbar.get(), d_rls.get(), d_rus.get(),
                       d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_ybase.get(), d_dr.get(),
-                      d_ychk.get(), d_pr.get(), d_itb.get(), rho};
+                      d_ychk.get(), d_gr.get(), grn, d_itb.get(), rho};
     launch::EnhCheck CK{d_rptr.get(), d_ridx.get(), d_cptr.get(), d_cidx.get(), d_rval64.get(),
                         d_cval64.get(), d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
                         d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
-                        d_dr.get(), d_pr.get(), d_pc.get(), d_status.get(), cfg.refine_ratio,
+                        d_dr.get(), d_gc.get(), gcn, d_gr.get(), grn, d_status.get(), cfg.refine_ratio,
                         cfg.max_refinements, d_xbest.get(), d_lambest.get(),
                         d_ybest.get(), d_axbest.get(), d_xbase.get(), d_ybase.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_rls.get(),
@@ -353,8 +345,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     int it = 0, cycle = 0, covered = na;
     bool finished = false;
     std::vector<DBuf<Tile>> keep_rt, keep_ct;  // compacted tile lists, alive until the end
-    std::vector<DBuf<int32_t>> keep_rng;
-    std::vector<std::vector<int32_t>> host_keep;
+    std::vector<std::vector<Tile>> host_keep_rt, host_keep_ct;  // sources of their async uploads
     // One check cycle of len iterations in stream order: F fused launch pairs,
     // the FP64 check and decisions, the iteration advance.
     auto issue_cycle = [&](int len) {
@@ -436,29 +427,22 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     // tiles.  Stream order puts the range update after the cycle
                     // already in flight, which still uses the old lists.
                     std::vector<Tile> nrt, nct;
-                    std::vector<int32_t> rng(4 * static_cast<size_t>(na), 0);
                     for (int a = 0; a < na; ++a) {
                         if (sv[a] != kRunning) continue;
                         const EnhInst &s0 = hh.inst[static_cast<size_t>(a)];
-                        rng[4 * static_cast<size_t>(a)] = static_cast<int32_t>(nrt.size());
                         nrt.insert(nrt.end(), hh.rtiles.begin() + s0.row_tile_begin, hh.rtiles.begin() + s0.row_tile_end);
-                        rng[4 * static_cast<size_t>(a) + 1] = static_cast<int32_t>(nrt.size());
-                        rng[4 * static_cast<size_t>(a) + 2] = static_cast<int32_t>(nct.size());
                         nct.insert(nct.end(), hh.ctiles.begin() + s0.col_tile_begin, hh.ctiles.begin() + s0.col_tile_end);
-                        rng[4 * static_cast<size_t>(a) + 3] = static_cast<int32_t>(nct.size());
                     }
                     keep_rt.emplace_back();
                     keep_rt.back().upload(nrt, st);
                     keep_ct.emplace_back();
                     keep_ct.back().upload(nct, st);
-                    keep_rng.emplace_back();
-                    keep_rng.back().upload(rng, st);
-                    launch::enh_set_ranges(st, d_inst.get(), na, keep_rng.back().get());
                     rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn, rgo);
                     cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn, cgo);
                     covered = running;
                     ++lists;
-                    host_keep.push_back(std::move(rng));
+                    host_keep_rt.push_back(std::move(nrt));
+                    host_keep_ct.push_back(std::move(nct));
                 }
             }
             ++cycle;

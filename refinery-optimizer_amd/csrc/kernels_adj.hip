@@ -9,8 +9,9 @@
 // Ruiz/Pock-Chambolle-scaled space (min-norm least squares, so degenerate
 // active sets with |R| != |B| are handled), in FP64, with the same tile
 // machinery as the forward kernels: one masked SpMV over the CSC tiles and
-// one over the CSR tiles per CGLS step, per-tile partial sums and a
-// per-instance finalize.
+// one over the CSR tiles per CGLS step, each leaving per-group partials of
+// |q|^2 or |s|^2, and per-instance scalar kernels that sum the partials in a
+// fixed order.
 #include <hip/hip_runtime.h>
 
 #include "engine_dev.hpp"
@@ -29,8 +30,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(const Tile *__restrict__ ti
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
@@ -43,22 +42,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(const Tile *__restrict__ ti
         A.q[j] = q;
         q2 = q * q;
     }
-    q2 = block_sum_d(q2, red);
-    if (threadIdx.x == 0) A.pc[static_cast<size_t>(t) * kPcCount + kPcPow] = q2;
+    put_group(A.gq, tl, group_sum_d(q2));
 }
 
 // alpha = gamma / |q|^2 per instance.
-__global__ __launch_bounds__(kBlock) void k_adj_alpha(AdjInst *__restrict__ st, int ninst,
-                                                      const double *__restrict__ pc) {
-    __shared__ double red[kBlock / 64];
+__global__ __launch_bounds__(kInstBlock) void k_adj_alpha(AdjInst *__restrict__ st, int ninst,
+                                                          const double *__restrict__ gq) {
+    __shared__ double red[kInstBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     AdjInst &s = st[q];
     if (s.done) return;
-    double acc = 0.0;
-    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock)
-        acc += pc[static_cast<size_t>(t) * kPcCount + kPcPow];
-    acc = block_sum_d(acc, red);
+    const double acc = canon_sum(s.cgrp_begin, s.cgrp_end, [&](int g) { return gq[g]; }, red);
     if (threadIdx.x == 0) s.alpha = acc > 0.0 ? s.gamma / acc : 0.0;
 }
 
@@ -79,14 +74,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_upd_cols(const Tile *__restrict_
     }
 }
 
-// s_i = mR_i dr_i sum_j A_ij rt_j ; mu += alpha p ; partial |s|^2.
+// s_i = mR_i dr_i sum_j A_ij rt_j ; mu += alpha p ; group partials of |s|^2.
 template <int TN, bool G>
 __global__ __launch_bounds__(kBlock) void k_adj_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjRowArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
@@ -101,23 +94,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_rows(const Tile *__restrict__ ti
         A.mu[i] += s.alpha * A.p[i];
         s2 = sv * sv;
     }
-    s2 = block_sum_d(s2, red);
-    if (threadIdx.x == 0) A.pr[static_cast<size_t>(t) * kPrCount + kPrDy2] = s2;
+    put_group(A.gs, tl, group_sum_d(s2));
 }
 
 // gamma' = |s|^2, beta = gamma'/gamma, stopping test, per instance.
-__global__ __launch_bounds__(kBlock) void k_adj_beta(AdjInst *__restrict__ st, int ninst,
-                                                     const double *__restrict__ pr, double tol,
-                                                     int max_iter) {
-    __shared__ double red[kBlock / 64];
+__global__ __launch_bounds__(kInstBlock) void k_adj_beta(AdjInst *__restrict__ st, int ninst,
+                                                         const double *__restrict__ gs, double tol,
+                                                         int max_iter) {
+    __shared__ double red[kInstBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     AdjInst &s = st[q];
     if (s.done) return;
-    double acc = 0.0;
-    for (int t = s.row_tile_begin + static_cast<int>(threadIdx.x); t < s.row_tile_end; t += kBlock)
-        acc += pr[static_cast<size_t>(t) * kPrCount + kPrDy2];
-    acc = block_sum_d(acc, red);
+    const double acc = canon_sum(s.rgrp_begin, s.rgrp_end, [&](int g) { return gs[g]; }, red);
     if (threadIdx.x != 0) return;
     if (s.iters == 0) s.gamma0 = acc;
     s.beta = s.gamma > 0.0 ? acc / s.gamma : 0.0;
@@ -175,10 +164,10 @@ namespace launch {
 void adj_step(hipStream_t s, const TileSet &rts, const TileSet &cts, AdjInst *st, int ninst,
               const AdjColArgs &c, const AdjRowArgs &r, double tol, int max_iter) {
     if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_adj_cols, st, c);
-    hipLaunchKernelGGL(dev::k_adj_alpha, dim3(ninst), dim3(kBlock), 0, s, st, ninst, c.pc);
+    hipLaunchKernelGGL(dev::k_adj_alpha, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, c.gq);
     if (cts.count) hipLaunchKernelGGL(dev::k_adj_upd_cols, MPDHG_TILE_GRID(cts), st, c);
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_adj_rows, st, r);
-    hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kBlock), 0, s, st, ninst, r.pr, tol, max_iter);
+    hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, r.gs, tol, max_iter);
     if (rts.count) hipLaunchKernelGGL(dev::k_adj_upd_rows, MPDHG_TILE_GRID(rts), st, r);
 }
 
@@ -186,7 +175,7 @@ void adj_start(hipStream_t s, const TileSet &rts, AdjInst *st, int ninst, const 
                double tol, int max_iter) {
     // s = K^T r0 with alpha = 0 (mu unchanged), gamma0 = |s|^2, p = s
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_adj_rows, st, r);
-    hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kBlock), 0, s, st, ninst, r.pr, tol, max_iter);
+    hipLaunchKernelGGL(dev::k_adj_beta, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, r.gs, tol, max_iter);
     if (rts.count) hipLaunchKernelGGL(dev::k_adj_upd_rows, MPDHG_TILE_GRID(rts), st, r);
 }
 

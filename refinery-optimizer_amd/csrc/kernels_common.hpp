@@ -60,10 +60,19 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
         t.k0 = ptr[r];
         t.k1 = ptr[r + 1];
     }
-    t.lo = ptr[tl.begin];
-    t.hi = ptr[tl.end];
+    t.lo = tl.lo;
+    t.hi = tl.hi;
     t.base = tl.base;
     return t;
+}
+
+// Refreshes the run hints of a tile list from the instance statuses (after
+// every kernel that may stop an instance).
+template <typename I>
+__global__ __launch_bounds__(kBlock) void k_tile_run(Tile *__restrict__ tiles, int ntiles,
+                                                     const I *__restrict__ st) {
+    const int t = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (t < ntiles) tiles[t].run = st[tiles[t].inst].status == kRunning ? 1 : 0;
 }
 
 // Row sums of one tile: sum_k val[k] * xs[col(k)] for every row of the
@@ -161,7 +170,8 @@ __device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ pt
 }
 
 // Block-wide max of non-negative floats (NaN operands are dropped, as
-// fmaxf does); every lane receives the result.
+// fmaxf does) over NT threads; every lane receives the result.
+template <int NT = kBlock>
 __device__ __forceinline__ float block_max(float v, float *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -171,10 +181,11 @@ __device__ __forceinline__ float block_max(float v, float *red) {
     __syncthreads();
     float r = red[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / 64; ++q) r = fmaxf(r, red[q]);
+    for (int q = 1; q < NT / 64; ++q) r = fmaxf(r, red[q]);
     return r;
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ double block_max_d(double v, double *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -184,12 +195,13 @@ __device__ __forceinline__ double block_max_d(double v, double *red) {
     __syncthreads();
     double r = red[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / 64; ++q) r = fmax(r, red[q]);
+    for (int q = 1; q < NT / 64; ++q) r = fmax(r, red[q]);
     return r;
 }
 
-// Block-wide FP64 sum in a fixed tree order (deterministic for a given
-// block shape); every lane receives the result.
+// Block-wide FP64 sum over NT threads in a fixed tree order; every lane
+// receives the result.
+template <int NT = kBlock>
 __device__ __forceinline__ double block_sum_d(double v, double *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -199,8 +211,49 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
     __syncthreads();
     double r = red[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / 64; ++q) r += red[q];
+    for (int q = 1; q < NT / 64; ++q) r += red[q];
     return r;
+}
+
+// FP64 sum / max over the kGroup lanes of a row group, in a fixed
+// butterfly order; every lane of the group receives the result.
+__device__ __forceinline__ double group_sum_d(double v) {
+#pragma unroll
+    for (int off = kGroup / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double group_max_d(double v) {
+#pragma unroll
+    for (int off = kGroup / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Stores a group partial: tiles start on group boundaries, so lanes
+// [kGroup g, kGroup (g + 1)) of a tile hold exactly group tl.grp + g (rows
+// past the tile end add 0).  Every lane of the workgroup must take part in
+// the group reduction before.
+__device__ __forceinline__ void put_group(double *part, const Tile &tl, double v) {
+    const int l = static_cast<int>(threadIdx.x);
+    if ((l & (kGroup - 1)) == 0 && tl.begin + l < tl.end) part[tl.grp + l / kGroup] = v;
+}
+
+// sum_{g in [b, e)} f(g) in FP64, in an order fixed by e - b alone: lane l
+// adds f(b + l), f(b + l + kInstBlock), ... in ascending order, then the
+// lanes are combined by the fixed block tree.  Run by the kInstBlock threads
+// of a per-instance kernel over an instance's groups; with per-group
+// partials formed by one wave each, the result does not depend on how the
+// rows were tiled.  `red` holds kInstBlock / 64 doubles.
+template <typename F>
+__device__ __forceinline__ double canon_sum(int b, int e, F f, double *red) {
+    double acc = 0.0;
+    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) acc += f(g);
+    return block_sum_d<kInstBlock>(acc, red);
+}
+template <typename F>
+__device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
+    double acc = 0.0;
+    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) acc = fmax(acc, f(g));
+    return block_max_d<kInstBlock>(acc, red);
 }
 
 }  // namespace dev
@@ -209,15 +262,18 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
 // bodies, with the stream in `s`).
 #define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
 
-// Launches KERNEL<tn, gather> for the tile set's chunk size and layout.
+// Launches KERNEL<tn, gather> for the tile set's chunk size and layout
+// (tn in 1024 .. kTileNnz; the last branch is kTileNnz itself).
 #define MPDHG_LAUNCH_TN1(ts, G, KERNEL, ...)                                                 \
     do {                                                                                     \
         if ((ts).tn == 1024)                                                                 \
             hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<1024, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
         else if ((ts).tn == 2048)                                                            \
             hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<2048, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
-        else                                                                                 \
+        else if (kTileNnz > 4096 && (ts).tn == 4096)                                         \
             hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<4096, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
+        else                                                                                 \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<kTileNnz, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
     } while (0)
 #define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                     \
     do {                                                                                     \

@@ -56,10 +56,8 @@ __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ ti
                                                      const float *__restrict__ cval,
                                                      const float *__restrict__ w,
                                                      float *__restrict__ z,
-                                                     double *__restrict__ pc) {
+                                                     double *__restrict__ gpow) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
@@ -70,20 +68,16 @@ __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ ti
         z[j] = s;
         q = static_cast<double>(s) * static_cast<double>(s);
     }
-    q = block_sum_d(q, red);
-    if (threadIdx.x == 0) pc[static_cast<size_t>(t) * kPcCount + kPcPow] = q;
+    put_group(gpow, tl, group_sum_d(q));
 }
 
-__global__ __launch_bounds__(kBlock) void k_pow_fin(EnhInst *__restrict__ st, int ninst,
-                                                    const double *__restrict__ pc) {
-    __shared__ double red[kBlock / 64];
+__global__ __launch_bounds__(kInstBlock) void k_pow_fin(EnhInst *__restrict__ st, int ninst,
+                                                        const double *__restrict__ gpow) {
+    __shared__ double red[kInstBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     EnhInst &s = st[q];
-    double acc = 0.0;
-    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock)
-        acc += pc[static_cast<size_t>(t) * kPcCount + kPcPow];
-    acc = block_sum_d(acc, red);
+    const double acc = canon_sum(s.cgrp_begin, s.cgrp_end, [&](int g) { return gpow[g]; }, red);
     if (threadIdx.x == 0) s.pow_norm2 = acc;
 }
 
@@ -96,7 +90,8 @@ struct PrimalArgs {
     float *x, *x0, *xp, *xbar;
     const double *xbase, *dc;
     double *xchk;
-    double *pc;
+    double *gc;  // column-group partials, quantity-major (kGc*), gcn groups
+    int gcn;
     const int *itb;  // first iteration of the current check cycle (device)
     float rho;
 };
@@ -108,13 +103,13 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
                                                        int chunk, const EnhInst *__restrict__ st,
                                                        PrimalArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
+    if (!tl.run) return;
     // Put every load that does not depend on the instance state in flight
-    // before the state is consumed: row pointers and epilogue operands.
+    // before the state is consumed: row pointers, the nonzero stream and
+    // epilogue operands.
     const TileRows tr = tile_rows(tl, A.cptr);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     const bool mine = j < tl.end;
@@ -127,10 +122,10 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
         hi = A.u[j];
     }
     const EnhInst &s = st[tl.inst];
-    if (s.status != kRunning) return;
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
     const float g = tile_dot_rows<TN, G, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -158,12 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
         }
     }
     if (check) {
-        d2 = block_sum_d(d2, red);
-        e2 = block_sum_d(e2, red);
-        if (threadIdx.x == 0) {
-            A.pc[static_cast<size_t>(t) * kPcCount + kPcDx2] = d2;
-            A.pc[static_cast<size_t>(t) * kPcCount + kPcDxa2] = e2;
-        }
+        put_group(A.gc + kGcDx2 * A.gcn, tl, group_sum_d(d2));
+        put_group(A.gc + kGcDxa2 * A.gcn, tl, group_sum_d(e2));
     }
 }
 
@@ -176,7 +167,8 @@ struct DualArgs {
     float *y, *y0, *yp;
     const double *ybase, *dr;
     double *ychk;
-    double *pr;
+    double *gr;  // row-group partials, quantity-major (kGr*), grn groups
+    int grn;
     const int *itb;
     float rho;
 };
@@ -186,11 +178,10 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
                                                      int chunk, const EnhInst *__restrict__ st,
                                                      DualArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
+    if (!tl.run) return;
     const TileRows tr = tile_rows(tl, A.rptr);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     const bool mine = i < tl.end;
@@ -203,10 +194,10 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
         hi = A.ru[i];
     }
     const EnhInst &s = st[tl.inst];
-    if (s.status != kRunning) return;
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
     const float h = tile_dot_rows<TN, G, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
+    if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
     const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
@@ -235,12 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
         }
     }
     if (check) {
-        d2 = block_sum_d(d2, red);
-        e2 = block_sum_d(e2, red);
-        if (threadIdx.x == 0) {
-            A.pr[static_cast<size_t>(t) * kPrCount + kPrDy2] = d2;
-            A.pr[static_cast<size_t>(t) * kPrCount + kPrDya2] = e2;
-        }
+        put_group(A.gr + kGrDy2 * A.grn, tl, group_sum_d(d2));
+        put_group(A.gr + kGrDya2 * A.grn, tl, group_sum_d(e2));
     }
 }
 
@@ -253,7 +240,8 @@ struct CheckRowArgs {
     const double *xchk, *ychk;
     const double *rl, *ru;
     double *ax;
-    double *pr;
+    double *gr;  // row-group partials, quantity-major (kGr*), grn groups
+    int grn;
 };
 
 template <int TN, bool G>
@@ -261,12 +249,10 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict_
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckRowArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const double axi = tile_dot_t<TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double viol = 0.0, sviol = 0.0, dobj = 0.0;
@@ -283,15 +269,9 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict_
         if (yi > 0.0) dobj = -hi * yi;
         else if (yi < 0.0) dobj = -lo * yi;
     }
-    viol = block_max_d(viol, red);
-    sviol = block_max_d(sviol, red);
-    dobj = block_sum_d(dobj, red);
-    if (threadIdx.x == 0) {
-        double *p = A.pr + static_cast<size_t>(t) * kPrCount;
-        p[kPrRowViol] = viol;
-        p[kPrSignViol] = sviol;
-        p[kPrDobj] = dobj;
-    }
+    put_group(A.gr + kGrRowViol * A.grn, tl, group_max_d(viol));
+    put_group(A.gr + kGrSignViol * A.grn, tl, group_max_d(sviol));
+    put_group(A.gr + kGrDobj * A.grn, tl, group_sum_d(dobj));
 }
 
 struct CheckColArgs {
@@ -300,7 +280,8 @@ struct CheckColArgs {
     const double *ychk, *xchk;
     const double *c, *l, *u;
     double *lam;
-    double *pc;
+    double *gc;  // column-group partials, quantity-major (kGc*), gcn groups
+    int gcn;
 };
 
 template <int TN, bool G>
@@ -308,12 +289,10 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict_
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
-    // block reductions run after the row sums, in the then idle staging area
-    double *const red = reinterpret_cast<double *>(lds);
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const double aty = tile_dot_t<TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
@@ -331,22 +310,18 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict_
         if (fin(hi) && xj > hi) cv = xj - hi;
         pobj = cj * xj;
     }
-    dv = block_max_d(dv, red);
-    cv = block_max_d(cv, red);
-    pobj = block_sum_d(pobj, red);
-    dobj = block_sum_d(dobj, red);
-    if (threadIdx.x == 0) {
-        double *p = A.pc + static_cast<size_t>(t) * kPcCount;
-        p[kPcDualViol] = dv;
-        p[kPcColViol] = cv;
-        p[kPcPobj] = pobj;
-        p[kPcDobj] = dobj;
-    }
+    put_group(A.gc + kGcDualViol * A.gcn, tl, group_max_d(dv));
+    put_group(A.gc + kGcColViol * A.gcn, tl, group_max_d(cv));
+    put_group(A.gc + kGcPobj * A.gcn, tl, group_sum_d(pobj));
+    put_group(A.gc + kGcDobj * A.gcn, tl, group_sum_d(dobj));
 }
 
 // ---- per-instance decisions -------------------------------------------------
 struct FinalizeArgs {
-    const double *pc, *pr;
+    const double *gc;  // column-group partials (kGc*), gcn groups
+    int gcn;
+    const double *gr;  // row-group partials (kGr*), grn groups
+    int grn;
     int32_t *status_out;
     double refine_ratio;
     int max_refinements;
@@ -359,9 +334,9 @@ __device__ __forceinline__ void set_steps(EnhInst &s) {
     s.inv_sigma = static_cast<float>(1.0 / (s.eta * s.omega));
 }
 
-__global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ st, int ninst,
-                                                         FinalizeArgs A, int off) {
-    __shared__ double red[kBlock / 64];
+__global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict__ st, int ninst,
+                                                             FinalizeArgs A, int off) {
+    __shared__ double red[kInstBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     EnhInst &s = st[q];
@@ -372,36 +347,21 @@ __global__ __launch_bounds__(kBlock) void k_enh_finalize(EnhInst *__restrict__ s
         }
         return;
     }
-    double dx2 = 0, dxa2 = 0, dv = 0, cv = 0, pobj = 0, dobjc = 0;
-    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock) {
-        const double *p = A.pc + static_cast<size_t>(t) * kPcCount;
-        dx2 += p[kPcDx2];
-        dxa2 += p[kPcDxa2];
-        dv = fmax(dv, p[kPcDualViol]);
-        cv = fmax(cv, p[kPcColViol]);
-        pobj += p[kPcPobj];
-        dobjc += p[kPcDobj];
-    }
-    double dy2 = 0, dya2 = 0, rv = 0, sv = 0, dobjr = 0;
-    for (int t = s.row_tile_begin + static_cast<int>(threadIdx.x); t < s.row_tile_end; t += kBlock) {
-        const double *p = A.pr + static_cast<size_t>(t) * kPrCount;
-        dy2 += p[kPrDy2];
-        dya2 += p[kPrDya2];
-        rv = fmax(rv, p[kPrRowViol]);
-        sv = fmax(sv, p[kPrSignViol]);
-        dobjr += p[kPrDobj];
-    }
-    dx2 = block_sum_d(dx2, red);
-    dxa2 = block_sum_d(dxa2, red);
-    dv = block_max_d(dv, red);
-    cv = block_max_d(cv, red);
-    pobj = block_sum_d(pobj, red);
-    dobjc = block_sum_d(dobjc, red);
-    dy2 = block_sum_d(dy2, red);
-    dya2 = block_sum_d(dya2, red);
-    rv = block_max_d(rv, red);
-    sv = block_max_d(sv, red);
-    dobjr = block_sum_d(dobjr, red);
+    // per instance over its groups, in a fixed order
+    const int cb = s.cgrp_begin, ce = s.cgrp_end, rb = s.rgrp_begin, re = s.rgrp_end;
+    const double *gc = A.gc, *gr = A.gr;
+    const int gcn = A.gcn, grn = A.grn;
+    const double dx2 = canon_sum(cb, ce, [&](int g) { return gc[kGcDx2 * gcn + g]; }, red);
+    const double dxa2 = canon_sum(cb, ce, [&](int g) { return gc[kGcDxa2 * gcn + g]; }, red);
+    const double pobj = canon_sum(cb, ce, [&](int g) { return gc[kGcPobj * gcn + g]; }, red);
+    const double dobjc = canon_sum(cb, ce, [&](int g) { return gc[kGcDobj * gcn + g]; }, red);
+    const double dv = canon_max(cb, ce, [&](int g) { return gc[kGcDualViol * gcn + g]; }, red);
+    const double cv = canon_max(cb, ce, [&](int g) { return gc[kGcColViol * gcn + g]; }, red);
+    const double dy2 = canon_sum(rb, re, [&](int g) { return gr[kGrDy2 * grn + g]; }, red);
+    const double dya2 = canon_sum(rb, re, [&](int g) { return gr[kGrDya2 * grn + g]; }, red);
+    const double dobjr = canon_sum(rb, re, [&](int g) { return gr[kGrDobj * grn + g]; }, red);
+    const double rv = canon_max(rb, re, [&](int g) { return gr[kGrRowViol * grn + g]; }, red);
+    const double sv = canon_max(rb, re, [&](int g) { return gr[kGrSignViol * grn + g]; }, red);
     if (threadIdx.x != 0) return;
 
     const int done = *A.itb + off + 1;
@@ -589,16 +549,6 @@ __global__ __launch_bounds__(kBlock) void k_enh_refine_rows(const Tile *__restri
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_enh_set_ranges(EnhInst *__restrict__ st, int ninst,
-                                                           const int32_t *__restrict__ rng) {
-    const int q = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
-    if (q >= ninst) return;
-    st[q].row_tile_begin = rng[4 * q];
-    st[q].row_tile_end = rng[4 * q + 1];
-    st[q].col_tile_begin = rng[4 * q + 2];
-    st[q].col_tile_end = rng[4 * q + 3];
-}
-
 // End of a check cycle of `len` iterations: the next cycle starts at *itb.
 __global__ void k_enh_advance(int *__restrict__ itb, int len) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *itb += len;
@@ -609,12 +559,6 @@ __global__ void k_enh_advance(int *__restrict__ itb, int len) {
 // ---- launchers --------------------------------------------------------------
 namespace launch {
 
-void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng) {
-    hipLaunchKernelGGL(dev::k_enh_set_ranges, dim3((ninst + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                       s, st, ninst, rng);
-}
-
-
 void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v) {
     if (cts.count) hipLaunchKernelGGL(dev::k_pow_scale, MPDHG_TILE_GRID(cts), st, z, v);
 }
@@ -623,22 +567,22 @@ void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *rid
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_pow_rows, rptr, ridx, rval, v, w);
 }
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
-              const float *cval, const float *w, float *z, double *pc) {
-    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_pow_cols, cptr, cidx, cval, w, z, pc);
+              const float *cval, const float *w, float *z, double *gpow) {
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_pow_cols, cptr, cidx, cval, w, z, gpow);
 }
-void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc) {
-    hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kBlock), 0, s, st, ninst, pc);
+void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow) {
+    hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, gpow);
 }
 void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
                 int check) {
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
-                      a.xbar, a.xbase, a.dc, a.xchk, a.pc, a.itb, a.rho};
+                      a.xbar, a.xbase, a.dc, a.xchk, a.gc, a.gcn, a.itb, a.rho};
     if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, off, check);
 }
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
               int check) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
-                    a.ybase, a.dr, a.ychk, a.pr, a.itb, a.rho};
+                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
 }
 void enh_advance(hipStream_t s, int *itb, int len) {
@@ -646,13 +590,15 @@ void enh_advance(hipStream_t s, int *itb, int len) {
 }
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
                const EnhCheck &a, int off) {
-    dev::CheckRowArgs R{a.rptr, a.ridx, a.rval64, a.xchk, a.ychk, a.rl64, a.ru64, a.ax, a.pr};
+    dev::CheckRowArgs R{a.rptr, a.ridx, a.rval64, a.xchk, a.ychk, a.rl64, a.ru64, a.ax, a.gr,
+                        a.grn};
     dev::CheckColArgs Cc{a.cptr, a.cidx, a.cval64, a.ychk, a.xchk, a.c64, a.l64, a.u64, a.lam,
-                         a.pc};
+                         a.gc, a.gcn};
     if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_chk_rows, st, R);
     if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_chk_cols, st, Cc);
-    dev::FinalizeArgs F{a.pc, a.pr, a.status_out, a.refine_ratio, a.max_refinements, a.itb};
-    hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, F, off);
+    dev::FinalizeArgs F{a.gc, a.gcn, a.gr, a.grn, a.status_out, a.refine_ratio, a.max_refinements,
+                        a.itb};
+    hipLaunchKernelGGL(dev::k_enh_finalize, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, F, off);
     if (cts.count)
         hipLaunchKernelGGL(dev::k_enh_best_cols, MPDHG_TILE_GRID(cts), st, a.xchk, a.lam, a.xbest,
                            a.lambest);
@@ -667,6 +613,14 @@ void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *s
         if (cts.count) hipLaunchKernelGGL(dev::k_enh_refine_cols, MPDHG_TILE_GRID(cts), st, RC);
         if (rts.count) hipLaunchKernelGGL(dev::k_enh_refine_rows, MPDHG_TILE_GRID(rts), st, RR);
     }
+    tile_run(s, rts, st);
+    tile_run(s, cts, st);
+}
+
+void tile_run(hipStream_t s, const TileSet &ts, const EnhInst *st) {
+    if (ts.count)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<EnhInst>), dim3((ts.count + kBlock - 1) / kBlock),
+                           dim3(kBlock), 0, s, ts.tiles, ts.count, st);
 }
 
 

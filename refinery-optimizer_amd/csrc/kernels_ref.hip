@@ -26,7 +26,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) {
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const float h = tile_dot<TN, G, float>(tl, rptr, ridx, rval, xbar, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) {
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_rowres(
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const float ax = tile_dot<TN, G, float>(tl, rptr, ridx, rval, x, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_colres(
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    if (st[tl.inst].status != kRunning) return;
+    if (!tl.run || st[tl.inst].status != kRunning) return;
     const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
@@ -126,11 +126,11 @@ __global__ __launch_bounds__(kBlock) void k_ref_colres(
 }
 
 // gpu_solver.cpp:252-270 per instance: P, D, best-x bookkeeping, stop test.
-__global__ __launch_bounds__(kBlock) void k_ref_finalize(RefInst *__restrict__ st, int ninst,
-                                                         const float *__restrict__ prow,
-                                                         const float *__restrict__ pcol, int iter,
-                                                         int max_iter, float tol_p, float tol_d) {
-    __shared__ float red[kBlock / 64];
+__global__ __launch_bounds__(kInstBlock) void k_ref_finalize(RefInst *__restrict__ st, int ninst,
+                                                             const float *__restrict__ prow,
+                                                             const float *__restrict__ pcol, int iter,
+                                                             int max_iter, float tol_p, float tol_d) {
+    __shared__ float red[kInstBlock / 64];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     RefInst &s = st[q];
@@ -139,12 +139,12 @@ __global__ __launch_bounds__(kBlock) void k_ref_finalize(RefInst *__restrict__ s
         return;
     }
     float mp = 0.0f, md = 0.0f;
-    for (int t = s.row_tile_begin + static_cast<int>(threadIdx.x); t < s.row_tile_end; t += kBlock)
+    for (int t = s.row_tile_begin + static_cast<int>(threadIdx.x); t < s.row_tile_end; t += kInstBlock)
         mp = fmaxf(mp, prow[t]);
-    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kBlock)
+    for (int t = s.col_tile_begin + static_cast<int>(threadIdx.x); t < s.col_tile_end; t += kInstBlock)
         md = fmaxf(md, pcol[t]);
-    mp = block_max(mp, red);
-    md = block_max(md, red);
+    mp = block_max<kInstBlock>(mp, red);
+    md = block_max<kInstBlock>(md, red);
     if (threadIdx.x == 0) {
         const float p = mp / s.max_abs_b;
         const float d = md / s.max_abs_c;
@@ -212,7 +212,7 @@ void ref_colres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *
 
 void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, const float *pcol,
                   int iter, int max_iter, float tol_p, float tol_d) {
-    hipLaunchKernelGGL(dev::k_ref_finalize, dim3(ninst), dim3(kBlock), 0, s, st, ninst, prow,
+    hipLaunchKernelGGL(dev::k_ref_finalize, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, prow,
                        pcol, iter, max_iter, tol_p, tol_d);
 }
 
@@ -220,6 +220,12 @@ void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const fl
                    float *xbest) {
     hipLaunchKernelGGL(dev::k_ref_copy_best, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles,
                        ts.count, ts.chunk, st, x, xbest);
+}
+
+void tile_run(hipStream_t s, const TileSet &ts, const RefInst *st) {
+    if (ts.count)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<RefInst>), dim3((ts.count + kBlock - 1) / kBlock),
+                           dim3(kBlock), 0, s, ts.tiles, ts.count, st);
 }
 
 }  // namespace launch

@@ -9,7 +9,7 @@ namespace mpdhg {
 
 // A device tile table with its padded, XCD-remapped launch grid.
 struct TileSet {
-    const Tile *tiles = nullptr;
+    Tile *tiles = nullptr;  // run hints are refreshed in place (tile_run)
     int count = 0;
     int chunk = 0;        // ceil(count / 8)
     int grid = 0;         // 8 * chunk
@@ -36,6 +36,9 @@ void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, cons
                   int iter, int max_iter, float tol_p, float tol_d);
 void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const float *x,
                    float *xbest);
+// Tile::run = (instance status == kRunning) for every tile of the set.
+void tile_run(hipStream_t s, const TileSet &ts, const RefInst *st);
+void tile_run(hipStream_t s, const TileSet &ts, const EnhInst *st);
 
 // enhanced mode
 struct EnhPrimal {
@@ -46,7 +49,8 @@ struct EnhPrimal {
     float *x, *x0, *xp, *xbar;
     const double *xbase, *dc;
     double *xchk;
-    double *pc;
+    double *gc;  // column-group partials (kGc* quantities x gcn groups)
+    int gcn;
     const int *itb;  // device: first iteration of the current check cycle
     float rho;
 };
@@ -58,7 +62,8 @@ struct EnhDual {
     float *y, *y0, *yp;
     const double *ybase, *dr;
     double *ychk;
-    double *pr;
+    double *gr;  // row-group partials (kGr* quantities x grn groups)
+    int grn;
     const int *itb;
     float rho;
 };
@@ -67,7 +72,10 @@ struct EnhCheck {
     const double *rval64, *cval64;
     double *xchk, *ychk, *ax, *lam;
     const double *rl64, *ru64, *c64, *l64, *u64, *dc, *dr;
-    double *pr, *pc;
+    double *gc;  // column-group partials (kGc* quantities x gcn groups)
+    int gcn;
+    double *gr;  // row-group partials (kGr* quantities x grn groups)
+    int grn;
     int32_t *status_out;
     double refine_ratio;
     int max_refinements;
@@ -79,9 +87,11 @@ struct EnhCheck {
 void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v);
 void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
               const float *rval, const float *v, float *w);
+// z = A^T w, with the column-group partials of |z|^2 in gpow.
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
-              const float *cval, const float *w, float *z, double *pc);
-void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *pc);
+              const float *cval, const float *w, float *z, double *gpow);
+// pow_norm2 = |z|^2 per instance, summed over its column groups in a fixed order.
+void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow);
 // Iteration *a.itb + off of the current check cycle.
 void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
                 int check);
@@ -89,9 +99,6 @@ void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDua
               int check);
 // *itb += len (end of a check cycle of len iterations).
 void enh_advance(hipStream_t s, int *itb, int len);
-// Point instance q's partial-reduction ranges at rng[4q..4q+3] (row tile
-// begin/end, column tile begin/end) after a tile-list compaction.
-void enh_set_ranges(hipStream_t s, EnhInst *st, int ninst, const int32_t *rng);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)
 // residual-LP construction, in that stream order.
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,
@@ -104,7 +111,7 @@ struct AdjColArgs {
     const double *pt;    // gathered: mR * dr * p (or dr * mu for the final pass)
     const double *mb, *dc;
     double *q, *r, *rt;
-    double *pc;
+    double *gq;  // column-group partials of |q|^2
 };
 struct AdjRowArgs {
     const int *rptr, *ridx;
@@ -112,7 +119,7 @@ struct AdjRowArgs {
     const double *rt;    // gathered: mB * dc * r
     const double *mr, *dr;
     double *s, *mu, *p, *pt;
-    double *pr;
+    double *gs;  // row-group partials of |s|^2
 };
 void adj_start(hipStream_t s, const TileSet &rts, AdjInst *st, int ninst, const AdjRowArgs &r,
                double tol, int max_iter);

@@ -113,17 +113,23 @@ def test_iteration_limit_and_short_runs(need_gpu):
     assert S.solve_gpu_pdhg(e, enh()).status == S.SolutionStatus.Optimal
 
 
-def test_gather_order_layout_is_bit_identical(need_gpu):
+@pytest.mark.parametrize("layout,chunk", [(1, 0), (0, 1024), (0, 2048), (1, 4096)])
+def test_spmv_schedules_are_bit_identical(need_gpu, layout, chunk):
+    # The tile schedule (layout, chunk size) changes how rows are grouped
+    # into workgroups, never a result: row sums keep storage order, and the
+    # FP64 sums behind step size, restarts and stopping are reduced per
+    # instance in a fixed order.  Same iterates, decisions and KKT bits.
     probs = [S.gen_random(300, 720, 8, 3), S.gen_staircase(2, 40, 90, 4), random_lp(60, 90, 0.1, 2)[0],
-             toy01()]
+             toy01(), S.gen_random(2000, 4800, 8, 9)]
+    a = S.solve_batch(probs, enh(tol=1e-6))
     cfg = enh(tol=1e-6)
-    a = S.solve_batch(probs, cfg)
-    cfg.spmv_layout = 1
+    cfg.spmv_layout = layout
+    cfg.spmv_chunk = chunk
     b = S.solve_batch(probs, cfg)
     for p, s, t in zip(probs, a, b):
         assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
         assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
-        assert s.rel_kkt == t.rel_kkt, p.name
+        assert s.rel_kkt == t.rel_kkt and s.step_size == t.step_size, p.name
 
 
 def test_graph_replay_equals_eager_launches(need_gpu):
