@@ -77,17 +77,23 @@ typedef struct mpdhg_config {
     double refine_ratio;          /* enhanced: at a restart, re-centre the FP32 iterates on the
                                      restart point (one refinement round) once the FP64 KKT fell
                                      below ratio * its value at the last re-centring (1e-2) */
-    int32_t spmv_layout;          /* 0: auto -- each chunk of <= 4096 nonzeros stored sorted by
-                                     gathered index where the instance's vectors have < 2^20
-                                     entries; 1: storage order everywhere.  Results are
-                                     bit-identical either way (same per-row summation order). */
-    int32_t spmv_chunk;           /* nonzeros staged through LDS per workgroup pass: 0 = auto
-                                     (the smallest of 1024/2048/4096 holding 256 rows of average
-                                     length, per direction), or 1024, 2048, 4096 */
+    int32_t spmv_layout;          /* 0: auto -- each LDS chunk of nonzeros stored sorted by
+                                     gathered index where every instance's vectors are short
+                                     enough for the packed index word (< 2^32 / chunk entries);
+                                     1: storage order everywhere. */
+    int32_t spmv_chunk;           /* nonzeros staged through LDS per workgroup pass: 0 = auto,
+                                     or 1024, 2048, 4096, 8192 */
     int32_t launch_graph;         /* enhanced: 1 = record one check cycle (the fused launch pairs,
                                      the FP64 check, the decisions) as a HIP graph and replay it
-                                     for every full cycle; 0 = launch kernel by kernel.  Same
-                                     results either way. */
+                                     for every full cycle; 0 = launch kernel by kernel. */
+    int32_t spmv_block;           /* threads per SpMV workgroup (rows per tile at most): 0 = auto,
+                                     or 128, 256, 512.  (spmv_block, spmv_chunk) must be one of
+                                     (128,1024) (128,2048) (256,1024) (256,2048) (256,4096)
+                                     (512,4096) (512,8192) when both are set.  Layout, chunk,
+                                     block and launch_graph change speed only: every result is
+                                     bit-identical for every choice (rows are summed in storage
+                                     order, and every FP64 sum behind a decision is reduced per
+                                     16-row group and then per instance in a fixed order). */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite
@@ -128,6 +134,8 @@ typedef struct mpdhg_result {
     double bytes_csr_launch, bytes_csc_launch;     /* algorithmic bytes per fused launch */
     double step_size;        /* enhanced: eta (tau = eta/omega, sigma = eta*omega) */
     double primal_weight;    /* enhanced: final omega */
+    int32_t csr_block, csr_chunk;  /* tile schedule the solve used for A x (CSR) ... */
+    int32_t csc_block, csc_chunk;  /* ... and for A^T y (CSC) */
 } mpdhg_result;
 
 typedef struct mpdhg_ctx mpdhg_ctx;

@@ -50,7 +50,12 @@ int check_config(const mpdhg_config *cfg) {
     if (cfg->spmv_layout != 0 && cfg->spmv_layout != 1)
         return mpdhg::fail(MPDHG_E_ARG, "spmv_layout must be 0 (auto) or 1 (storage order)");
     if (cfg->spmv_chunk != 0 && !mpdhg::valid_chunk(cfg->spmv_chunk))
-        return mpdhg::fail(MPDHG_E_ARG, "spmv_chunk must be 0 (auto), 1024, 2048 or 4096");
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_chunk must be 0 (auto), 1024, 2048, 4096 or 8192");
+    if (cfg->spmv_block != 0 && !mpdhg::valid_block(cfg->spmv_block))
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_block must be 0 (auto), 128, 256 or 512");
+    if (cfg->spmv_block != 0 && cfg->spmv_chunk != 0 &&
+        !mpdhg::valid_schedule(cfg->spmv_block, cfg->spmv_chunk))
+        return mpdhg::fail(MPDHG_E_ARG, "(spmv_block, spmv_chunk) is not a compiled tile schedule");
     if (cfg->launch_graph != 0 && cfg->launch_graph != 1)
         return mpdhg::fail(MPDHG_E_ARG, "launch_graph must be 0 or 1");
     return MPDHG_OK;
@@ -114,6 +119,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->spmv_layout = 0;
     cfg->spmv_chunk = 0;
     cfg->launch_graph = 1;
+    cfg->spmv_block = 0;
 }
 
 int mpdhg_device_count(int *count) {

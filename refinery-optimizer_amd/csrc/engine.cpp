@@ -75,16 +75,16 @@ void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
         }
 }
 
-// Rows [start, i) form one tile while the tile has at most kBlock rows and
+// Rows [start, i) form one tile while the tile has at most `block` rows and
 // at most tn nonzeros, growing by whole kGroup-row groups; a single denser
 // group gets a tile of its own (the kernel streams it in several chunks).
 void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int64_t nz_off,
-                int64_t grp_off, int inst, int tn, std::vector<Tile> &out) {
+                int64_t grp_off, int inst, int block, int tn, std::vector<Tile> &out) {
     int i = 0;
     while (i < count) {
         const int start = i;
         int nz = 0;
-        while (i < count && i - start < kBlock) {
+        while (i < count && i - start < block) {
             const int ge = std::min(count, i + kGroup);
             const int gn = ptr[static_cast<size_t>(ge)] - ptr[static_cast<size_t>(i)];
             if (i > start && nz + gn > tn) break;
@@ -126,24 +126,45 @@ BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &ac
     return L;
 }
 
-TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather) {
+TileSet make_tileset(const DBuf<Tile> &d, int count, Schedule sc, bool gather) {
     TileSet ts;
     ts.tiles = d.get();
     ts.count = count;
     ts.chunk = (count + 7) / 8;
     ts.grid = 8 * ts.chunk;
-    ts.tn = tn;
+    ts.block = sc.block;
+    ts.tn = sc.tn;
     ts.gather = gather;
     return ts;
 }
 
-int choose_chunk(int64_t nnz, int64_t rows, int forced) {
-    if (forced) return forced;
-    int tn = kChunkMin;
-    while (tn < kTileNnz && static_cast<double>(tn) < static_cast<double>(kBlock) * nnz /
-                                                         static_cast<double>(std::max<int64_t>(rows, 1)))
-        tn *= 2;
-    return tn;
+Schedule choose_schedule(int64_t nnz, int64_t rows, int forced_block, int forced_tn) {
+    if (forced_block && forced_tn) {
+        if (!valid_schedule(forced_block, forced_tn))
+            throw Error(MPDHG_E_ARG, "spmv_block / spmv_chunk name no compiled tile schedule");
+        return Schedule{forced_block, forced_tn};
+    }
+    const double avg = static_cast<double>(nnz) / static_cast<double>(std::max<int64_t>(rows, 1));
+    Schedule best{0, 0};
+    for (int b : {512, 256, 128}) {
+        if (forced_block && b != forced_block) continue;
+        // the smallest chunk of this workgroup size that holds b rows of
+        // average length (the largest one if none does)
+        const double need = b * avg;
+        Schedule sc{0, 0};
+        for (const Schedule &c : kSchedules) {
+            if (c.block != b || (forced_tn && c.tn != forced_tn)) continue;
+            const bool cf = c.tn >= need, sf = sc.block && sc.tn >= need;
+            if (!sc.block || (cf && !sf) || (cf && sf && c.tn < sc.tn) || (!cf && !sf && c.tn > sc.tn))
+                sc = c;
+        }
+        if (!sc.block) continue;
+        const int64_t tiles = std::max<int64_t>((rows + b - 1) / b, (nnz + sc.tn - 1) / sc.tn);
+        best = sc;
+        if (tiles >= kTilesWanted) break;
+    }
+    if (!best.block) throw Error(MPDHG_E_ARG, "spmv_block / spmv_chunk name no compiled tile schedule");
+    return best;
 }
 
 static inline bool finite_bound(double v) { return std::isfinite(v); }
@@ -252,8 +273,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
-    const int ctn = choose_chunk(L.nnz, L.cols, cfg.spmv_chunk);
+    const Schedule rsc = choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk);
+    const Schedule csc = choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk);
     hh.cptr.resize(C + 1);
     hh.cidx.resize(Z);
     hh.rptr.resize(R + 1);
@@ -315,10 +336,10 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         s.max_abs_c = mac;
         s.max_abs_b = mab;
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, ro, zo, L.rgrp_off[static_cast<size_t>(a)], a, rtn, hh.rtiles);
+        plan_tiles(h.rptr, h.m, ro, zo, L.rgrp_off[static_cast<size_t>(a)], a, rsc.block, rsc.tn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, co, zo, L.cgrp_off[static_cast<size_t>(a)], a, ctn, hh.ctiles);
+        plan_tiles(h.cptr, h.n, co, zo, L.cgrp_off[static_cast<size_t>(a)], a, csc.block, csc.tn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     bool rgo = false, cgo = false;
@@ -330,8 +351,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        rgo = gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
-        cgo = gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
+        rgo = gather_order(lay, rsc.tn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
+        cgo = gather_order(lay, csc.tn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
     }
     // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
     hipStream_t st = ctx.stream;
@@ -350,8 +371,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
     DBuf<float> d_prow(hh.rtiles.size()), d_pcol(hh.ctiles.size());
-    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn, rgo);
-    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn, cgo);
+    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rsc, rgo);
+    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), csc, cgo);
     MPDHG_HIP(hipStreamSynchronize(st));
 
     // ---- profiling samples: HIP events around the fused SpMV launches ----
@@ -469,6 +490,13 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         r.kernel_launches_csc = r.kernel_launches_csr = used;
         r.bytes_csc_launch = 8.0 * L.nnz + 4.0 * (L.cols + 1) + 4.0 * L.rows + 4.0 * 7 * L.cols;
         r.bytes_csr_launch = 8.0 * L.nnz + 4.0 * (L.rows + 1) + 4.0 * L.cols + 4.0 * 5 * L.rows;
+    }
+    for (int a = 0; a < na; ++a) {
+        mpdhg_result &r = res[act[static_cast<size_t>(a)]];
+        r.csr_block = rsc.block;
+        r.csr_chunk = rsc.tn;
+        r.csc_block = csc.block;
+        r.csc_chunk = csc.tn;
     }
     const double wall = now_sec() - t_call;
     for (int q = 0; q < count; ++q) res[q].time_wall_sec = wall;

@@ -100,19 +100,21 @@ struct BatchLayout {
 BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &act);
 
 // Greedy tile plan over one instance's compressed structure: whole
-// kGroup-row groups, at most kBlock rows and at most tn nonzeros per tile (a
-// denser group alone, streamed in several chunks).  global_off / nz_off /
+// kGroup-row groups, at most `block` rows and at most tn nonzeros per tile
+// (a denser group alone, streamed in several chunks).  global_off / nz_off /
 // grp_off: the instance's first row, first nonzero and first group in the
 // batch.
 void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int64_t nz_off,
-                int64_t grp_off, int inst, int tn, std::vector<Tile> &out);
-TileSet make_tileset(const DBuf<Tile> &d, int count, int tn, bool gather);
+                int64_t grp_off, int inst, int block, int tn, std::vector<Tile> &out);
+TileSet make_tileset(const DBuf<Tile> &d, int count, Schedule sc, bool gather);
 
-// Chunk size of a tile set over `rows` rows holding `nnz` nonzeros: the
-// smallest of 1024, 2048, 4096 that holds kBlock rows of average length, so
-// that the LDS a workgroup reserves matches what a full tile stages.
-// `forced` (mpdhg_config::spmv_chunk) overrides it when nonzero.
-int choose_chunk(int64_t nnz, int64_t rows, int forced);
+// Schedule of a tile set over `rows` rows holding `nnz` nonzeros, by rule:
+// the largest workgroup that still gives at least kTilesWanted tiles (a
+// full GPU), with the smallest chunk holding `block` rows of average
+// length.  forced_block / forced_tn (mpdhg_config::spmv_block / spmv_chunk)
+// override either when nonzero; throws if they name no schedule.
+constexpr int64_t kTilesWanted = 1024;
+Schedule choose_schedule(int64_t nnz, int64_t rows, int forced_block, int forced_tn);
 
 struct Context {
     int device = 0;

@@ -35,8 +35,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const int rtn = choose_chunk(L.nnz, L.rows, 0);
-    const int ctn = choose_chunk(L.nnz, L.cols, 0);
+    const Schedule rsc = choose_schedule(L.nnz, L.rows, 0, 0);
+    const Schedule csc = choose_schedule(L.nnz, L.cols, 0, 0);
     std::vector<int> cptr(C + 1), cidx(Z), rptr(R + 1), ridx(Z);
     std::vector<double> cval(Z), rval(Z), dc(C), dr(R), mb(C), mr(R), r0(C), rt0(C);
     std::vector<AdjInst> inst(static_cast<size_t>(na));
@@ -109,8 +109,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         s.rgrp_end = static_cast<int32_t>(L.rgrp_off[ua] + (h.m + kGroup - 1) / kGroup);
         s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
-        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rtn, rtiles);
-        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, ctn, ctiles);
+        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rsc.block, rsc.tn, rtiles);
+        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, csc.block, csc.tn, ctiles);
     }
     bool rgo = false, cgo = false;
     {
@@ -120,8 +120,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
             rlen[static_cast<size_t>(a)] = h.m;
             clen[static_cast<size_t>(a)] = h.n;
         }
-        rgo = gather_order(Layout::Auto, rtn, rptr, rtiles, L.col_off, clen, ridx, rval);
-        cgo = gather_order(Layout::Auto, ctn, cptr, ctiles, L.row_off, rlen, cidx, cval);
+        rgo = gather_order(Layout::Auto, rsc.tn, rptr, rtiles, L.col_off, clen, ridx, rval);
+        cgo = gather_order(Layout::Auto, csc.tn, cptr, ctiles, L.row_off, rlen, cidx, cval);
     }
     hipStream_t st = ctx.stream;
     DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
@@ -138,8 +138,8 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     MPDHG_HIP(hipMemsetAsync(d_q.get(), 0, C * sizeof(double), st));
     // per-group partials of |q|^2 (columns) and |s|^2 (rows)
     DBuf<double> d_gq(static_cast<size_t>(L.cgroups)), d_gs(static_cast<size_t>(L.rgroups));
-    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rtn, rgo);
-    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), ctn, cgo);
+    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rsc, rgo);
+    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), csc, cgo);
     launch::AdjColArgs CA{d_cptr.get(), d_cidx.get(), d_cval.get(), d_pt.get(), d_mb.get(), d_dc.get(),
                           d_q.get(), d_r.get(), d_rt.get(), d_gq.get()};
     launch::AdjRowArgs RA{d_rptr.get(), d_ridx.get(), d_rval.get(), d_rt.get(), d_mr.get(), d_dr.get(),

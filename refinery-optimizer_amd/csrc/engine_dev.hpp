@@ -5,27 +5,40 @@
 
 namespace mpdhg {
 
-#ifndef MPDHG_BLOCK
-#define MPDHG_BLOCK 256
-#endif
-
-// Workgroup shape of every tile kernel: kBlock / 64 wave64s, one row per
-// lane in the epilogue.
-constexpr int kBlock = MPDHG_BLOCK;
-static_assert(kBlock == 256 || kBlock == 512, "tile workgroups of 4 or 8 waves");
-
-// Chunks: a tile set has a chunk size tn, a power of two in [kChunkMin,
-// kTileNnz]; a tile holds at most kBlock rows and at most tn nonzeros
-// unless it is a single longer row, and its nonzeros [lo, hi) are staged
-// through LDS in chunks [lo + j tn, lo + (j+1) tn).  The LDS a workgroup
-// reserves is sized by tn, so sparse structures (short rows) use small
-// chunks and more workgroups fit on a CU.
-constexpr int kTileNnz = 16 * kBlock;
+// Tile schedules: a tile set runs one workgroup of `block` threads per tile
+// (one row per lane in the epilogue: a tile holds at most `block` rows) and
+// stages `tn` nonzeros per LDS pass (a tile holds at most tn nonzeros unless
+// it is a single denser row group, streamed in several passes
+// [lo + j tn, lo + (j+1) tn)).  The LDS a workgroup reserves is sized by tn.
+// Each (block, tn) pair below is compiled into every tile kernel; which one
+// a solve uses is a performance choice only (results do not depend on it).
+struct Schedule {
+    int block;
+    int tn;
+};
+constexpr Schedule kSchedules[] = {{128, 1024}, {128, 2048}, {256, 1024}, {256, 2048},
+                                   {256, 4096}, {512, 4096}, {512, 8192}};
+constexpr int kNumSchedules = static_cast<int>(sizeof(kSchedules) / sizeof(kSchedules[0]));
+constexpr int kBlockMax = 512;
+constexpr int kTileNnz = 8192;  // largest tn
 constexpr int kChunkMin = 1024;
+
+constexpr bool valid_schedule(int block, int tn) {
+    for (int i = 0; i < kNumSchedules; ++i)
+        if (kSchedules[i].block == block && kSchedules[i].tn == tn) return true;
+    return false;
+}
+constexpr bool valid_block(int block) {
+    for (int i = 0; i < kNumSchedules; ++i)
+        if (kSchedules[i].block == block) return true;
+    return false;
+}
 
 constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
 constexpr bool valid_chunk(int tn) {
-    return tn >= kChunkMin && tn <= kTileNnz && (tn & (tn - 1)) == 0;
+    for (int i = 0; i < kNumSchedules; ++i)
+        if (kSchedules[i].tn == tn) return true;
+    return false;
 }
 
 // Gather order: within each chunk the nonzeros may be stored sorted by the
@@ -49,7 +62,8 @@ constexpr int kGroup = 16;
 static_assert(kGroup <= 64 && (kGroup & (kGroup - 1)) == 0, "groups within a wave");
 
 // A tile: rows [begin, end) of one instance's compressed structure
-// (CSR rows for A x, CSC columns for A^T y).  end - begin <= kBlock, and
+// (CSR rows for A x, CSC columns for A^T y).  end - begin <= the tile set's
+// block, and
 // begin is the first row of a group (grp: that group's index in the batch).
 // base >= 0: the tile's chunks are in gather order, gathered vector entries
 // at base + local; base == -1: storage order, index words are global.

@@ -122,8 +122,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const int rtn = choose_chunk(L.nnz, L.rows, cfg.spmv_chunk);
-    const int ctn = choose_chunk(L.nnz, L.cols, cfg.spmv_chunk);
+    const Schedule rsc = choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk);
+    const Schedule csc = choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk);
     EnhHost hh;
     hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.ridx.resize(Z);
     hh.cval.resize(Z); hh.rval.resize(Z); hh.cval64.resize(Z); hh.rval64.resize(Z);
@@ -215,10 +215,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
         s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rtn, hh.rtiles);
+        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rsc.block, rsc.tn, hh.rtiles);
         s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
         s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, ctn, hh.ctiles);
+        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, csc.block, csc.tn, hh.ctiles);
         s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
     }
     bool rgo = false, cgo = false;
@@ -230,8 +230,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             clen[static_cast<size_t>(a)] = h.n;
         }
         const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        rgo = gather_order(lay, rtn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
-        cgo = gather_order(lay, ctn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
+        rgo = gather_order(lay, rsc.tn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
+        cgo = gather_order(lay, csc.tn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
     }
 
     // ---- device buffers ----
@@ -272,8 +272,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
         d_status.upload(run, st);
     }
-    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rtn, rgo);
-    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), ctn, cgo);
+    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rsc, rgo);
+    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), csc, cgo);
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -437,8 +437,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     keep_rt.back().upload(nrt, st);
                     keep_ct.emplace_back();
                     keep_ct.back().upload(nct, st);
-                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rtn, rgo);
-                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), ctn, cgo);
+                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rsc, rgo);
+                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), csc, cgo);
                     covered = running;
                     ++lists;
                     host_keep_rt.push_back(std::move(nrt));
@@ -494,6 +494,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         r.rel_kkt = s.best_kkt;
         r.step_size = s.eta;
         r.primal_weight = s.omega;
+        r.csr_block = rsc.block;
+        r.csr_chunk = rsc.tn;
+        r.csc_block = csc.block;
+        r.csc_chunk = csc.tn;
         if (r.x) std::memcpy(r.x, xb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
         if (r.reduced_costs) std::memcpy(r.reduced_costs, lb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
         if (r.row_activity) std::memcpy(r.row_activity, ab.data() + ro, static_cast<size_t>(h.m) * sizeof(double));

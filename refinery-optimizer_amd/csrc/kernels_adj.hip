@@ -25,8 +25,8 @@ using launch::AdjRowArgs;
 
 // q_j = mB_j dc_j sum_i A_ij pt_i (pt = masked, D_r-scaled row vector);
 // also rt_j for the next row SpMV after the residual update r -= alpha q.
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_adj_cols(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_adj_cols(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(const Tile *__restrict__ ti
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].done) return;
-    const double s = tile_dot_t<TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q2 = 0.0;
     if (j < tl.end) {
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kInstBlock) void k_adj_alpha(AdjInst *__restrict__ 
 }
 
 // r -= alpha q; rt = mB dc r (column side, elementwise).
-__global__ __launch_bounds__(kBlock) void k_adj_upd_cols(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_adj_upd_cols(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const AdjInst *__restrict__ st,
                                                          AdjColArgs A) {
     const int t = xcd_tile(blockIdx.x, chunk);
@@ -75,8 +75,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_upd_cols(const Tile *__restrict_
 }
 
 // s_i = mR_i dr_i sum_j A_ij rt_j ; mu += alpha p ; group partials of |s|^2.
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_adj_rows(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_adj_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const AdjInst *__restrict__ st,
                                                      AdjRowArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_rows(const Tile *__restrict__ ti
     const Tile tl = tiles[t];
     const AdjInst &s = st[tl.inst];
     if (s.done) return;
-    const double v = tile_dot_t<TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.rt, lds);
+    const double v = tile_dot_t<B, TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.rt, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double s2 = 0.0;
     if (i < tl.end) {
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kInstBlock) void k_adj_beta(AdjInst *__restrict__ s
 }
 
 // p = s + beta p ; pt = mR dr p (row side, elementwise).
-__global__ __launch_bounds__(kBlock) void k_adj_upd_rows(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_adj_upd_rows(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const AdjInst *__restrict__ st,
                                                          AdjRowArgs A) {
     const int t = xcd_tile(blockIdx.x, chunk);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_upd_rows(const Tile *__restrict_
 }
 
 // mu (scaled) -> pt_i = dr_i mu_i, the multiplier of the original rows.
-__global__ __launch_bounds__(kBlock) void k_adj_unscale(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_adj_unscale(const Tile *__restrict__ tiles, int ntiles,
                                                         int chunk, AdjRowArgs A) {
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -144,14 +144,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_unscale(const Tile *__restrict__
 
 // Final column pass in the original space: q_j = sum_i A_ij pt_i = (A^T mu)_j
 // with the unscaled multipliers, for the bound gradients.
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_adj_atmu(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_adj_atmu(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, AdjColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const double s = tile_dot_t<TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) A.q[j] = s;
 }

@@ -69,14 +69,15 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
 // Refreshes the run hints of a tile list from the instance statuses (after
 // every kernel that may stop an instance).
 template <typename I>
-__global__ __launch_bounds__(kBlock) void k_tile_run(Tile *__restrict__ tiles, int ntiles,
-                                                     const I *__restrict__ st) {
-    const int t = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+__global__ __launch_bounds__(256) void k_tile_run(Tile *__restrict__ tiles, int ntiles,
+                                                  const I *__restrict__ st) {
+    const int t = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (t < ntiles) tiles[t].run = st[tiles[t].inst].status == kRunning ? 1 : 0;
 }
 
 // Row sums of one tile: sum_k val[k] * xs[col(k)] for every row of the
-// tile, in storage order (k ascending), one row per lane.  Each chunk of TN
+// tile, in storage order (k ascending), one row per lane of a workgroup of
+// B threads.  Each chunk of TN
 // nonzeros is streamed with coalesced loads of the index words and values,
 // the gathers are issued back to back, and the (value, x) pairs are parked
 // in LDS at their storage positions (the position is k - chunk start in
@@ -87,17 +88,17 @@ __global__ __launch_bounds__(kBlock) void k_tile_run(Tile *__restrict__ tiles, i
 // hold stage_bytes(TN).  The layout is a template parameter (one per tile
 // set) so that each instantiation carries one addressing path: two paths in
 // one kernel cost a third more registers and occupancy with it.
-template <int TN, bool G, typename V, typename X, typename A>
+template <int B, int TN, bool G, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw) {
-    static_assert(valid_chunk(TN), "chunk size");
+    static_assert(valid_schedule(B, TN), "tile schedule");
     constexpr int kBits = ilog2(TN);
     constexpr uint32_t kMask = static_cast<uint32_t>(TN) - 1u;
     constexpr int S = stage_bytes(TN) / static_cast<int>(sizeof(Pair<V, X>));  // pairs per pass
     constexpr int NP = TN / S;                                                 // passes per chunk
     static_assert(S * NP == TN, "staging must tile the chunk");
-    constexpr int PT = TN / kBlock;
+    constexpr int PT = TN / B;
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
     const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi;
@@ -115,7 +116,7 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
         V v[PT];
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
-            const uint32_t o = min(static_cast<uint32_t>(tid + q * kBlock), last);
+            const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
             id[q] = ib[o];
             v[q] = vb[o];
         }
@@ -133,7 +134,7 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
 #pragma unroll
             for (int q = 0; q < PT; ++q) {
                 const int d = G ? static_cast<int>(static_cast<uint32_t>(id[q]) & kMask)
-                                : static_cast<int>(min(static_cast<uint32_t>(tid + q * kBlock), last));
+                                : static_cast<int>(min(static_cast<uint32_t>(tid + q * B), last));
                 if (NP == 1 || d / S == p) lds[d - p * S] = Pair<V, X>{v[q], g[q]};
             }
             __syncthreads();
@@ -155,23 +156,23 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
     return sum;
 }
 
-template <int TN, bool G, typename V, typename X, typename A>
+template <int B, int TN, bool G, typename V, typename X, typename A>
 __device__ __forceinline__ A tile_dot_t(const Tile &tl, const int *__restrict__ ptr,
                                         const int *__restrict__ idx, const V *__restrict__ val,
                                         const X *__restrict__ xs, void *lds_raw) {
-    return tile_dot_rows<TN, G, V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
+    return tile_dot_rows<B, TN, G, V, X, A>(tile_rows(tl, ptr), idx, val, xs, lds_raw);
 }
 
-template <int TN, bool G, typename A>
+template <int B, int TN, bool G, typename A>
 __device__ __forceinline__ A tile_dot(const Tile &tl, const int *__restrict__ ptr,
                                       const int *__restrict__ idx, const float *__restrict__ val,
                                       const float *__restrict__ xs, void *lds) {
-    return tile_dot_t<TN, G, float, float, A>(tl, ptr, idx, val, xs, lds);
+    return tile_dot_t<B, TN, G, float, float, A>(tl, ptr, idx, val, xs, lds);
 }
 
 // Block-wide max of non-negative floats (NaN operands are dropped, as
 // fmaxf does) over NT threads; every lane receives the result.
-template <int NT = kBlock>
+template <int NT>
 __device__ __forceinline__ float block_max(float v, float *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -185,7 +186,7 @@ __device__ __forceinline__ float block_max(float v, float *red) {
     return r;
 }
 
-template <int NT = kBlock>
+template <int NT>
 __device__ __forceinline__ double block_max_d(double v, double *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -201,7 +202,7 @@ __device__ __forceinline__ double block_max_d(double v, double *red) {
 
 // Block-wide FP64 sum over NT threads in a fixed tree order; every lane
 // receives the result.
-template <int NT = kBlock>
+template <int NT>
 __device__ __forceinline__ double block_sum_d(double v, double *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -259,28 +260,29 @@ __device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
 }  // namespace dev
 
 // Launch-argument prefix of a tile kernel over a tile set (in launcher
-// bodies, with the stream in `s`).
-#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3(kBlock), 0, s, (ts).tiles, (ts).count, (ts).chunk
+// bodies, with the stream in `s`): one workgroup of ts.block threads per tile.
+#define MPDHG_TILE_GRID(ts) dim3((ts).grid), dim3((ts).block), 0, s, (ts).tiles, (ts).count, (ts).chunk
 
-// Launches KERNEL<tn, gather> for the tile set's chunk size and layout
-// (tn in 1024 .. kTileNnz; the last branch is kTileNnz itself).
-#define MPDHG_LAUNCH_TN1(ts, G, KERNEL, ...)                                                 \
-    do {                                                                                     \
-        if ((ts).tn == 1024)                                                                 \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<1024, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
-        else if ((ts).tn == 2048)                                                            \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<2048, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
-        else if (kTileNnz > 4096 && (ts).tn == 4096)                                         \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<4096, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
-        else                                                                                 \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<kTileNnz, G>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
+// Launches KERNEL<block, tn, gather> for the tile set's schedule and layout
+// (every pair of kSchedules).
+#define MPDHG_LAUNCH_G(ts, B_, TN_, KERNEL, ...)                                                   \
+    do {                                                                                           \
+        if ((ts).gather)                                                                           \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<B_, TN_, true>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
+        else                                                                                       \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<B_, TN_, false>), MPDHG_TILE_GRID(ts), __VA_ARGS__); \
     } while (0)
-#define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                     \
-    do {                                                                                     \
-        if ((ts).gather)                                                                     \
-            MPDHG_LAUNCH_TN1(ts, true, KERNEL, __VA_ARGS__);                                  \
-        else                                                                                 \
-            MPDHG_LAUNCH_TN1(ts, false, KERNEL, __VA_ARGS__);                                 \
+#define MPDHG_LAUNCH_TN(ts, KERNEL, ...)                                                           \
+    do {                                                                                           \
+        const int b_ = (ts).block, t_ = (ts).tn;                                                   \
+        if (b_ == 128 && t_ == 1024) MPDHG_LAUNCH_G(ts, 128, 1024, KERNEL, __VA_ARGS__);           \
+        else if (b_ == 128 && t_ == 2048) MPDHG_LAUNCH_G(ts, 128, 2048, KERNEL, __VA_ARGS__);      \
+        else if (b_ == 256 && t_ == 1024) MPDHG_LAUNCH_G(ts, 256, 1024, KERNEL, __VA_ARGS__);      \
+        else if (b_ == 256 && t_ == 2048) MPDHG_LAUNCH_G(ts, 256, 2048, KERNEL, __VA_ARGS__);      \
+        else if (b_ == 256 && t_ == 4096) MPDHG_LAUNCH_G(ts, 256, 4096, KERNEL, __VA_ARGS__);      \
+        else if (b_ == 512 && t_ == 4096) MPDHG_LAUNCH_G(ts, 512, 4096, KERNEL, __VA_ARGS__);      \
+        else MPDHG_LAUNCH_G(ts, 512, 8192, KERNEL, __VA_ARGS__);                                   \
     } while (0)
+static_assert(kNumSchedules == 7, "MPDHG_LAUNCH_TN dispatches every schedule");
 
 }  // namespace mpdhg

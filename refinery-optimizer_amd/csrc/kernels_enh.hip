@@ -20,7 +20,7 @@ __device__ __forceinline__ void halpern_weights(int it, int last_restart, float 
 }
 
 // ---- power iteration on A^T A (step size) ----------------------------------
-__global__ __launch_bounds__(kBlock) void k_pow_scale(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_pow_scale(const Tile *__restrict__ tiles, int ntiles,
                                                       int chunk, const EnhInst *__restrict__ st,
                                                       const float *__restrict__ z,
                                                       float *__restrict__ v) {
@@ -33,8 +33,8 @@ __global__ __launch_bounds__(kBlock) void k_pow_scale(const Tile *__restrict__ t
     if (j < tl.end) v[j] = z[j] * inv;
 }
 
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_pow_rows(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ rptr,
                                                      const int *__restrict__ ridx,
                                                      const float *__restrict__ rval,
@@ -44,13 +44,13 @@ __global__ __launch_bounds__(kBlock) void k_pow_rows(const Tile *__restrict__ ti
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<TN, G, float>(tl, rptr, ridx, rval, v, lds);
+    const float s = tile_dot<B, TN, G, float>(tl, rptr, ridx, rval, v, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) w[i] = s;
 }
 
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_pow_cols(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const int *__restrict__ cptr,
                                                      const int *__restrict__ cidx,
                                                      const float *__restrict__ cval,
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_pow_cols(const Tile *__restrict__ ti
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<TN, G, float>(tl, cptr, cidx, cval, w, lds);
+    const float s = tile_dot<B, TN, G, float>(tl, cptr, cidx, cval, w, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q = 0.0;
     if (j < tl.end) {
@@ -98,8 +98,8 @@ struct PrimalArgs {
 
 // Iteration it = *A.itb + off: the iteration number lives on the device, so
 // one recorded check cycle (a HIP graph) replays for every cycle.
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
                                                        int chunk, const EnhInst *__restrict__ st,
                                                        PrimalArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_primal(const Tile *__restrict__ 
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot_rows<TN, G, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    const float g = tile_dot_rows<B, TN, G, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -173,8 +173,8 @@ struct DualArgs {
     float rho;
 };
 
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
                                                      int chunk, const EnhInst *__restrict__ st,
                                                      DualArgs A, int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_dual(const Tile *__restrict__ ti
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<TN, G, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<B, TN, G, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -244,8 +244,8 @@ struct CheckRowArgs {
     int grn;
 };
 
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_enh_chk_rows(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckRowArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_rows(const Tile *__restrict_
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const double axi = tile_dot_t<TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
+    const double axi = tile_dot_t<B, TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double viol = 0.0, sviol = 0.0, dobj = 0.0;
     if (i < tl.end) {
@@ -284,8 +284,8 @@ struct CheckColArgs {
     int gcn;
 };
 
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict__ tiles, int ntiles,
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_enh_chk_cols(const Tile *__restrict__ tiles, int ntiles,
                                                          int chunk, const EnhInst *__restrict__ st,
                                                          CheckColArgs A) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_chk_cols(const Tile *__restrict_
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const double aty = tile_dot_t<TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
+    const double aty = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
     if (j < tl.end) {
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict
 }
 
 // ---- best-point copies and residual-LP construction -----------------------
-__global__ __launch_bounds__(kBlock) void k_enh_best_cols(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_enh_best_cols(const Tile *__restrict__ tiles, int ntiles,
                                                           int chunk, const EnhInst *__restrict__ st,
                                                           const double *__restrict__ xchk,
                                                           const double *__restrict__ lam,
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_enh_best_cols(const Tile *__restrict
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_enh_best_rows(const Tile *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(kBlockMax) void k_enh_best_rows(const Tile *__restrict__ tiles, int ntiles,
                                                           int chunk, const EnhInst *__restrict__ st,
                                                           const double *__restrict__ ychk,
                                                           const double *__restrict__ ax,
@@ -494,7 +494,7 @@ struct RefineColArgs {
 // Residual LP around this check's point (x~, y~): costs alpha_d * lambda~,
 // bounds alpha_p * ([l, u] - x~), row bounds alpha_p * ([rl, ru] - A x~), dual
 // offset alpha_d * y~, all in the scaled space; the correction restarts at 0.
-__global__ __launch_bounds__(kBlock) void k_enh_refine_cols(const Tile *__restrict__ tiles,
+__global__ __launch_bounds__(kBlockMax) void k_enh_refine_cols(const Tile *__restrict__ tiles,
                                                             int ntiles, int chunk,
                                                             const EnhInst *__restrict__ st,
                                                             RefineColArgs A) {
@@ -527,7 +527,7 @@ struct RefineRowArgs {
     float *rl, *ru, *ybar, *y, *y0, *yp;
 };
 
-__global__ __launch_bounds__(kBlock) void k_enh_refine_rows(const Tile *__restrict__ tiles,
+__global__ __launch_bounds__(kBlockMax) void k_enh_refine_rows(const Tile *__restrict__ tiles,
                                                             int ntiles, int chunk,
                                                             const EnhInst *__restrict__ st,
                                                             RefineRowArgs A) {
@@ -619,8 +619,8 @@ void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *s
 
 void tile_run(hipStream_t s, const TileSet &ts, const EnhInst *st) {
     if (ts.count)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<EnhInst>), dim3((ts.count + kBlock - 1) / kBlock),
-                           dim3(kBlock), 0, s, ts.tiles, ts.count, st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<EnhInst>), dim3((ts.count + 255) / 256),
+                           dim3(256), 0, s, ts.tiles, ts.count, st);
 }
 
 

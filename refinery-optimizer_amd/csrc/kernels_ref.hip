@@ -15,8 +15,8 @@ namespace mpdhg {
 namespace dev {
 
 // cuda_pdhg.cu:33-49 (A^T y) fused with :54-77 (projected primal step).
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_ref_primal(
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_ref_primal(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
     const float *__restrict__ y, const float *__restrict__ c, const float *__restrict__ l,
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<B, TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) {
         const float xo = x[j];
@@ -42,8 +42,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_primal(
 }
 
 // cuda_pdhg.cu:14-30 (A xbar) fused with :82-103 (Moreau dual step).
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_ref_dual(
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_ref_dual(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
     const float *__restrict__ xbar, const float *__restrict__ rl, const float *__restrict__ ru,
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const float h = tile_dot<TN, G, float>(tl, rptr, ridx, rval, xbar, lds);
+    const float h = tile_dot<B, TN, G, float>(tl, rptr, ridx, rval, xbar, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) {
         const float sg = sigma[i];
@@ -68,8 +68,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_dual(
 
 // cuda_pdhg.cu:114-129 row residuals of A x, max-reduced per tile (the
 // reference copies m floats to the host and reduces there, :249-255).
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_ref_rowres(
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_ref_rowres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
     const float *__restrict__ x, const float *__restrict__ rl, const float *__restrict__ ru,
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_rowres(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const float ax = tile_dot<TN, G, float>(tl, rptr, ridx, rval, x, lds);
+    const float ax = tile_dot<B, TN, G, float>(tl, rptr, ridx, rval, x, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (i < tl.end) {
@@ -89,13 +89,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_rowres(
         if (lo > -1e15f && ax < lo) v = fmaxf(v, lo - ax);
         if (hi < 1e15f && ax > hi) v = fmaxf(v, ax - hi);
     }
-    v = block_max(v, red);
+    v = block_max<B>(v, red);
     if (threadIdx.x == 0) part[t] = v;
 }
 
 // cuda_pdhg.cu:131-151 column fixed-point residuals from A^T y, max-reduced per tile.
-template <int TN, bool G>
-__global__ __launch_bounds__(kBlock) void k_ref_colres(
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_ref_colres(
     const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
     const int *__restrict__ cptr, const int *__restrict__ cidx, const float *__restrict__ cval,
     const float *__restrict__ y, const float *__restrict__ x, const float *__restrict__ c,
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_colres(
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const float g = tile_dot<TN, G, float>(tl, cptr, cidx, cval, y, lds);
+    const float g = tile_dot<B, TN, G, float>(tl, cptr, cidx, cval, y, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     float v = 0.0f;
     if (j < tl.end) {
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_colres(
         if (proj > hi) proj = hi;
         v = fabsf(xj - proj) / fmaxf(1e-4f, step);
     }
-    v = block_max(v, red);
+    v = block_max<B>(v, red);
     if (threadIdx.x == 0) part[t] = v;
 }
 
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kInstBlock) void k_ref_finalize(RefInst *__restrict
 }
 
 // gpu_solver.cpp:261-264 best-x copy, device-side, for flagged instances.
-__global__ __launch_bounds__(kBlock) void k_ref_copy_best(const Tile *__restrict__ tiles,
+__global__ __launch_bounds__(kBlockMax) void k_ref_copy_best(const Tile *__restrict__ tiles,
                                                           int ntiles, int chunk,
                                                           const RefInst *__restrict__ st,
                                                           const float *__restrict__ x,
@@ -218,14 +218,14 @@ void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, cons
 
 void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const float *x,
                    float *xbest) {
-    hipLaunchKernelGGL(dev::k_ref_copy_best, dim3(ts.grid), dim3(kBlock), 0, s, ts.tiles,
+    hipLaunchKernelGGL(dev::k_ref_copy_best, dim3(ts.grid), dim3(ts.block), 0, s, ts.tiles,
                        ts.count, ts.chunk, st, x, xbest);
 }
 
 void tile_run(hipStream_t s, const TileSet &ts, const RefInst *st) {
     if (ts.count)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<RefInst>), dim3((ts.count + kBlock - 1) / kBlock),
-                           dim3(kBlock), 0, s, ts.tiles, ts.count, st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(dev::k_tile_run<RefInst>), dim3((ts.count + 255) / 256),
+                           dim3(256), 0, s, ts.tiles, ts.count, st);
 }
 
 }  // namespace launch

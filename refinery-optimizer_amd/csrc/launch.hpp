@@ -13,7 +13,8 @@ struct TileSet {
     int count = 0;
     int chunk = 0;        // ceil(count / 8)
     int grid = 0;         // 8 * chunk
-    int tn = kTileNnz;    // nonzeros per LDS chunk (1024, 2048 or 4096)
+    int block = 256;      // threads per workgroup (rows per tile at most)
+    int tn = 4096;        // nonzeros per LDS chunk; (block, tn) is one of kSchedules
     bool gather = false;  // every tile's chunks in gather order (Tile::base >= 0)
 };
 

@@ -261,6 +261,10 @@ class Solution:
     bytes_csc_launch: float = 0.0
     step_size: float = 0.0
     primal_weight: float = 0.0
+    csr_block: int = 0  # tile schedule used for A x (threads per workgroup, LDS chunk)
+    csr_chunk: int = 0
+    csc_block: int = 0  # and for A^T y
+    csc_chunk: int = 0
 
     def is_optimal(self) -> bool:
         return self.status == SolutionStatus.Optimal
@@ -293,8 +297,9 @@ class GpuPdhgConfig:
     dual_sign: int = 0
     refine_ratio: float = 1e-2
     spmv_layout: int = 0  # 0: gather order where it applies, 1: storage order
-    spmv_chunk: int = 0  # nonzeros per LDS pass: 0 = auto, or 1024 / 2048 / 4096
+    spmv_chunk: int = 0  # nonzeros per LDS pass: 0 = auto, or 1024 / 2048 / 4096 / 8192
     launch_graph: int = 1  # enhanced: replay each check cycle as a recorded HIP graph
+    spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
@@ -333,7 +338,8 @@ def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
     for name in ("restarts", "refinements", "checks", "rel_primal", "rel_dual", "rel_gap",
                  "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
                  "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
-                 "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight"):
+                 "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight",
+                 "csr_block", "csr_chunk", "csc_block", "csc_chunk"):
         setattr(s, name, getattr(r, name))
     return s
 

@@ -38,6 +38,7 @@ class MpdhgConfig(C.Structure):
         ("spmv_layout", C.c_int32),
         ("spmv_chunk", C.c_int32),
         ("launch_graph", C.c_int32),
+        ("spmv_block", C.c_int32),
     ]
 
 
@@ -86,6 +87,10 @@ class MpdhgResult(C.Structure):
         ("bytes_csc_launch", C.c_double),
         ("step_size", C.c_double),
         ("primal_weight", C.c_double),
+        ("csr_block", C.c_int32),
+        ("csr_chunk", C.c_int32),
+        ("csc_block", C.c_int32),
+        ("csc_chunk", C.c_int32),
     ]
 
 

@@ -103,13 +103,19 @@ def test_config_defaults_are_the_reference_defaults():
 
 
 @pytest.mark.parametrize("field,value", [("spmv_chunk", 3000), ("spmv_layout", 2), ("mode", 7),
-                                         ("check_frequency", 0), ("launch_graph", 2)])
+                                         ("check_frequency", 0), ("launch_graph", 2), ("spmv_block", 384),
+                                         ("spmv_block", (128, 4096)), ("spmv_block", (512, 1024))])
 def test_invalid_configuration_is_rejected(field, value):
     # mpdhg_solve with a NULL context validates the configuration before it
     # creates a temporary context, so this holds with or without a device.
+    # A (block, chunk) value sets both: each is valid alone, the pair is no
+    # compiled tile schedule.
     p = S.gen_random(20, 50, 4, 1)
     c = S.GpuPdhgConfig()
-    setattr(c, field, value)
+    if isinstance(value, tuple):
+        c.spmv_block, c.spmv_chunk = value
+    else:
+        setattr(c, field, value)
     lp, keep = p._c_view()
     res = _abi.MpdhgResult()
     rc = _abi.lib().mpdhg_solve(None, C.byref(lp), C.byref(c._c()), C.byref(res))

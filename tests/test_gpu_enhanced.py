@@ -113,19 +113,23 @@ def test_iteration_limit_and_short_runs(need_gpu):
     assert S.solve_gpu_pdhg(e, enh()).status == S.SolutionStatus.Optimal
 
 
-@pytest.mark.parametrize("layout,chunk", [(1, 0), (0, 1024), (0, 2048), (1, 4096)])
-def test_spmv_schedules_are_bit_identical(need_gpu, layout, chunk):
-    # The tile schedule (layout, chunk size) changes how rows are grouped
-    # into workgroups, never a result: row sums keep storage order, and the
-    # FP64 sums behind step size, restarts and stopping are reduced per
-    # instance in a fixed order.  Same iterates, decisions and KKT bits.
+@pytest.mark.parametrize("layout,block,chunk", [(1, 0, 0), (0, 128, 1024), (0, 128, 2048), (0, 256, 1024),
+                                                (0, 256, 2048), (1, 256, 4096), (0, 512, 4096), (0, 512, 8192),
+                                                (1, 512, 8192)])
+def test_spmv_schedules_are_bit_identical(need_gpu, layout, block, chunk):
+    # The tile schedule (layout, workgroup size, chunk size) changes how rows
+    # are grouped into workgroups, never a result: row sums keep storage
+    # order, and the FP64 sums behind step size, restarts and stopping are
+    # formed per 16-row group and per instance in a fixed order.  Same
+    # iterates, decisions and KKT bits.
     probs = [S.gen_random(300, 720, 8, 3), S.gen_staircase(2, 40, 90, 4), random_lp(60, 90, 0.1, 2)[0],
              toy01(), S.gen_random(2000, 4800, 8, 9)]
     a = S.solve_batch(probs, enh(tol=1e-6))
     cfg = enh(tol=1e-6)
-    cfg.spmv_layout = layout
-    cfg.spmv_chunk = chunk
+    cfg.spmv_layout, cfg.spmv_block, cfg.spmv_chunk = layout, block, chunk
     b = S.solve_batch(probs, cfg)
+    if block:
+        assert (b[0].csr_block, b[0].csr_chunk, b[0].csc_block, b[0].csc_chunk) == (block, chunk, block, chunk)
     for p, s, t in zip(probs, a, b):
         assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
         assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name

@@ -129,17 +129,22 @@ def test_dual_sign_option_and_kkt_fields(need_gpu):
     np.testing.assert_allclose(a.slack, p.A.mat_vec(a.x), rtol=1e-12, atol=1e-12)
 
 
-def test_layouts_and_chunk_sizes_are_bit_identical(need_gpu):
+SCHEDULES = [(128, 1024), (128, 2048), (256, 1024), (256, 2048), (256, 4096), (512, 4096), (512, 8192)]
+
+
+def test_layouts_and_schedules_are_bit_identical(need_gpu):
     # spmv_layout 0 stores each chunk sorted by gathered index, 1 keeps
-    # storage order; spmv_chunk sets the nonzeros staged per workgroup pass
-    # (and so the tiling).  The per-row summation order is the same in every
-    # case and the reference check reduces with max, so every result equals
-    # the oracle's bit for bit.
+    # storage order; (spmv_block, spmv_chunk) sets the rows per workgroup and
+    # the nonzeros staged per pass (and so the tiling).  The per-row summation
+    # order is the same in every case and the reference check reduces with
+    # max, so every result equals the oracle's bit for bit.
     probs = _mixed_batch()
     c = cfg(max_iterations=300, check_frequency=25)
     refs = [oracle_of(p, c) for p in probs]
     for layout in (0, 1):
-        for chunk in (0, 1024, 2048, 4096):
-            c.spmv_layout, c.spmv_chunk = layout, chunk
+        for block, chunk in [(0, 0)] + SCHEDULES:
+            c.spmv_layout, c.spmv_block, c.spmv_chunk = layout, block, chunk
             for p, s, r in zip(probs, S.solve_batch(probs, c), refs):
-                assert_bitexact(s, r, f"{p.name} layout={layout} chunk={chunk}")
+                assert_bitexact(s, r, f"{p.name} layout={layout} block={block} chunk={chunk}")
+                if block:
+                    assert (s.csr_block, s.csr_chunk, s.csc_block, s.csc_chunk) == (block, chunk, block, chunk)
