@@ -83,7 +83,11 @@ def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=True):
              "frac": g / HBM_PEAK_GBS, "traffic": measured_traffic(name) if traffic else None,
              "bytes_per_launch": b, "us_per_launch": us},
             {"csc_fused_gbs": gbs_csc, "csr_fused_gbs": gbs_csr, "csc_fused_us": csc_us,
-             "csr_fused_us": csr_us})
+             "csr_fused_us": csr_us,
+             # tile schedules (threads per workgroup, nonzeros per LDS chunk) the solve used,
+             # and the time the schedule search took
+             "csr_schedule": [sol.csr_block, sol.csr_chunk], "csc_schedule": [sol.csc_block, sol.csc_chunk],
+             "schedule_search_ms": sol.tune_ms})
 
 
 def cpu_port_sample(probs, tol, threads):

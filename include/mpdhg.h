@@ -94,6 +94,12 @@ typedef struct mpdhg_config {
                                      bit-identical for every choice (rows are summed in storage
                                      order, and every FP64 sum behind a decision is reduced per
                                      16-row group and then per instance in a fixed order). */
+    int32_t spmv_tune;            /* enhanced and reference modes: 1 (default) = time every
+                                     compiled schedule allowed by spmv_block / spmv_chunk on this
+                                     batch's own matrix, per direction, and keep the fastest,
+                                     when the batch holds >= 262144 nonzeros (smaller batches use
+                                     the rule); 2 = always; 0 = never (the rule: the largest
+                                     workgroup that still gives >= 1024 tiles). */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite
@@ -136,6 +142,7 @@ typedef struct mpdhg_result {
     double primal_weight;    /* enhanced: final omega */
     int32_t csr_block, csr_chunk;  /* tile schedule the solve used for A x (CSR) ... */
     int32_t csc_block, csc_chunk;  /* ... and for A^T y (CSC) */
+    double tune_ms;                /* host + device time of the schedule search (0: rule) */
 } mpdhg_result;
 
 typedef struct mpdhg_ctx mpdhg_ctx;

@@ -56,6 +56,8 @@ int check_config(const mpdhg_config *cfg) {
     if (cfg->spmv_block != 0 && cfg->spmv_chunk != 0 &&
         !mpdhg::valid_schedule(cfg->spmv_block, cfg->spmv_chunk))
         return mpdhg::fail(MPDHG_E_ARG, "(spmv_block, spmv_chunk) is not a compiled tile schedule");
+    if (cfg->spmv_tune < 0 || cfg->spmv_tune > 2)
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_tune must be 0, 1 or 2");
     if (cfg->launch_graph != 0 && cfg->launch_graph != 1)
         return mpdhg::fail(MPDHG_E_ARG, "launch_graph must be 0 or 1");
     return MPDHG_OK;
@@ -120,6 +122,7 @@ void mpdhg_config_default(mpdhg_config *cfg) {
     cfg->spmv_chunk = 0;
     cfg->launch_graph = 1;
     cfg->spmv_block = 0;
+    cfg->spmv_tune = 1;
 }
 
 int mpdhg_device_count(int *count) {

@@ -228,7 +228,6 @@ namespace {
 struct RefHost {
     std::vector<int> cptr, cidx, rptr, ridx;
     std::vector<float> cval, rval, c, l, u, tau, rl, ru, sigma, x0;
-    std::vector<Tile> rtiles, ctiles;
     std::vector<RefInst> inst;
 };
 
@@ -273,8 +272,6 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const Schedule rsc = choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk);
-    const Schedule csc = choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk);
     hh.cptr.resize(C + 1);
     hh.cidx.resize(Z);
     hh.rptr.resize(R + 1);
@@ -335,44 +332,49 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         s.best_res = 1e30f;
         s.max_abs_c = mac;
         s.max_abs_b = mab;
-        s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, ro, zo, L.rgrp_off[static_cast<size_t>(a)], a, rsc.block, rsc.tn, hh.rtiles);
-        s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
-        s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, co, zo, L.cgrp_off[static_cast<size_t>(a)], a, csc.block, csc.tn, hh.ctiles);
-        s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
-    }
-    bool rgo = false, cgo = false;
-    {
-        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
-        for (int a = 0; a < na; ++a) {
-            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-            rlen[static_cast<size_t>(a)] = h.m;
-            clen[static_cast<size_t>(a)] = h.n;
-        }
-        const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        rgo = gather_order(lay, rsc.tn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval);
-        cgo = gather_order(lay, csc.tn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval);
     }
     // ---- device buffers and upload (gpu_solver.cpp:153-216) ----
     hipStream_t st = ctx.stream;
-    DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
-    DBuf<float> d_cval, d_rval, d_c, d_l, d_u, d_tau, d_rl, d_ru, d_sigma;
+    DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s;
+    DBuf<float> d_cval_s, d_rval_s, d_c, d_l, d_u, d_tau, d_rl, d_ru, d_sigma;
     DBuf<float> d_x(C), d_xbar(C), d_xbest(C), d_y(R);
-    DBuf<Tile> d_rt, d_ct;
     DBuf<RefInst> d_inst;
-    d_cptr.upload(hh.cptr, st); d_cidx.upload(hh.cidx, st); d_cval.upload(hh.cval, st);
-    d_rptr.upload(hh.rptr, st); d_ridx.upload(hh.ridx, st); d_rval.upload(hh.rval, st);
+    d_cptr.upload(hh.cptr, st); d_cidx_s.upload(hh.cidx, st); d_cval_s.upload(hh.cval, st);
+    d_rptr.upload(hh.rptr, st); d_ridx_s.upload(hh.ridx, st); d_rval_s.upload(hh.rval, st);
     d_c.upload(hh.c, st); d_l.upload(hh.l, st); d_u.upload(hh.u, st); d_tau.upload(hh.tau, st);
     d_rl.upload(hh.rl, st); d_ru.upload(hh.ru, st); d_sigma.upload(hh.sigma, st);
-    d_rt.upload(hh.rtiles, st); d_ct.upload(hh.ctiles, st); d_inst.upload(hh.inst, st);
+    // ---- tile schedules: searched on this batch's own matrix (the iterate
+    // buffers serve as scratch before they are initialised), or by rule ----
+    const Layout lay = static_cast<Layout>(cfg.spmv_layout);
+    const DirPlan RP = dir_plan(H, act, L, true), CP = dir_plan(H, act, L, false);
+    const DirSource RS{d_ridx_s.get(), d_rval_s.get(), nullptr}, CS{d_cidx_s.get(), d_cval_s.get(), nullptr};
+    DevDir RD, CD;
+    double tune_ms = 0.0;
+    if (tune_wanted(cfg, L.nnz)) {
+        std::vector<TuneTrial> trials;
+        DBuf<double> d_gtmp(static_cast<size_t>(L.cgroups));
+        tune_ms += tune_dir(RP, true, lay, RS, d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, false,
+                            d_x.get(), d_y.get(), nullptr, st, RD, trials);
+        tune_ms += tune_dir(CP, false, lay, CS, d_cptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, false,
+                            d_y.get(), d_x.get(), d_gtmp.get(), st, CD, trials);
+    } else {
+        build_dir(RP, choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk), lay, RS, true, false, st, RD);
+        build_dir(CP, choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk), lay, CS, true, false, st, CD);
+    }
+    for (int a = 0; a < na; ++a) {
+        RefInst &s = hh.inst[static_cast<size_t>(a)];
+        s.row_tile_begin = RD.first[static_cast<size_t>(a)];
+        s.row_tile_end = RD.first[static_cast<size_t>(a) + 1];
+        s.col_tile_begin = CD.first[static_cast<size_t>(a)];
+        s.col_tile_end = CD.first[static_cast<size_t>(a) + 1];
+    }
+    d_inst.upload(hh.inst, st);
     MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
-    DBuf<float> d_prow(hh.rtiles.size()), d_pcol(hh.ctiles.size());
-    const TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rsc, rgo);
-    const TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), csc, cgo);
+    DBuf<float> d_prow(RD.host.size()), d_pcol(CD.host.size());
+    const TileSet rts = RD.set(), cts = CD.set();
     MPDHG_HIP(hipStreamSynchronize(st));
 
     // ---- profiling samples: HIP events around the fused SpMV launches ----
@@ -401,20 +403,20 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     for (iter = 1; iter <= K; ++iter) {
         const bool samp = nsamp && (iter % stride == 0) && used < nsamp;
         if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
-        launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), d_cidx.get(), d_cval.get(),
+        launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32,
                            d_y.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(), d_x.get(),
                            d_xbar.get());
         if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
-        launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), d_ridx.get(), d_rval.get(),
+        launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32,
                          d_xbar.get(), d_rl.get(), d_ru.get(), d_sigma.get(), d_y.get());
         if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
         if (samp) ++used;
         // (:237-239 restart kernel omitted: it writes xbar = x, and the next
         //  primal update overwrites xbar before anything reads it.)
         if (iter % F == 0 || iter == K) {
-            launch::ref_rowres(st, rts, d_inst.get(), d_rptr.get(), d_ridx.get(), d_rval.get(),
+            launch::ref_rowres(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32,
                                d_x.get(), d_rl.get(), d_ru.get(), d_prow.get());
-            launch::ref_colres(st, cts, d_inst.get(), d_cptr.get(), d_cidx.get(), d_cval.get(),
+            launch::ref_colres(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32,
                                d_y.get(), d_x.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(),
                                d_pcol.get());
             launch::ref_finalize(st, d_inst.get(), na, d_prow.get(), d_pcol.get(), iter, K,
@@ -493,10 +495,11 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     }
     for (int a = 0; a < na; ++a) {
         mpdhg_result &r = res[act[static_cast<size_t>(a)]];
-        r.csr_block = rsc.block;
-        r.csr_chunk = rsc.tn;
-        r.csc_block = csc.block;
-        r.csc_chunk = csc.tn;
+        r.csr_block = RD.sc.block;
+        r.csr_chunk = RD.sc.tn;
+        r.csc_block = CD.sc.block;
+        r.csc_chunk = CD.sc.tn;
+        r.tune_ms = tune_ms;
     }
     const double wall = now_sec() - t_call;
     for (int q = 0; q < count; ++q) res[q].time_wall_sec = wall;

@@ -140,62 +140,75 @@ void parallel_for_each(int count, const std::function<void(int)> &f);
 // SpMV layout choice for a solve (mpdhg_config::spmv_layout).
 enum class Layout { Auto = 0, Storage = 1 };
 
-namespace detail {
-template <typename V>
-void permute_chunk(std::vector<V> &vals, int c0, const std::vector<uint32_t> &key, uint32_t mask,
-                   std::vector<V> &tmp) {
-    const int nc = static_cast<int>(key.size());
-    tmp.assign(vals.begin() + c0, vals.begin() + c0 + nc);
-    for (int q = 0; q < nc; ++q) vals[static_cast<size_t>(c0 + q)] = tmp[key[static_cast<size_t>(q)] & mask];
-}
-}  // namespace detail
+// One direction of a batch (CSR rows for A x, or CSC columns for A^T y) as
+// the tile planner sees it: per instance its local row pointers and row
+// count, and its offsets in the batch (first row, first nonzero, first row
+// group, and the offset and length of the vector its rows gather from).
+struct DirPlan {
+    std::vector<const std::vector<int> *> ptr;
+    std::vector<int> count;
+    std::vector<int64_t> row_off, nnz_off, grp_off, vec_off, vec_len;
+    int64_t rows = 0, nnz = 0;
+};
+DirPlan dir_plan(const std::vector<HostLp> &H, const std::vector<int> &act, const BatchLayout &L,
+                 bool csr);
 
-// Puts every chunk (tn nonzeros) of every tile of a tile set into gather
-// order (see Tile) when the layout is Auto and every instance gathers from a
-// vector shorter than gather_local_limit(tn): the global indices in `idx`
-// (storage order) become packed (local << log2 tn | position) words sorted
-// ascending, each value array is permuted alike, and Tile::base is set to
-// the vector's offset.  Otherwise the whole set keeps storage order
-// (base = -1).  Returns whether the set is in gather order.  vec_off /
-// vec_len: offset and length of each instance's gathered vector.
-template <typename... Vs>
-bool gather_order(Layout layout, int tn, const std::vector<int> &ptr, std::vector<Tile> &tiles,
-                  const std::vector<int64_t> &vec_off, const std::vector<int64_t> &vec_len,
-                  std::vector<int> &idx, std::vector<Vs> &...vals) {
-    bool all = (layout == Layout::Auto);
-    for (int64_t len : vec_len) all = all && (len < gather_local_limit(tn));
-    if (!all) {
-        for (Tile &tl : tiles) tl.base = -1;
-        return false;
-    }
-    constexpr int kGroup = 64;  // tiles per work item
-    const int bits = ilog2(tn);
-    const uint32_t mask = static_cast<uint32_t>(tn) - 1u;
-    const int nt = static_cast<int>(tiles.size());
-    parallel_for_each((nt + kGroup - 1) / kGroup, [&](int g) {
-        std::vector<uint32_t> key;
-        std::tuple<std::vector<Vs>...> tmp;
-        for (int t = g * kGroup; t < std::min(nt, (g + 1) * kGroup); ++t) {
-            Tile &tl = tiles[static_cast<size_t>(t)];
-            const size_t a = static_cast<size_t>(tl.inst);
-            const int64_t off = vec_off[a];
-            tl.base = static_cast<int32_t>(off);
-            const int lo = ptr[static_cast<size_t>(tl.begin)], hi = ptr[static_cast<size_t>(tl.end)];
-            for (int c0 = lo; c0 < hi; c0 += tn) {
-                const int nc = std::min(hi, c0 + tn) - c0;
-                key.resize(static_cast<size_t>(nc));
-                for (int q = 0; q < nc; ++q)
-                    key[static_cast<size_t>(q)] =
-                        (static_cast<uint32_t>(idx[static_cast<size_t>(c0 + q)] - off) << bits) |
-                        static_cast<uint32_t>(q);
-                std::sort(key.begin(), key.end());
-                for (int q = 0; q < nc; ++q) idx[static_cast<size_t>(c0 + q)] = static_cast<int>(key[static_cast<size_t>(q)]);
-                std::apply([&](auto &...tv) { (detail::permute_chunk(vals, c0, key, mask, tv), ...); }, tmp);
-            }
-        }
-    });
-    return true;
-}
+// The matrix of one direction on the device in storage order: gathered
+// (global) index per nonzero and the value copies the solve uses.
+struct DirSource {
+    const int *idx = nullptr;
+    const float *v32 = nullptr;
+    const double *v64 = nullptr;
+};
+
+// A direction laid out for one schedule: tile records (also kept on the
+// host, for compaction) and the matrix copy the tile kernels read.  In
+// gather order the copy is owned here (built on the device from the
+// source); in storage order the source arrays are used as they are.
+struct DevDir {
+    Schedule sc{0, 0};
+    bool gather = false;
+    std::vector<Tile> host;
+    std::vector<int32_t> first;  // tiles [first[a], first[a + 1]) belong to instance a
+    DBuf<Tile> tiles;
+    DBuf<int> own_idx;
+    DBuf<float> own32;
+    DBuf<double> own64;
+    const int *idx = nullptr;
+    const float *v32 = nullptr;
+    const double *v64 = nullptr;
+    TileSet set() const { return make_tileset(tiles, static_cast<int>(host.size()), sc, gather); }
+};
+
+// Plans the tiles of schedule sc and lays the source out for them: gather
+// order (each chunk sorted on the device by gathered index, packed index
+// words, values permuted alike) when `layout` is Auto and every instance's
+// vector is shorter than gather_local_limit(sc.tn), else storage order.
+// want32 / want64: which value copies the solve reads.
+void build_dir(const DirPlan &P, Schedule sc, Layout layout, const DirSource &src, bool want32,
+               bool want64, hipStream_t st, DevDir &out);
+
+// Schedule search of one direction: builds every compiled schedule allowed
+// by forced_block / forced_tn and times `reps` launches of the FP32 SpMV of
+// this direction (y = A v for CSR, z = A^T w for CSC) on each, keeping the
+// fastest in `out` (with the value copies of want32 / want64).  ptr: the
+// direction's global row pointers on the device; vin / vout / gpart:
+// scratch of the gathered length, the row count and the row-group count.
+// Returns the milliseconds spent; `trials` receives (schedule, microseconds
+// per launch) for each candidate.
+struct TuneTrial {
+    Schedule sc;
+    bool gather;
+    double us;
+};
+// Whether a solve searches its schedules (mpdhg_config::spmv_tune: 0 never,
+// 1 when the batch holds at least kTuneMinNnz nonzeros, 2 always; never
+// when both spmv_block and spmv_chunk are fixed).
+constexpr int64_t kTuneMinNnz = 256 * 1024;
+bool tune_wanted(const mpdhg_config &cfg, int64_t nnz);
+double tune_dir(const DirPlan &P, bool csr, Layout layout, const DirSource &src, const int *ptr,
+                int forced_block, int forced_tn, bool want32, bool want64, float *vin, float *vout,
+                double *gpart, hipStream_t st, DevDir &out, std::vector<TuneTrial> &trials);
 
 // FP64 post-processing on the host for reference-mode results: A x,
 // c + A^T y and the relative KKT of (x, y).

@@ -35,8 +35,6 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const Schedule rsc = choose_schedule(L.nnz, L.rows, 0, 0);
-    const Schedule csc = choose_schedule(L.nnz, L.cols, 0, 0);
     std::vector<int> cptr(C + 1), cidx(Z), rptr(R + 1), ridx(Z);
     std::vector<double> cval(Z), rval(Z), dc(C), dr(R), mb(C), mr(R), r0(C), rt0(C);
     std::vector<AdjInst> inst(static_cast<size_t>(na));
@@ -100,7 +98,6 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     });
     cptr[C] = static_cast<int>(Z);
     rptr[R] = static_cast<int>(Z);
-    std::vector<Tile> rtiles, ctiles;
     for (int a = 0; a < na; ++a) {
         const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
         AdjInst &s = inst[static_cast<size_t>(a)];
@@ -109,40 +106,32 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
         s.rgrp_end = static_cast<int32_t>(L.rgrp_off[ua] + (h.m + kGroup - 1) / kGroup);
         s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
-        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rsc.block, rsc.tn, rtiles);
-        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, csc.block, csc.tn, ctiles);
-    }
-    bool rgo = false, cgo = false;
-    {
-        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
-        for (int a = 0; a < na; ++a) {
-            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-            rlen[static_cast<size_t>(a)] = h.m;
-            clen[static_cast<size_t>(a)] = h.n;
-        }
-        rgo = gather_order(Layout::Auto, rsc.tn, rptr, rtiles, L.col_off, clen, ridx, rval);
-        cgo = gather_order(Layout::Auto, csc.tn, cptr, ctiles, L.row_off, rlen, cidx, cval);
     }
     hipStream_t st = ctx.stream;
-    DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
-    DBuf<double> d_cval, d_rval, d_dc, d_dr, d_mb, d_mr, d_r, d_rt;
+    DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s;
+    DBuf<double> d_cval_s, d_rval_s, d_dc, d_dr, d_mb, d_mr, d_r, d_rt;
     DBuf<double> d_q(C), d_s(R), d_mu(R), d_p(R), d_pt(R);
-    DBuf<Tile> d_rt_tiles, d_ct_tiles;
     DBuf<AdjInst> d_inst;
-    d_cptr.upload(cptr, st); d_cidx.upload(cidx, st); d_rptr.upload(rptr, st); d_ridx.upload(ridx, st);
-    d_cval.upload(cval, st); d_rval.upload(rval, st); d_dc.upload(dc, st); d_dr.upload(dr, st);
+    d_cptr.upload(cptr, st); d_cidx_s.upload(cidx, st); d_rptr.upload(rptr, st); d_ridx_s.upload(ridx, st);
+    d_cval_s.upload(cval, st); d_rval_s.upload(rval, st); d_dc.upload(dc, st); d_dr.upload(dr, st);
     d_mb.upload(mb, st); d_mr.upload(mr, st); d_r.upload(r0, st); d_rt.upload(rt0, st);
-    d_rt_tiles.upload(rtiles, st); d_ct_tiles.upload(ctiles, st); d_inst.upload(inst, st);
+    d_inst.upload(inst, st);
+    // tile schedules by rule (the CGLS kernels stream FP64 values)
+    const DirPlan RP = dir_plan(H, act, L, true), CP = dir_plan(H, act, L, false);
+    DevDir RD, CD;
+    build_dir(RP, choose_schedule(L.nnz, L.rows, 0, 0), Layout::Auto, DirSource{d_ridx_s.get(), nullptr, d_rval_s.get()},
+              false, true, st, RD);
+    build_dir(CP, choose_schedule(L.nnz, L.cols, 0, 0), Layout::Auto, DirSource{d_cidx_s.get(), nullptr, d_cval_s.get()},
+              false, true, st, CD);
     for (double *p : {d_s.get(), d_mu.get(), d_p.get(), d_pt.get()})
         MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(double), st));
     MPDHG_HIP(hipMemsetAsync(d_q.get(), 0, C * sizeof(double), st));
     // per-group partials of |q|^2 (columns) and |s|^2 (rows)
     DBuf<double> d_gq(static_cast<size_t>(L.cgroups)), d_gs(static_cast<size_t>(L.rgroups));
-    const TileSet rts = make_tileset(d_rt_tiles, static_cast<int>(rtiles.size()), rsc, rgo);
-    const TileSet cts = make_tileset(d_ct_tiles, static_cast<int>(ctiles.size()), csc, cgo);
-    launch::AdjColArgs CA{d_cptr.get(), d_cidx.get(), d_cval.get(), d_pt.get(), d_mb.get(), d_dc.get(),
+    const TileSet rts = RD.set(), cts = CD.set();
+    launch::AdjColArgs CA{d_cptr.get(), CD.idx, CD.v64, d_pt.get(), d_mb.get(), d_dc.get(),
                           d_q.get(), d_r.get(), d_rt.get(), d_gq.get()};
-    launch::AdjRowArgs RA{d_rptr.get(), d_ridx.get(), d_rval.get(), d_rt.get(), d_mr.get(), d_dr.get(),
+    launch::AdjRowArgs RA{d_rptr.get(), RD.idx, RD.v64, d_rt.get(), d_mr.get(), d_dr.get(),
                           d_s.get(), d_mu.get(), d_p.get(), d_pt.get(), d_gs.get()};
     launch::adj_start(st, rts, d_inst.get(), na, RA, tol, max_iter);
     std::vector<AdjInst> hs(static_cast<size_t>(na));

@@ -21,7 +21,6 @@ struct EnhHost {
     std::vector<int> cptr, cidx, rptr, ridx;
     std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
     std::vector<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
-    std::vector<Tile> rtiles, ctiles;
     std::vector<EnhInst> inst;
 };
 
@@ -122,8 +121,6 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const BatchLayout L = make_layout(H, act);
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
-    const Schedule rsc = choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk);
-    const Schedule csc = choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk);
     EnhHost hh;
     hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.ridx.resize(Z);
     hh.cval.resize(Z); hh.rval.resize(Z); hh.cval64.resize(Z); hh.rval64.resize(Z);
@@ -214,34 +211,16 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.rgrp_end = static_cast<int32_t>(L.rgrp_off[ua] + (h.m + kGroup - 1) / kGroup);
         s.cgrp_begin = static_cast<int32_t>(L.cgrp_off[ua]);
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
-        s.row_tile_begin = static_cast<int32_t>(hh.rtiles.size());
-        plan_tiles(h.rptr, h.m, L.row_off[ua], L.nnz_off[ua], L.rgrp_off[ua], a, rsc.block, rsc.tn, hh.rtiles);
-        s.row_tile_end = static_cast<int32_t>(hh.rtiles.size());
-        s.col_tile_begin = static_cast<int32_t>(hh.ctiles.size());
-        plan_tiles(h.cptr, h.n, L.col_off[ua], L.nnz_off[ua], L.cgrp_off[ua], a, csc.block, csc.tn, hh.ctiles);
-        s.col_tile_end = static_cast<int32_t>(hh.ctiles.size());
-    }
-    bool rgo = false, cgo = false;
-    {
-        std::vector<int64_t> rlen(static_cast<size_t>(na)), clen(static_cast<size_t>(na));
-        for (int a = 0; a < na; ++a) {
-            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
-            rlen[static_cast<size_t>(a)] = h.m;
-            clen[static_cast<size_t>(a)] = h.n;
-        }
-        const Layout lay = static_cast<Layout>(cfg.spmv_layout);
-        rgo = gather_order(lay, rsc.tn, hh.rptr, hh.rtiles, L.col_off, clen, hh.ridx, hh.rval, hh.rval64);
-        cgo = gather_order(lay, csc.tn, hh.cptr, hh.ctiles, L.row_off, rlen, hh.cidx, hh.cval, hh.cval64);
     }
 
-    // ---- device buffers ----
+    // ---- device buffers (the matrix in storage order: the layout source) ----
     hipStream_t st = ctx.stream;
-    DBuf<int> d_cptr, d_cidx, d_rptr, d_ridx;
-    DBuf<float> d_cval, d_rval, d_cs, d_ls, d_us, d_rls, d_rus;
-    DBuf<double> d_cval64, d_rval64, d_c64, d_l64, d_u64, d_dc, d_rl64, d_ru64, d_dr;
-    d_cptr.upload(hh.cptr, st); d_cidx.upload(hh.cidx, st); d_rptr.upload(hh.rptr, st);
-    d_ridx.upload(hh.ridx, st); d_cval.upload(hh.cval, st); d_rval.upload(hh.rval, st);
-    d_cval64.upload(hh.cval64, st); d_rval64.upload(hh.rval64, st);
+    DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s;
+    DBuf<float> d_cval_s, d_rval_s, d_cs, d_ls, d_us, d_rls, d_rus;
+    DBuf<double> d_cval64_s, d_rval64_s, d_c64, d_l64, d_u64, d_dc, d_rl64, d_ru64, d_dr;
+    d_cptr.upload(hh.cptr, st); d_cidx_s.upload(hh.cidx, st); d_rptr.upload(hh.rptr, st);
+    d_ridx_s.upload(hh.ridx, st); d_cval_s.upload(hh.cval, st); d_rval_s.upload(hh.rval, st);
+    d_cval64_s.upload(hh.cval64, st); d_rval64_s.upload(hh.rval64, st);
     d_cs.upload(hh.cs, st); d_ls.upload(hh.ls, st); d_us.upload(hh.us, st);
     d_rls.upload(hh.rls, st); d_rus.upload(hh.rus, st);
     d_c64.upload(hh.c64, st); d_l64.upload(hh.l64, st); d_u64.upload(hh.u64, st);
@@ -250,14 +229,37 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     DBuf<float> d_x(C), d_x0(C), d_xp(C), d_xbar(C), d_y(R), d_y0(R), d_yp(R), d_ybar(R);
     DBuf<double> d_xbase(C), d_xchk(C), d_lam(C), d_xbest(C), d_lambest(C);
     DBuf<double> d_ybase(R), d_ychk(R), d_ax(R), d_ybest(R), d_axbest(R);
-    DBuf<Tile> d_rt, d_ct;
     DBuf<EnhInst> d_inst;
-    d_rt.upload(hh.rtiles, st);
-    d_ct.upload(hh.ctiles, st);
-    d_inst.upload(hh.inst, st);
     // per-group partials of the decision sums and maxima (quantity-major)
     const int gcn = static_cast<int>(L.cgroups), grn = static_cast<int>(L.rgroups);
     DBuf<double> d_gc(static_cast<size_t>(kGcCount) * L.cgroups), d_gr(static_cast<size_t>(kGrCount) * L.rgroups);
+
+    // ---- tile schedules: searched on this batch's own matrix (the iterate
+    // buffers serve as scratch before they are initialised), or by rule ----
+    const Layout lay = static_cast<Layout>(cfg.spmv_layout);
+    const DirPlan RP = dir_plan(H, act, L, true), CP = dir_plan(H, act, L, false);
+    const DirSource RS{d_ridx_s.get(), d_rval_s.get(), d_rval64_s.get()};
+    const DirSource CS{d_cidx_s.get(), d_cval_s.get(), d_cval64_s.get()};
+    DevDir RD, CD;
+    double tune_ms = 0.0;
+    if (tune_wanted(cfg, L.nnz)) {
+        std::vector<TuneTrial> trials;
+        tune_ms += tune_dir(RP, true, lay, RS, d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, true,
+                            d_x.get(), d_y.get(), nullptr, st, RD, trials);
+        tune_ms += tune_dir(CP, false, lay, CS, d_cptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, true,
+                            d_y.get(), d_x.get(), d_gc.get(), st, CD, trials);
+    } else {
+        build_dir(RP, choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk), lay, RS, true, true, st, RD);
+        build_dir(CP, choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk), lay, CS, true, true, st, CD);
+    }
+    for (int a = 0; a < na; ++a) {
+        EnhInst &s = hh.inst[static_cast<size_t>(a)];
+        s.row_tile_begin = RD.first[static_cast<size_t>(a)];
+        s.row_tile_end = RD.first[static_cast<size_t>(a) + 1];
+        s.col_tile_begin = CD.first[static_cast<size_t>(a)];
+        s.col_tile_end = CD.first[static_cast<size_t>(a) + 1];
+    }
+    d_inst.upload(hh.inst, st);
     DBuf<int32_t> d_status(static_cast<size_t>(na));
     MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_x0.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
@@ -272,8 +274,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::vector<int32_t> run(static_cast<size_t>(na), kRunning);
         d_status.upload(run, st);
     }
-    TileSet rts = make_tileset(d_rt, static_cast<int>(hh.rtiles.size()), rsc, rgo);
-    TileSet cts = make_tileset(d_ct, static_cast<int>(hh.ctiles.size()), csc, cgo);
+    TileSet rts = RD.set(), cts = CD.set();
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -282,8 +283,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     // ---- step size: power iteration on (D_r A D_c)^T (D_r A D_c) ----
     for (int p = 0; p < 64; ++p) {
         launch::pow_scale(st, cts, d_inst.get(), d_xbar.get(), d_xp.get());
-        launch::pow_rows(st, rts, d_rptr.get(), d_ridx.get(), d_rval.get(), d_xp.get(), d_yp.get());
-        launch::pow_cols(st, cts, d_cptr.get(), d_cidx.get(), d_cval.get(), d_yp.get(), d_xbar.get(),
+        launch::pow_rows(st, rts, d_rptr.get(), RD.idx, RD.v32, d_xp.get(), d_yp.get());
+        launch::pow_cols(st, cts, d_cptr.get(), CD.idx, CD.v32, d_yp.get(), d_xbar.get(),
                          d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
         launch::pow_fin(st, d_inst.get(), na, d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
     }
@@ -311,14 +312,14 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     DBuf<int> d_itb(1);
     MPDHG_HIP(hipMemsetAsync(d_itb.get(), 0, sizeof(int), st));
     const float rho = cfg.reflection;
-    launch::EnhPrimal P{d_cptr.get(), d_cidx.get(), d_cval.get(), d_y.get(), d_yp.get(), d_cs.get(),
+    launch::EnhPrimal P{d_cptr.get(), CD.idx, CD.v32, d_y.get(), d_yp.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_xbar.get(),
                         d_xbase.get(), d_dc.get(), d_xchk.get(), d_gc.get(), gcn, d_itb.get(), rho};
-    launch::EnhDual D{d_rptr.get(), d_ridx.get(), d_rval.get(), d_xbar.get(), d_rls.get(), d_rus.get(),
+    launch::EnhDual D{d_rptr.get(), RD.idx, RD.v32, d_xbar.get(), d_rls.get(), d_rus.get(),
                       d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_ybase.get(), d_dr.get(),
                       d_ychk.get(), d_gr.get(), grn, d_itb.get(), rho};
-    launch::EnhCheck CK{d_rptr.get(), d_ridx.get(), d_cptr.get(), d_cidx.get(), d_rval64.get(),
-                        d_cval64.get(), d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
+    launch::EnhCheck CK{d_rptr.get(), RD.idx, d_cptr.get(), CD.idx, RD.v64,
+                        CD.v64, d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
                         d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
                         d_dr.get(), d_gc.get(), gcn, d_gr.get(), grn, d_status.get(), cfg.refine_ratio,
                         cfg.max_refinements, d_xbest.get(), d_lambest.get(),
@@ -430,15 +431,15 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     for (int a = 0; a < na; ++a) {
                         if (sv[a] != kRunning) continue;
                         const EnhInst &s0 = hh.inst[static_cast<size_t>(a)];
-                        nrt.insert(nrt.end(), hh.rtiles.begin() + s0.row_tile_begin, hh.rtiles.begin() + s0.row_tile_end);
-                        nct.insert(nct.end(), hh.ctiles.begin() + s0.col_tile_begin, hh.ctiles.begin() + s0.col_tile_end);
+                        nrt.insert(nrt.end(), RD.host.begin() + s0.row_tile_begin, RD.host.begin() + s0.row_tile_end);
+                        nct.insert(nct.end(), CD.host.begin() + s0.col_tile_begin, CD.host.begin() + s0.col_tile_end);
                     }
                     keep_rt.emplace_back();
                     keep_rt.back().upload(nrt, st);
                     keep_ct.emplace_back();
                     keep_ct.back().upload(nct, st);
-                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), rsc, rgo);
-                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), csc, cgo);
+                    rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), RD.sc, RD.gather);
+                    cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), CD.sc, CD.gather);
                     covered = running;
                     ++lists;
                     host_keep_rt.push_back(std::move(nrt));
@@ -494,10 +495,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         r.rel_kkt = s.best_kkt;
         r.step_size = s.eta;
         r.primal_weight = s.omega;
-        r.csr_block = rsc.block;
-        r.csr_chunk = rsc.tn;
-        r.csc_block = csc.block;
-        r.csc_chunk = csc.tn;
+        r.csr_block = RD.sc.block;
+        r.csr_chunk = RD.sc.tn;
+        r.csc_block = CD.sc.block;
+        r.csc_chunk = CD.sc.tn;
+        r.tune_ms = tune_ms;
         if (r.x) std::memcpy(r.x, xb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
         if (r.reduced_costs) std::memcpy(r.reduced_costs, lb.data() + co, static_cast<size_t>(h.n) * sizeof(double));
         if (r.row_activity) std::memcpy(r.row_activity, ab.data() + ro, static_cast<size_t>(h.m) * sizeof(double));

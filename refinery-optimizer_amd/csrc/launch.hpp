@@ -41,6 +41,12 @@ void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const fl
 void tile_run(hipStream_t s, const TileSet &ts, const RefInst *st);
 void tile_run(hipStream_t s, const TileSet &ts, const EnhInst *st);
 
+// Gather order of a tile set's matrix copy, built on the device (see
+// build_dir): dst = src with every chunk sorted by gathered index.  d32 /
+// d64 may be null (value copies not wanted).
+void gather_order(hipStream_t s, const TileSet &ts, const int *src_idx, int *dst_idx, const float *s32,
+                  float *d32, const double *s64, double *d64);
+
 // enhanced mode
 struct EnhPrimal {
     const int *cptr, *cidx;

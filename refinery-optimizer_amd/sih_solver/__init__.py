@@ -265,6 +265,7 @@ class Solution:
     csr_chunk: int = 0
     csc_block: int = 0  # and for A^T y
     csc_chunk: int = 0
+    tune_ms: float = 0.0  # time of the schedule search
 
     def is_optimal(self) -> bool:
         return self.status == SolutionStatus.Optimal
@@ -300,6 +301,7 @@ class GpuPdhgConfig:
     spmv_chunk: int = 0  # nonzeros per LDS pass: 0 = auto, or 1024 / 2048 / 4096 / 8192
     launch_graph: int = 1  # enhanced: replay each check cycle as a recorded HIP graph
     spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
+    spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
@@ -339,7 +341,7 @@ def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
                  "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
                  "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
                  "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight",
-                 "csr_block", "csr_chunk", "csc_block", "csc_chunk"):
+                 "csr_block", "csr_chunk", "csc_block", "csc_chunk", "tune_ms"):
         setattr(s, name, getattr(r, name))
     return s
 

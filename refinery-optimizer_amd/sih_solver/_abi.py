@@ -39,6 +39,7 @@ class MpdhgConfig(C.Structure):
         ("spmv_chunk", C.c_int32),
         ("launch_graph", C.c_int32),
         ("spmv_block", C.c_int32),
+        ("spmv_tune", C.c_int32),
     ]
 
 
@@ -91,6 +92,7 @@ class MpdhgResult(C.Structure):
         ("csr_chunk", C.c_int32),
         ("csc_block", C.c_int32),
         ("csc_chunk", C.c_int32),
+        ("tune_ms", C.c_double),
     ]
 
 

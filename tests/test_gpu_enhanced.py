@@ -136,6 +136,26 @@ def test_spmv_schedules_are_bit_identical(need_gpu, layout, block, chunk):
         assert s.rel_kkt == t.rel_kkt and s.step_size == t.step_size, p.name
 
 
+def test_schedule_search_changes_speed_only(need_gpu):
+    # spmv_tune = 2 times every compiled schedule on the batch's own matrix
+    # (layouts built on the device) and keeps the fastest per direction;
+    # whatever it picks, the results equal those of the rule's schedule.
+    probs = [S.gen_random(300, 720, 8, 3), S.gen_staircase(2, 40, 90, 4), toy01(),
+             S.gen_random(2000, 4800, 8, 9), S.gen_staircase(3, 200, 450, 5)]
+    cfg = enh(tol=1e-6)
+    cfg.spmv_tune = 0
+    a = S.solve_batch(probs, cfg)
+    cfg.spmv_tune = 2
+    b = S.solve_batch(probs, cfg)
+    schedules = {(128, 1024), (128, 2048), (256, 1024), (256, 2048), (256, 4096), (512, 4096), (512, 8192)}
+    assert a[0].tune_ms == 0.0 and b[0].tune_ms > 0.0
+    assert (b[0].csr_block, b[0].csr_chunk) in schedules and (b[0].csc_block, b[0].csc_chunk) in schedules
+    for p, s, t in zip(probs, a, b):
+        assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
+        assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
+        assert s.rel_kkt == t.rel_kkt, p.name
+
+
 def test_graph_replay_equals_eager_launches(need_gpu):
     # Full check cycles replay a recorded HIP graph, re-recorded after each
     # tile-list compaction; eager work (status copies, compactions) runs

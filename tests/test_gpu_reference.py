@@ -148,3 +148,9 @@ def test_layouts_and_schedules_are_bit_identical(need_gpu):
                 assert_bitexact(s, r, f"{p.name} layout={layout} block={block} chunk={chunk}")
                 if block:
                     assert (s.csr_block, s.csr_chunk, s.csc_block, s.csc_chunk) == (block, chunk, block, chunk)
+    # the schedule search (every schedule timed on this batch) picks one of them
+    c.spmv_layout, c.spmv_block, c.spmv_chunk, c.spmv_tune = 0, 0, 0, 2
+    sols = S.solve_batch(probs, c)
+    assert sols[0].tune_ms > 0.0 and (sols[0].csr_block, sols[0].csr_chunk) in SCHEDULES
+    for p, s, r in zip(probs, sols, refs):
+        assert_bitexact(s, r, f"{p.name} searched schedule")
