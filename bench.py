@@ -116,7 +116,7 @@ def bench_sweep(args):
     nnz = sum(p.num_nonzeros() for p in probs)
     log(f"generated 512 LPs ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
     cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=args.max_iterations,
-                                   launch_graph=0 if args.no_graph else 1)
+                                   launch_graph=0 if args.no_graph else 1, spmv_tune=args.spmv_tune)
     for w in range(args.warmup):
         S.solve_batch(probs, cfg)
         log(f"warmup step {w} done")
@@ -186,9 +186,12 @@ def bench_large(args, mode):
     if mode == "ref":
         cfg = S.GpuPdhgConfig(max_iterations=max(args.warmup, 1), tol_primal=0.0, tol_dual=0.0)
         S.solve_gpu_pdhg(p, cfg)
-        cfg = S.GpuPdhgConfig(max_iterations=args.steps, tol_primal=0.0, tol_dual=0.0, profile=True)
+        cfg = S.GpuPdhgConfig(max_iterations=args.steps, tol_primal=0.0, tol_dual=0.0,
+                              spmv_tune=args.spmv_tune, launch_graph=0 if args.no_graph else 1)
         sol = S.solve_gpu_pdhg(p, cfg)
-        rf, spmv = roofline(sol, ("k_ref_primal", "k_ref_dual"))
+        # kernel timings from a separate run whose first cycles launch eagerly with events
+        cfg.profile, cfg.max_iterations = True, min(args.steps, 100)
+        rf, spmv = roofline(S.solve_gpu_pdhg(p, cfg), ("k_ref_primal", "k_ref_dual"))
         out = {"metric": "PDHG iterations/s (reference mode); SpMV HBM GB/s vs peak",
                "value": args.steps / sol.time_solver_sec, "unit": "iterations/s",
                "ms_per_step": 1e3 * sol.time_solver_sec / args.steps, "dtype": "fp32",
@@ -205,7 +208,8 @@ def bench_large(args, mode):
             log(f"rocSPARSE {name}: {v.us_per_iter:.1f} us/iter (A^T y {v.us_spmv_t:.1f} us, "
                 f"A x {v.us_spmv:.1f} us); engine {1e6 * sol.time_solver_sec / args.steps:.1f} us/iter")
     else:
-        cfg = S.GpuPdhgConfig.enhanced(tol=1e-4, max_iterations=args.steps, profile=True)
+        cfg = S.GpuPdhgConfig.enhanced(tol=1e-4, max_iterations=args.steps, profile=True,
+                                       spmv_tune=args.spmv_tune)
         sol = S.solve_gpu_pdhg(p, cfg)
         rf, spmv = roofline(sol)
         out = {"metric": "time to 1e-4 rel KKT (enhanced); SpMV HBM GB/s vs peak",
@@ -241,7 +245,7 @@ def bench_staircase(args):
     p = S.gen_staircase(T, R, Cb, seed)
     log(f"staircase T={T}: m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()}")
     cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=400000,
-                                   launch_graph=0 if args.no_graph else 1)
+                                   launch_graph=0 if args.no_graph else 1, spmv_tune=args.spmv_tune)
     for _ in range(args.warmup):
         S.solve_gpu_pdhg(p, cfg)
     times = []
@@ -419,6 +423,9 @@ def main():
     ap.add_argument("--cpu-lps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-LP-per-call comparison")
+    ap.add_argument("--spmv-tune", type=int, default=1, choices=(0, 1, 2),
+                    help="tile-schedule search (mpdhg_config.spmv_tune): 1 = time the schedules on "
+                         "large batches, 0 = the rule (deterministic choice, used by profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying recorded check cycles "
                          "(the kernels are the same; rocprofv3's kernel tracing faults on graph "

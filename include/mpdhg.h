@@ -83,7 +83,7 @@ typedef struct mpdhg_config {
                                      1: storage order everywhere. */
     int32_t spmv_chunk;           /* nonzeros staged through LDS per workgroup pass: 0 = auto,
                                      or 1024, 2048, 4096, 8192 */
-    int32_t launch_graph;         /* enhanced: 1 = record one check cycle (the fused launch pairs,
+    int32_t launch_graph;         /* 1 = record one check cycle (the fused launch pairs,
                                      the FP64 check, the decisions) as a HIP graph and replay it
                                      for every full cycle; 0 = launch kernel by kernel. */
     int32_t spmv_block;           /* threads per SpMV workgroup (rows per tile at most): 0 = auto,

@@ -380,7 +380,6 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     // ---- profiling samples: HIP events around the fused SpMV launches ----
     const int K = cfg.max_iterations, F = cfg.check_frequency;
     const int nsamp = cfg.profile ? std::min(256, std::max(K, 0)) : 0;
-    const int stride = nsamp ? std::max(1, K / nsamp) : 1;
     if (nsamp) event_pool(ctx, static_cast<size_t>(3 * nsamp));
     int used = 0;
 
@@ -397,44 +396,80 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
     } ev;
     for (auto &v : ev.e) MPDHG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    // The iteration count lives on the device (advanced by the last launch of
+    // each check cycle), so one recorded cycle -- F fused pairs, the check,
+    // the decisions -- replays unchanged as a HIP graph for every full cycle.
+    DBuf<int> d_itb(1);
+    MPDHG_HIP(hipMemsetAsync(d_itb.get(), 0, sizeof(int), st));
+    auto issue_cycle = [&](int len, bool sample) {
+        for (int i = 0; i < len; ++i) {
+            const bool samp = sample && used < nsamp;
+            if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
+            launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32, d_y.get(), d_c.get(),
+                               d_l.get(), d_u.get(), d_tau.get(), d_x.get(), d_xbar.get());
+            if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
+            launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32, d_xbar.get(), d_rl.get(),
+                             d_ru.get(), d_sigma.get(), d_y.get());
+            if (samp) {
+                MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
+                ++used;
+            }
+            // (:237-239 restart kernel omitted: it writes xbar = x, and the next
+            //  primal update overwrites xbar before anything reads it.)
+        }
+        launch::ref_rowres(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32, d_x.get(), d_rl.get(),
+                           d_ru.get(), d_prow.get());
+        launch::ref_colres(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32, d_y.get(), d_x.get(),
+                           d_c.get(), d_l.get(), d_u.get(), d_tau.get(), d_pcol.get());
+        launch::ref_finalize(st, d_inst.get(), na, d_prow.get(), d_pcol.get(), d_itb.get(), len, K,
+                             cfg.tol_primal, cfg.tol_dual);
+        launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
+        launch::tile_run(st, rts, d_inst.get());
+        launch::tile_run(st, cts, d_inst.get());
+        launch::enh_advance(st, d_itb.get(), len);
+    };
+    struct CycleGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        ~CycleGraph() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } graph;
     int nchk = 0;
     const double t_loop = now_sec();
-    int iter;
-    for (iter = 1; iter <= K; ++iter) {
-        const bool samp = nsamp && (iter % stride == 0) && used < nsamp;
-        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
-        launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32,
-                           d_y.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(), d_x.get(),
-                           d_xbar.get());
-        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
-        launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32,
-                         d_xbar.get(), d_rl.get(), d_ru.get(), d_sigma.get(), d_y.get());
-        if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
-        if (samp) ++used;
-        // (:237-239 restart kernel omitted: it writes xbar = x, and the next
-        //  primal update overwrites xbar before anything reads it.)
-        if (iter % F == 0 || iter == K) {
-            launch::ref_rowres(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32,
-                               d_x.get(), d_rl.get(), d_ru.get(), d_prow.get());
-            launch::ref_colres(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32,
-                               d_y.get(), d_x.get(), d_c.get(), d_l.get(), d_u.get(), d_tau.get(),
-                               d_pcol.get());
-            launch::ref_finalize(st, d_inst.get(), na, d_prow.get(), d_pcol.get(), iter, K,
-                                 cfg.tol_primal, cfg.tol_dual);
-            launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
-            launch::tile_run(st, rts, d_inst.get());
-            launch::tile_run(st, cts, d_inst.get());
-            MPDHG_HIP(hipGetLastError());
-            const int cur = nchk & 1;
-            MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(RefInst),
-                                     hipMemcpyDeviceToHost, st));
-            MPDHG_HIP(hipEventRecord(ev.e[cur], st));
-            if (nchk++ > 0) {
-                MPDHG_HIP(hipEventSynchronize(ev.e[cur ^ 1]));
-                bool any = false;
-                for (int a = 0; a < na; ++a) any = any || (snaps[cur ^ 1][a].status == kRunning);
-                if (!any) break;
+    for (int done = 0; done < K;) {
+        const int len = std::min(F, K - done);
+        if (nsamp && used < nsamp) {
+            issue_cycle(len, true);  // profiled cycles: eager, with events around the fused pairs
+        } else if (cfg.launch_graph && len == F) {
+            if (!graph.e) {
+                MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    issue_cycle(F, false);
+                } catch (...) {
+                    hipGraph_t partial = nullptr;
+                    (void)hipStreamEndCapture(st, &partial);
+                    if (partial) (void)hipGraphDestroy(partial);
+                    throw;
+                }
+                MPDHG_HIP(hipStreamEndCapture(st, &graph.g));
+                MPDHG_HIP(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
             }
+            MPDHG_HIP(hipGraphLaunch(graph.e, st));
+        } else {
+            issue_cycle(len, false);
+        }
+        done += len;
+        MPDHG_HIP(hipGetLastError());
+        const int cur = nchk & 1;
+        MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(RefInst), hipMemcpyDeviceToHost, st));
+        MPDHG_HIP(hipEventRecord(ev.e[cur], st));
+        if (nchk++ > 0) {
+            MPDHG_HIP(hipEventSynchronize(ev.e[cur ^ 1]));
+            bool any = false;
+            for (int a = 0; a < na; ++a) any = any || (snaps[cur ^ 1][a].status == kRunning);
+            if (!any) break;
         }
     }
     MPDHG_HIP(hipStreamSynchronize(st));

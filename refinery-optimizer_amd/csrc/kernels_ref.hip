@@ -128,7 +128,8 @@ __global__ __launch_bounds__(B) void k_ref_colres(
 // gpu_solver.cpp:252-270 per instance: P, D, best-x bookkeeping, stop test.
 __global__ __launch_bounds__(kInstBlock) void k_ref_finalize(RefInst *__restrict__ st, int ninst,
                                                              const float *__restrict__ prow,
-                                                             const float *__restrict__ pcol, int iter,
+                                                             const float *__restrict__ pcol,
+                                                             const int *__restrict__ itb, int len,
                                                              int max_iter, float tol_p, float tol_d) {
     __shared__ float red[kInstBlock / 64];
     const int q = blockIdx.x;
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(kInstBlock) void k_ref_finalize(RefInst *__restrict
         md = fmaxf(md, pcol[t]);
     mp = block_max<kInstBlock>(mp, red);
     md = block_max<kInstBlock>(md, red);
+    const int iter = *itb + len;  // the iteration this check follows
     if (threadIdx.x == 0) {
         const float p = mp / s.max_abs_b;
         const float d = md / s.max_abs_c;
@@ -211,9 +213,9 @@ void ref_colres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *
 }
 
 void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, const float *pcol,
-                  int iter, int max_iter, float tol_p, float tol_d) {
+                  const int *itb, int len, int max_iter, float tol_p, float tol_d) {
     hipLaunchKernelGGL(dev::k_ref_finalize, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, prow,
-                       pcol, iter, max_iter, tol_p, tol_d);
+                       pcol, itb, len, max_iter, tol_p, tol_d);
 }
 
 void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const float *x,

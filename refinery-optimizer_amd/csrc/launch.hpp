@@ -33,8 +33,9 @@ void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *
 void ref_colres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *cptr,
                 const int *cidx, const float *cval, const float *y, const float *x,
                 const float *c, const float *l, const float *u, const float *tau, float *part);
+// The check after iteration *itb + len (itb: iterations before the cycle).
 void ref_finalize(hipStream_t s, RefInst *st, int ninst, const float *prow, const float *pcol,
-                  int iter, int max_iter, float tol_p, float tol_d);
+                  const int *itb, int len, int max_iter, float tol_p, float tol_d);
 void ref_copy_best(hipStream_t s, const TileSet &ts, const RefInst *st, const float *x,
                    float *xbest);
 // Tile::run = (instance status == kRunning) for every tile of the set.

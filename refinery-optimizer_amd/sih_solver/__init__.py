@@ -299,7 +299,7 @@ class GpuPdhgConfig:
     refine_ratio: float = 1e-2
     spmv_layout: int = 0  # 0: gather order where it applies, 1: storage order
     spmv_chunk: int = 0  # nonzeros per LDS pass: 0 = auto, or 1024 / 2048 / 4096 / 8192
-    launch_graph: int = 1  # enhanced: replay each check cycle as a recorded HIP graph
+    launch_graph: int = 1  # replay each full check cycle as a recorded HIP graph
     spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
     spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
 

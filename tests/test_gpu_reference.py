@@ -102,9 +102,10 @@ def test_edge_configurations(need_gpu):
     p = random_lp(50, 80, 0.1, 9)[0]
     for kw in (dict(max_iterations=0), dict(max_iterations=1), dict(check_frequency=1, max_iterations=40),
                dict(step_tau=0.01, step_sigma=0.02, max_iterations=300),
-               dict(max_iterations=251, check_frequency=250)):
-        c = cfg(**kw)
-        assert_bitexact(S.solve_gpu_pdhg(p, c), oracle_of(p, c), str(kw))
+               dict(max_iterations=251, check_frequency=250), dict(max_iterations=260, check_frequency=25)):
+        for graph in (1, 0):  # full check cycles replayed as a recorded graph, or launched one by one
+            c = cfg(launch_graph=graph, **kw)
+            assert_bitexact(S.solve_gpu_pdhg(p, c), oracle_of(p, c), f"{kw} graph={graph}")
     z = S.solve_gpu_pdhg(p, cfg(max_iterations=0))
     assert z.simplex_iterations == 1 and z.status == S.SolutionStatus.IterationLimit
     assert np.array_equal(z.x, np.clip(0.0, p.col_lower, p.col_upper))
