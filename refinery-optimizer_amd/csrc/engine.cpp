@@ -380,7 +380,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     // ---- profiling samples: HIP events around the fused SpMV launches ----
     const int K = cfg.max_iterations, F = cfg.check_frequency;
     const int nsamp = cfg.profile ? std::min(256, std::max(K, 0)) : 0;
-    if (nsamp) event_pool(ctx, static_cast<size_t>(3 * nsamp));
+    if (nsamp) event_pool(ctx, static_cast<size_t>(4 * nsamp));
     int used = 0;
 
     // ---- hot loop (gpu_solver.cpp:227-271) ----
@@ -403,17 +403,15 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     MPDHG_HIP(hipMemsetAsync(d_itb.get(), 0, sizeof(int), st));
     auto issue_cycle = [&](int len, bool sample) {
         for (int i = 0; i < len; ++i) {
+            // sampled pairs: events at each fused kernel's own start and end
             const bool samp = sample && used < nsamp;
-            if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
+            hipEvent_t *e = samp ? ctx.events.data() + 4 * used : nullptr;
             launch::ref_primal(st, cts, d_inst.get(), d_cptr.get(), CD.idx, CD.v32, d_y.get(), d_c.get(),
-                               d_l.get(), d_u.get(), d_tau.get(), d_x.get(), d_xbar.get());
-            if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
+                               d_l.get(), d_u.get(), d_tau.get(), d_x.get(), d_xbar.get(), e ? e[0] : nullptr,
+                               e ? e[1] : nullptr);
             launch::ref_dual(st, rts, d_inst.get(), d_rptr.get(), RD.idx, RD.v32, d_xbar.get(), d_rl.get(),
-                             d_ru.get(), d_sigma.get(), d_y.get());
-            if (samp) {
-                MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
-                ++used;
-            }
+                             d_ru.get(), d_sigma.get(), d_y.get(), e ? e[2] : nullptr, e ? e[3] : nullptr);
+            if (samp) ++used;
             // (:237-239 restart kernel omitted: it writes xbar = x, and the next
             //  primal update overwrites xbar before anything reads it.)
         }
@@ -477,8 +475,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     double ms_csc = 0, ms_csr = 0;
     for (int s = 0; s < used; ++s) {
         float a = 0, b = 0;
-        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(3 * s)], ctx.events[static_cast<size_t>(3 * s + 1)]));
-        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(3 * s + 1)], ctx.events[static_cast<size_t>(3 * s + 2)]));
+        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(4 * s)], ctx.events[static_cast<size_t>(4 * s + 1)]));
+        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(4 * s + 2)], ctx.events[static_cast<size_t>(4 * s + 3)]));
         ms_csc += a;
         ms_csr += b;
     }

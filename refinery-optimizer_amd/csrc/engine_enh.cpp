@@ -328,7 +328,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                         d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_itb.get()};
     const int nsamp = cfg.profile ? 256 : 0;
     if (nsamp) {
-        while (ctx.events.size() < static_cast<size_t>(3 * nsamp + 4)) {
+        while (ctx.events.size() < static_cast<size_t>(4 * nsamp + 4)) {
             hipEvent_t e;
             MPDHG_HIP(hipEventCreate(&e));
             ctx.events.push_back(e);
@@ -376,18 +376,15 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         while (it < K && !finished) {
             const int len = std::min(F, K - it);
             if (nsamp && used < nsamp && it >= F && covered == na) {
-                // a profiled cycle: eager launches with events around the fused pairs
+                // a profiled cycle: eager launches, each fused kernel with events at
+                // its own start and end
                 for (int i = 0; i < len; ++i) {
                     const int check = (i + 1 == len);
                     const bool samp = used < nsamp && !check;
-                    if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used)], st));
-                    launch::enh_primal(st, cts, d_inst.get(), P, i, check);
-                    if (samp) MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 1)], st));
-                    launch::enh_dual(st, rts, d_inst.get(), D, i, check);
-                    if (samp) {
-                        MPDHG_HIP(hipEventRecord(ctx.events[static_cast<size_t>(3 * used + 2)], st));
-                        ++used;
-                    }
+                    hipEvent_t *e = samp ? ctx.events.data() + 4 * used : nullptr;
+                    launch::enh_primal(st, cts, d_inst.get(), P, i, check, e ? e[0] : nullptr, e ? e[1] : nullptr);
+                    launch::enh_dual(st, rts, d_inst.get(), D, i, check, e ? e[2] : nullptr, e ? e[3] : nullptr);
+                    if (samp) ++used;
                 }
                 launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
                 launch::enh_advance(st, d_itb.get(), len);
@@ -460,8 +457,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     double ms_csc = 0, ms_csr = 0;
     for (int s = 0; s < used; ++s) {
         float a = 0, b = 0;
-        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(3 * s)], ctx.events[static_cast<size_t>(3 * s + 1)]));
-        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(3 * s + 1)], ctx.events[static_cast<size_t>(3 * s + 2)]));
+        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(4 * s)], ctx.events[static_cast<size_t>(4 * s + 1)]));
+        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(4 * s + 2)], ctx.events[static_cast<size_t>(4 * s + 3)]));
         ms_csc += a;
         ms_csr += b;
     }

@@ -1,6 +1,7 @@
 // Device helpers shared by the tile kernels (HIP translation units only).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "engine_dev.hpp"
@@ -284,5 +285,29 @@ __device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
         else MPDHG_LAUNCH_G(ts, 512, 8192, KERNEL, __VA_ARGS__);                                   \
     } while (0)
 static_assert(kNumSchedules == 7, "MPDHG_LAUNCH_TN dispatches every schedule");
+
+// The same launch with HIP events e0 / e1 recorded at the kernel's own start
+// and end (hipExtLaunchKernelGGL): the elapsed time between them is the
+// kernel's execution alone, the figure a kernel trace reports.
+#define MPDHG_LAUNCH_G_EV(ts, e0, e1, B_, TN_, KERNEL, ...)                                        \
+    do {                                                                                           \
+        if ((ts).gather)                                                                           \
+            hipExtLaunchKernelGGL(KERNEL<B_, TN_, true>, dim3((ts).grid), dim3((ts).block), 0, s,  \
+                                  e0, e1, 0, (ts).tiles, (ts).count, (ts).chunk, __VA_ARGS__);      \
+        else                                                                                       \
+            hipExtLaunchKernelGGL(KERNEL<B_, TN_, false>, dim3((ts).grid), dim3((ts).block), 0, s, \
+                                  e0, e1, 0, (ts).tiles, (ts).count, (ts).chunk, __VA_ARGS__);      \
+    } while (0)
+#define MPDHG_LAUNCH_TN_EV(ts, e0, e1, KERNEL, ...)                                                \
+    do {                                                                                           \
+        const int b_ = (ts).block, t_ = (ts).tn;                                                   \
+        if (b_ == 128 && t_ == 1024) MPDHG_LAUNCH_G_EV(ts, e0, e1, 128, 1024, KERNEL, __VA_ARGS__); \
+        else if (b_ == 128 && t_ == 2048) MPDHG_LAUNCH_G_EV(ts, e0, e1, 128, 2048, KERNEL, __VA_ARGS__); \
+        else if (b_ == 256 && t_ == 1024) MPDHG_LAUNCH_G_EV(ts, e0, e1, 256, 1024, KERNEL, __VA_ARGS__); \
+        else if (b_ == 256 && t_ == 2048) MPDHG_LAUNCH_G_EV(ts, e0, e1, 256, 2048, KERNEL, __VA_ARGS__); \
+        else if (b_ == 256 && t_ == 4096) MPDHG_LAUNCH_G_EV(ts, e0, e1, 256, 4096, KERNEL, __VA_ARGS__); \
+        else if (b_ == 512 && t_ == 4096) MPDHG_LAUNCH_G_EV(ts, e0, e1, 512, 4096, KERNEL, __VA_ARGS__); \
+        else MPDHG_LAUNCH_G_EV(ts, e0, e1, 512, 8192, KERNEL, __VA_ARGS__);                          \
+    } while (0)
 
 }  // namespace mpdhg

@@ -574,16 +574,24 @@ void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow) {
     hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, gpow);
 }
 void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
-                int check) {
+                int check, hipEvent_t e0, hipEvent_t e1) {
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
                       a.xbar, a.xbase, a.dc, a.xchk, a.gc, a.gcn, a.itb, a.rho};
-    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, off, check);
+    if (!cts.count) return;
+    if (e0)
+        MPDHG_LAUNCH_TN_EV(cts, e0, e1, dev::k_enh_primal, st, A, off, check);
+    else
+        MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, off, check);
 }
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
-              int check) {
+              int check, hipEvent_t e0, hipEvent_t e1) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
                     a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
-    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
+    if (!rts.count) return;
+    if (e0)
+        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual, st, A, off, check);
+    else
+        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
 }
 void enh_advance(hipStream_t s, int *itb, int len) {
     hipLaunchKernelGGL(dev::k_enh_advance, dim3(1), dim3(64), 0, s, itb, len);

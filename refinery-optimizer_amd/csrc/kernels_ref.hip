@@ -190,14 +190,20 @@ namespace launch {
 
 void ref_primal(hipStream_t s, const TileSet &ts, const RefInst *st, const int *cptr,
                 const int *cidx, const float *cval, const float *y, const float *c, const float *l,
-                const float *u, const float *tau, float *x, float *xbar) {
-    MPDHG_LAUNCH_TN(ts, dev::k_ref_primal, st, cptr, cidx, cval, y, c, l, u, tau, x, xbar);
+                const float *u, const float *tau, float *x, float *xbar, hipEvent_t e0, hipEvent_t e1) {
+    if (e0)
+        MPDHG_LAUNCH_TN_EV(ts, e0, e1, dev::k_ref_primal, st, cptr, cidx, cval, y, c, l, u, tau, x, xbar);
+    else
+        MPDHG_LAUNCH_TN(ts, dev::k_ref_primal, st, cptr, cidx, cval, y, c, l, u, tau, x, xbar);
 }
 
 void ref_dual(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
               const int *ridx, const float *rval, const float *xbar, const float *rl,
-              const float *ru, const float *sigma, float *y) {
-    MPDHG_LAUNCH_TN(ts, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
+              const float *ru, const float *sigma, float *y, hipEvent_t e0, hipEvent_t e1) {
+    if (e0)
+        MPDHG_LAUNCH_TN_EV(ts, e0, e1, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
+    else
+        MPDHG_LAUNCH_TN(ts, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
 }
 
 void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,

@@ -23,10 +23,12 @@ namespace launch {
 // reference mode
 void ref_primal(hipStream_t s, const TileSet &ts, const RefInst *st, const int *cptr,
                 const int *cidx, const float *cval, const float *y, const float *c, const float *l,
-                const float *u, const float *tau, float *x, float *xbar);
+                const float *u, const float *tau, float *x, float *xbar, hipEvent_t e0 = nullptr,
+                hipEvent_t e1 = nullptr);
 void ref_dual(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
               const int *ridx, const float *rval, const float *xbar, const float *rl,
-              const float *ru, const float *sigma, float *y);
+              const float *ru, const float *sigma, float *y, hipEvent_t e0 = nullptr,
+              hipEvent_t e1 = nullptr);
 void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
                 const int *ridx, const float *rval, const float *x, const float *rl,
                 const float *ru, float *part);
@@ -100,11 +102,12 @@ void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cid
               const float *cval, const float *w, float *z, double *gpow);
 // pow_norm2 = |z|^2 per instance, summed over its column groups in a fixed order.
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow);
-// Iteration *a.itb + off of the current check cycle.
+// Iteration *a.itb + off of the current check cycle.  e0 / e1 (optional):
+// events recorded at the kernel's own start and end.
 void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
-                int check);
+                int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
-              int check);
+              int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // *itb += len (end of a check cycle of len iterations).
 void enh_advance(hipStream_t s, int *itb, int len);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)
