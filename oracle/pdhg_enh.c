@@ -10,7 +10,8 @@
  *     (Pock & Chambolle, ICCV 2011; Applegate et al., NeurIPS 2021);
  *   - reflected restarted Halpern PDHG with fixed-point-residual restarts
  *     (Lu & Yang, "Restarted Halpern PDHG for linear programming", 2024);
- *   - primal weight smoothing at restarts (Applegate et al. 2021, theta=1/2);
+ *   - primal weight at restarts by a PI controller on log(omega |dx|/|dy|)
+ *     (Lu & Yang, "cuPDLPx", 2025; gains 0.6 and 0.05);
  *   - iterative refinement on the residual LP (Gleixner, Steffy & Wolter,
  *     INFORMS J. Comput. 2016), FP32 inner solves with FP64 outer updates,
  *     taken at restarts once the KKT error fell by refine_ratio.
@@ -202,6 +203,7 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     const float rho = (float)cfg->reflection, rho1 = 1.0f + rho;
     const int F = cfg->check_frequency;
     int last_restart = 0, restarts = 0, restart_next = 0, rounds = 0;
+    double pw_int = 0.0; /* primal-weight controller: sum of log-balance errors */
     double r_anchor = -1.0, r_prev = -1.0;
     double best[10] = {0}, kk[10] = {0};
     double best_kkt = INFINITY, round_start_kkt = INFINITY;
@@ -285,7 +287,11 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
         /* ---- restart to z+ = this check's point ---- */
         {
             const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
-            if (ddx > 1e-10 && ddy > 1e-10) omega = sqrt(omega * ddy / ddx);
+            if (ddx > 1e-10 && ddy > 1e-10) {
+                const double e = log(omega * ddx / ddy);
+                pw_int += e;
+                omega = exp(log(omega) - (0.6 * e + 0.05 * pw_int));
+            }
         }
         last_restart = done;
         restarts++;

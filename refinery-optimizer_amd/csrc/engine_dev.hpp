@@ -156,6 +156,7 @@ struct EnhInst {
     double best_kkt, best_eps_p, best_eps_d, best_eps_gap, best_pobj, best_dobj;
     double eps_p, eps_d, eps_gap, kkt, pobj, dobj;  // last check
     double pow_norm2;          // power iteration: |A^T A v|^2
+    double pw_int;             // primal-weight controller: sum of the log-balance errors
 };
 
 // Adjoint (CGLS on the active-set system) per-instance state.

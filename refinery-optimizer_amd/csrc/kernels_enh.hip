@@ -13,6 +13,9 @@
 namespace mpdhg {
 namespace dev {
 
+// Gains of the primal-weight controller (see k_enh_finalize).
+constexpr double kPwP = 0.6, kPwI = 0.05;
+
 __device__ __forceinline__ void halpern_weights(int it, int last_restart, float &a, float &b) {
     const int k = it - last_restart;
     a = static_cast<float>(k + 1) / static_cast<float>(k + 2);
@@ -419,10 +422,18 @@ __global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict
                     (static_cast<double>(kc) >= 0.36 * static_cast<double>(done));
     s.r_prev = r;
     if (!rs) return;
-    // restart to z+ = this check's point; primal weight from the anchor move
+    // restart to z+ = this check's point.  Primal weight: a PI controller on
+    // the log-balance e = log(omega |dx| / |dy|) of the primal and dual moves
+    // since the last restart (Lu & Yang, cuPDLPx, 2025): log omega -= kP e +
+    // kI sum(e).  (The PDLP smoothing omega <- sqrt(omega |dy| / |dx|) is the
+    // P term alone with kP = 1/2.)
     {
         const double ddx = sqrt(dxa2), ddy = sqrt(dya2);
-        if (ddx > 1e-10 && ddy > 1e-10) s.omega = sqrt(s.omega * ddy / ddx);
+        if (ddx > 1e-10 && ddy > 1e-10) {
+            const double e = log(s.omega * ddx / ddy);
+            s.pw_int += e;
+            s.omega = exp(log(s.omega) - (kPwP * e + kPwI * s.pw_int));
+        }
     }
     s.last_restart = done;
     s.restarts += 1;
