@@ -230,6 +230,44 @@ const mpdhg_lp *mpdhg_lp_view(const mpdhg_lp_owned *lp);
 const int8_t *mpdhg_lp_row_kind(const mpdhg_lp_owned *lp);
 void mpdhg_lp_free(mpdhg_lp_owned *lp);
 
+/* ---- MPS / QPS files: io::read_mps / io::write_mps (io/mps_io.hpp:10-13) */
+#define MPDHG_VAR_CONTINUOUS 0 /* model::VariableType (problem.hpp:19-24) */
+#define MPDHG_VAR_BINARY 1
+#define MPDHG_VAR_INTEGER 2
+/* The model::Problem fields beside the LP data: name, sense, objective
+ * constant, names, variable types and the quadratic objective 1/2 x^T Q x.
+ * The PDHG path reads none of them (integrality and Q are ignored, as in
+ * gpu_solver.cpp:60-307). */
+typedef struct mpdhg_lp_meta {
+    const char *name;
+    int32_t sense;                 /* 1 minimize, -1 maximize (ObjectiveSense) */
+    double obj_offset;             /* objective constant (minus the RHS of the N row) */
+    const char *const *row_names;  /* [m] or NULL (the writer then names rows R0, R1, ...) */
+    const char *const *col_names;  /* [n] or NULL (columns C0, C1, ...) */
+    const int8_t *var_type;        /* [n] MPDHG_VAR_*, or NULL: all continuous */
+    int32_t has_quadratic;         /* a QUADOBJ/QMATRIX section was read */
+    int64_t q_nnz;
+    const int64_t *q_col_ptr;      /* [n+1], Q in CSC (both triangles) */
+    const int64_t *q_row_ind;      /* [q_nnz] */
+    const double *q_vals;          /* [q_nnz] */
+} mpdhg_lp_meta;
+/* io::read_mps (mps_io.cpp:62-409): free- or fixed-column MPS/QPS by
+ * whitespace tokens; sections NAME, OBJSENSE, ROWS, COLUMNS (with integer
+ * MARKERs), RHS, RANGES, BOUNDS (UP LO FX FR MI PL BV UI LI), QUADOBJ,
+ * QMATRIX, ENDATA; duplicates summed and |a_ij| <= 1e-15 dropped
+ * (SparseMatrix::from_triplets, sparse_matrix.cpp:13-88).  Errors
+ * (MPDHG_E_ARG, message as the reference's exception text): file not
+ * readable, a malformed or out-of-range number, contradictory bounds. */
+int mpdhg_read_mps(const char *path, mpdhg_lp_owned **out);
+/* Borrowed view of an owned LP's meta fields (valid until mpdhg_lp_free). */
+const mpdhg_lp_meta *mpdhg_lp_meta_view(const mpdhg_lp_owned *lp);
+/* io::write_mps (mps_io.cpp:411-578): the same text the reference writes,
+ * byte for byte (12 significant digits).  meta may be NULL (name "problem",
+ * minimize, no names, continuous, no Q).  free_format is accepted and, as in
+ * the reference, does not change the output. */
+int mpdhg_write_mps(const char *path, const mpdhg_lp *lp, const mpdhg_lp_meta *meta,
+                    int32_t free_format);
+
 #ifdef __cplusplus
 }
 #endif

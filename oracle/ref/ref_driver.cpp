@@ -10,6 +10,12 @@
 // above (plus a first line "sense S") and solve it.
 // Output: "status S", "objective V", "iterations K", then x, row_duals,
 // reduced_costs, col_basis, row_basis, one vector per line.
+// Or: refsimplex --mps-dump file.mps dump.txt -- parse only, and write every
+// field of the parsed model::Problem (name, sense, offset, CSC, bounds,
+// variable types, names, quadratic CSC); on a parse error print
+// "error <what()>" and exit 3.
+// Or: refsimplex --mps-write in.mps out.mps -- parse with the reference reader
+// and write the problem back with the reference writer (io::write_mps).
 #include <sih/io/mps_io.hpp>
 #include <sih/model/problem.hpp>
 #include <sih/model/options.hpp>
@@ -50,10 +56,74 @@ static void dump(const sih::model::Problem &p, const char *path) {
     std::fclose(f);
 }
 
+static void full_dump(const sih::model::Problem &p, const char *path) {
+    std::FILE *f = std::fopen(path, "w");
+    auto ivec = [&](const char *k, const std::vector<int64_t> &x) {
+        std::fprintf(f, "%s %zu", k, x.size());
+        for (auto e : x) std::fprintf(f, " %lld", static_cast<long long>(e));
+        std::fprintf(f, "\n");
+    };
+    auto dvec = [&](const char *k, const std::vector<double> &x) {
+        std::fprintf(f, "%s %zu", k, x.size());
+        for (double e : x) {
+            if (std::isinf(e)) std::fprintf(f, " %s", e > 0 ? "inf" : "-inf");
+            else if (std::isnan(e)) std::fprintf(f, " nan");
+            else std::fprintf(f, " %a", e);
+        }
+        std::fprintf(f, "\n");
+    };
+    auto names = [&](const char *k, const std::vector<std::string> &x) {
+        std::fprintf(f, "%s %zu\n", k, x.size());
+        for (const auto &e : x) std::fprintf(f, "%s\n", e.c_str());
+    };
+    std::fprintf(f, "name\n%s\n", p.name().c_str());
+    std::fprintf(f, "sense %d\nobj_offset %a\n", static_cast<int>(p.sense()), p.obj_offset());
+    std::fprintf(f, "dims %lld %lld %lld\n", static_cast<long long>(p.num_rows()),
+                 static_cast<long long>(p.num_cols()), static_cast<long long>(p.num_nonzeros()));
+    ivec("col_ptr", p.A().csc_col_ptr());
+    ivec("row_ind", p.A().csc_row_ind());
+    dvec("vals", p.A().csc_values());
+    dvec("c", p.c());
+    dvec("col_lower", p.col_lower());
+    dvec("col_upper", p.col_upper());
+    dvec("row_lower", p.row_lower());
+    dvec("row_upper", p.row_upper());
+    std::vector<int64_t> vt;
+    for (auto t : p.var_types()) vt.push_back(static_cast<int64_t>(t));
+    ivec("var_types", vt);
+    names("row_names", p.row_names());
+    names("col_names", p.col_names());
+    std::fprintf(f, "has_quadratic %d\n", p.has_quadratic() ? 1 : 0);
+    if (p.has_quadratic()) {
+        ivec("q_col_ptr", p.Q().csc_col_ptr());
+        ivec("q_row_ind", p.Q().csc_row_ind());
+        dvec("q_vals", p.Q().csc_values());
+    }
+    std::fclose(f);
+}
+
 static void solve_and_print(const sih::model::Problem &p);
 
 int main(int argc, char **argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: refsimplex lp.txt | --mps file.mps dump.txt\n"); return 2; }
+    if (std::string(argv[1]) == "--mps-dump" && argc >= 4) {
+        try {
+            full_dump(sih::io::read_mps(argv[2]), argv[3]);
+        } catch (const std::exception &e) {
+            std::printf("error %s\n", e.what());
+            return 3;
+        }
+        return 0;
+    }
+    if (std::string(argv[1]) == "--mps-write" && argc >= 4) {
+        try {
+            sih::io::write_mps(sih::io::read_mps(argv[2]), argv[3]);
+        } catch (const std::exception &e) {
+            std::printf("error %s\n", e.what());
+            return 3;
+        }
+        return 0;
+    }
     if (std::string(argv[1]) == "--mps" && argc >= 4) {
         sih::model::Problem p = sih::io::read_mps(argv[2]);
         dump(p, argv[3]);

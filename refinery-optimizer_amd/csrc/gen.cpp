@@ -83,7 +83,9 @@ std::vector<int64_t> sample_big(Rng &g, int64_t n, int64_t k) {
     return out;
 }
 
-void finish_view(OwnedLp *o) {
+}  // namespace
+
+void finish_views(OwnedLp *o) {
     o->view.m = o->m;
     o->view.n = o->n;
     o->view.nnz = static_cast<int64_t>(o->vals.size());
@@ -95,9 +97,23 @@ void finish_view(OwnedLp *o) {
     o->view.col_upper = o->u.data();
     o->view.row_lower = o->rl.data();
     o->view.row_upper = o->ru.data();
+    o->row_name_ptr.clear();
+    for (const auto &nm : o->row_names) o->row_name_ptr.push_back(nm.c_str());
+    o->col_name_ptr.clear();
+    for (const auto &nm : o->col_names) o->col_name_ptr.push_back(nm.c_str());
+    mpdhg_lp_meta &e = o->meta;
+    e.name = o->name.c_str();
+    e.sense = o->sense;
+    e.obj_offset = o->obj_offset;
+    e.row_names = o->row_names.empty() ? nullptr : o->row_name_ptr.data();
+    e.col_names = o->col_names.empty() ? nullptr : o->col_name_ptr.data();
+    e.var_type = o->var_type.empty() ? nullptr : o->var_type.data();
+    e.has_quadratic = o->has_q ? 1 : 0;
+    e.q_nnz = static_cast<int64_t>(o->q_vals.size());
+    e.q_col_ptr = o->has_q ? o->q_col_ptr.data() : nullptr;
+    e.q_row_ind = o->has_q ? o->q_row_ind.data() : nullptr;
+    e.q_vals = o->has_q ? o->q_vals.data() : nullptr;
 }
-
-}  // namespace
 
 OwnedLp *gen_random(int64_t m, int64_t n, int32_t k, uint64_t seed) {
     auto *o = new OwnedLp();
@@ -138,7 +154,7 @@ OwnedLp *gen_random(int64_t m, int64_t n, int32_t k, uint64_t seed) {
     o->u.assign(static_cast<size_t>(n), 10.0);
     o->rl = b;
     o->ru = b;
-    finish_view(o);
+    finish_views(o);
     return o;
 }
 
@@ -225,7 +241,7 @@ OwnedLp *gen_staircase(int32_t T, int32_t R, int32_t C, uint64_t seed) {
             o->col_ptr[static_cast<size_t>(col + 1)] = static_cast<int64_t>(o->row_ind.size());
         }
     }
-    finish_view(o);
+    finish_views(o);
     return o;
 }
 
@@ -240,7 +256,7 @@ OwnedLp *gen_scenario(const OwnedLp *base, uint64_t seed, double shock, int32_t 
         if (!caps.empty())
             o->ru[static_cast<size_t>(caps[g.below(caps.size())])] = 0.0;
     }
-    finish_view(o);
+    finish_views(o);
     return o;
 }
 

@@ -52,6 +52,13 @@ class ObjectiveSense(enum.IntEnum):  # problem.hpp:11-14
     Maximize = -1
 
 
+class VariableType(enum.IntEnum):  # problem.hpp:19-24
+    Continuous = 0
+    Binary = 1
+    Integer = 2
+    SemiContinuous = 3
+
+
 @dataclass
 class Triplet:  # sparse_matrix.hpp:10-17
     row: int = 0
@@ -164,6 +171,10 @@ class Problem:
         self.row_upper = np.zeros(0)
         self.col_lower = np.zeros(0)
         self.col_upper = np.zeros(0)
+        self.var_types = np.zeros(0, np.int8)  # VariableType per column
+        self.row_names: List[str] = []
+        self.col_names: List[str] = []
+        self.Q: Optional[SparseMatrix] = None  # quadratic objective 1/2 x^T Q x (read, not solved)
         self.row_kind = None  # staircase workloads: 0 balance, 1 blend, 2 capacity
 
     def resize(self, num_rows: int, num_cols: int):  # problem.cpp:8-23
@@ -173,6 +184,8 @@ class Problem:
         self.row_upper = np.full(num_rows, INF)
         self.col_lower = np.zeros(num_cols)
         self.col_upper = np.full(num_cols, INF)
+        self.var_types = np.zeros(num_cols, np.int8)
+        self.row_names, self.col_names = [], []
 
     @property
     def A(self) -> SparseMatrix:
@@ -193,8 +206,18 @@ class Problem:
     def num_nonzeros(self) -> int:
         return self._A.num_nonzeros()
 
-    def is_mip(self) -> bool:
-        return False
+    def num_integers(self) -> int:  # problem.cpp:93-101
+        vt = np.asarray(self.var_types)
+        return int(np.count_nonzero((vt == VariableType.Integer) | (vt == VariableType.Binary)))
+
+    def is_mip(self) -> bool:  # problem.cpp:113-115; the PDHG path ignores integrality
+        return self.num_integers() > 0
+
+    def has_quadratic(self) -> bool:
+        return self.Q is not None
+
+    def is_qp(self) -> bool:  # problem.hpp:51; the PDHG path ignores Q
+        return self.Q is not None and self.Q.num_nonzeros() > 0
 
     # -- C view ------------------------------------------------------------
     def _c_view(self):
@@ -524,6 +547,7 @@ def _owned_to_problem(ptr: C.c_void_p, name: str) -> Problem:
     p.col_upper = arr(v.col_upper, n, np.float64)
     p.row_lower = arr(v.row_lower, m, np.float64)
     p.row_upper = arr(v.row_upper, m, np.float64)
+    p.var_types = np.zeros(n, np.int8)
     rk = L.mpdhg_lp_row_kind(ptr)
     if rk:
         p.row_kind = np.ctypeslib.as_array(rk, shape=(m,)).astype(np.int8, copy=True)
@@ -562,9 +586,101 @@ def gen_scenario(kind: str, size: tuple, base_seed: int, scenario_seed: int,
     return _owned_to_problem(out, f"{kind}_{'x'.join(map(str, size))}_s{base_seed}_{scenario_seed}")
 
 
+# ---- MPS / QPS files (io/mps_io.hpp:10-13) --------------------------------
+def read_mps(filepath: str) -> Problem:
+    """io::read_mps (mps_io.cpp:62-409), parsed by libmpdhg: the LP, name,
+    sense, objective constant, row/column names, variable types and, for QPS
+    files, Q.  Raises RuntimeError with the reference's message on a file that
+    cannot be opened, a malformed number or contradictory bounds."""
+    L = _abi.lib()
+    ptr = C.c_void_p()
+    _abi.check(L.mpdhg_read_mps(str(filepath).encode(), C.byref(ptr)), "read_mps")
+    try:
+        meta = L.mpdhg_lp_meta_view(ptr).contents
+        n = L.mpdhg_lp_view(ptr).contents.n
+        m = L.mpdhg_lp_view(ptr).contents.m
+        name = meta.name.decode()
+        sense = ObjectiveSense(meta.sense)
+        offset = meta.obj_offset
+        rows = [meta.row_names[i].decode() for i in range(m)] if meta.row_names else []
+        cols = [meta.col_names[j].decode() for j in range(n)] if meta.col_names else []
+        vt = (np.ctypeslib.as_array(meta.var_type, shape=(n,)).astype(np.int8, copy=True)
+              if meta.var_type and n else np.zeros(n, np.int8))
+        Q = None
+        if meta.has_quadratic:
+            qn = meta.q_nnz
+            cp = np.ctypeslib.as_array(meta.q_col_ptr, shape=(n + 1,)).astype(np.int64, copy=True)
+            ri = (np.ctypeslib.as_array(meta.q_row_ind, shape=(qn,)).astype(np.int64, copy=True)
+                  if qn else np.zeros(0, np.int64))
+            qv = (np.ctypeslib.as_array(meta.q_vals, shape=(qn,)).astype(np.float64, copy=True)
+                  if qn else np.zeros(0))
+            Q = SparseMatrix.from_csc(n, n, cp, ri, qv)
+        p = _owned_to_problem(ptr, name)
+        ptr = None  # freed by _owned_to_problem
+    finally:
+        if ptr is not None:
+            L.mpdhg_lp_free(ptr)
+    p.sense, p.obj_offset, p.row_names, p.col_names, p.var_types, p.Q = (
+        sense, offset, rows, cols, vt, Q)
+    return p
+
+
+def write_mps(problem: Problem, filepath: str, free_format: bool = True) -> None:
+    """io::write_mps (mps_io.cpp:411-578): the reference writer's text, byte
+    for byte; free_format does not change the output, as there."""
+    lp, keep = problem._c_view()
+    m, n = problem.num_rows(), problem.num_cols()
+    meta = _abi.MpdhgLpMeta()
+    meta.name = str(problem.name).encode()
+    meta.sense = int(problem.sense)
+    meta.obj_offset = float(problem.obj_offset)
+    names = []
+    if problem.row_names:
+        if len(problem.row_names) < m:  # the reference pads with R<i>
+            problem_rows = list(problem.row_names) + [f"R{i}" for i in range(len(problem.row_names), m)]
+        else:
+            problem_rows = list(problem.row_names)[:m]
+        arr = (C.c_char_p * max(m, 1))(*[s.encode() for s in problem_rows])
+        meta.row_names = C.cast(arr, C.POINTER(C.c_char_p))
+        names.append(arr)
+    if problem.col_names:
+        if len(problem.col_names) < n:
+            problem_cols = list(problem.col_names) + [f"C{j}" for j in range(len(problem.col_names), n)]
+        else:
+            problem_cols = list(problem.col_names)[:n]
+        arr = (C.c_char_p * max(n, 1))(*[s.encode() for s in problem_cols])
+        meta.col_names = C.cast(arr, C.POINTER(C.c_char_p))
+        names.append(arr)
+    vt = np.ascontiguousarray(problem.var_types if len(problem.var_types) == n else np.zeros(n), np.int8)
+    meta.var_type = vt.ctypes.data_as(C.POINTER(C.c_int8))
+    q = None
+    if problem.Q is not None:
+        Q = problem.Q
+        q = (np.ascontiguousarray(Q.csc_col_ptr(), np.int64), np.ascontiguousarray(Q.csc_row_ind(), np.int64),
+             np.ascontiguousarray(Q.csc_values(), np.float64))
+        meta.has_quadratic = 1
+        meta.q_nnz = Q.num_nonzeros()
+        meta.q_col_ptr = q[0].ctypes.data_as(C.POINTER(C.c_int64))
+        meta.q_row_ind = q[1].ctypes.data_as(C.POINTER(C.c_int64))
+        meta.q_vals = q[2].ctypes.data_as(C.POINTER(C.c_double))
+    _abi.check(_abi.lib().mpdhg_write_mps(str(filepath).encode(), C.byref(lp), C.byref(meta),
+                                          int(bool(free_format))), "write_mps")
+
+
+class MpsReader:
+    """Static read/write pair, as py/refinery/parameterized_generator.py:191 calls it."""
+
+    read = staticmethod(read_mps)
+
+    @staticmethod
+    def write(problem: Problem, filepath: str, free_format: bool = True) -> None:
+        write_mps(problem, filepath, free_format)
+
+
 __all__ = [
     "SparseMatrix", "Triplet", "Problem", "Solution", "SolutionStatus", "BasisStatus",
-    "ObjectiveSense", "GpuSolver", "GpuPdhgConfig", "GpuMemoryInfo", "solve_gpu_pdhg",
-    "solve_gpu_hybrid", "solve_batch", "identify_basis", "gen_random", "gen_staircase",
+    "ObjectiveSense", "VariableType", "GpuSolver", "GpuPdhgConfig", "GpuMemoryInfo",
+    "solve_gpu_pdhg", "solve_gpu_hybrid", "solve_batch", "identify_basis", "gen_random",
+    "gen_staircase", "read_mps", "write_mps", "MpsReader",
     "MPDHG_MODE_REFERENCE", "MPDHG_MODE_ENHANCED", "SIH_INFINITY",
 ]

@@ -112,6 +112,22 @@ class MpdhgAdjointIO(C.Structure):
     ]
 
 
+class MpdhgLpMeta(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("sense", C.c_int32),
+        ("obj_offset", C.c_double),
+        ("row_names", C.POINTER(C.c_char_p)),
+        ("col_names", C.POINTER(C.c_char_p)),
+        ("var_type", C.POINTER(C.c_int8)),
+        ("has_quadratic", C.c_int32),
+        ("q_nnz", C.c_int64),
+        ("q_col_ptr", C.POINTER(C.c_int64)),
+        ("q_row_ind", C.POINTER(C.c_int64)),
+        ("q_vals", C.POINTER(C.c_double)),
+    ]
+
+
 # name -> (restype, argtypes); exactly the symbols include/mpdhg.h declares.
 _VP = C.c_void_p
 SIGNATURES = {
@@ -137,6 +153,9 @@ SIGNATURES = {
     "mpdhg_lp_view": (C.POINTER(MpdhgLp), [_VP]),
     "mpdhg_lp_row_kind": (C.POINTER(C.c_int8), [_VP]),
     "mpdhg_lp_free": (None, [_VP]),
+    "mpdhg_read_mps": (C.c_int, [C.c_char_p, C.POINTER(_VP)]),
+    "mpdhg_lp_meta_view": (C.POINTER(MpdhgLpMeta), [_VP]),
+    "mpdhg_write_mps": (C.c_int, [C.c_char_p, C.POINTER(MpdhgLp), C.POINTER(MpdhgLpMeta), C.c_int32]),
 }
 
 _lib = None
