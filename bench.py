@@ -20,6 +20,8 @@ Other workloads:
   adjoint64  64 staircase LPs (48 blocks) forward to 1e-8 and backward
              through the torch.autograd layer, with central finite
              differences on 4 of them.
+  hybrid48   solve_hybrid on a 48-block staircase LP: GPU PDHG to 1e-8,
+             then the FP64 crossover on the host, against HiGHS's optimum.
   copy       a 256 MiB device-to-device elementwise copy (torch), the
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
 
@@ -271,6 +273,46 @@ def bench_staircase(args):
             "roofline": rf, "spmv": spmv}
 
 
+def bench_hybrid(args):
+    """GpuSolver::solve_hybrid on a 48-block staircase LP (19,200 x 43,200):
+    enhanced PDHG on the GPU to 1e-8, the basis hand-off, and the FP64
+    crossover (primal push from the PDHG point, then the simplex) on the
+    host.  Reported beside the HiGHS optimum and HiGHS's own solve time,
+    both from the committed fixture (the reference's simplex factorises in
+    O(m^3) and cannot run at this size)."""
+    import sih_solver as S
+
+    T, R, Cb, seed, tol = 48, 400, 900, 1, 1e-8
+    p = S.gen_staircase(T, R, Cb, seed)
+    log(f"hybrid staircase T={T}: m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()}")
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=400000, spmv_tune=args.spmv_tune)
+    for _ in range(args.warmup):
+        S.solve_gpu_hybrid(p, cfg)
+    walls = []
+    for s in range(args.steps):
+        sol = S.solve_gpu_hybrid(p, cfg)
+        walls.append(sol.time_wall_sec)
+        log(f"step {s}: status {int(sol.status)} objective {sol.primal_objective:.12g}; PDHG "
+            f"{sol.time_solver_sec:.3f}s on the device (kkt {sol.rel_kkt:.1e}), whole call "
+            f"{sol.time_wall_sec:.3f}s, {sol.simplex_iterations} simplex iterations")
+    ref = next((c for c in json.loads(HIGHS_FIXTURES.read_text())
+                if (c["T"], c["R"], c["C"], c["seed"]) == (T, R, Cb, seed)), None)
+    wall = sum(walls) / len(walls)
+    return {"metric": "seconds to an optimal FP64 vertex (GPU PDHG + host crossover)",
+            "value": wall, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * wall,
+            "dtype": "fp32 PDHG iterates, fp64 KKT and simplex",
+            "config": {"workload": f"hybrid staircase{T}: m={p.num_rows()} n={p.num_cols()} "
+                                   f"nnz={p.num_nonzeros()}, seed {seed}, PDHG to {tol:g}"},
+            "status": int(sol.status), "objective": sol.primal_objective,
+            "pdhg_seconds": sol.time_solver_sec, "pdhg_rel_kkt": sol.rel_kkt,
+            "crossover_seconds": sol.time_wall_sec - sol.time_solver_sec,
+            "simplex_iterations": sol.simplex_iterations,
+            "highs_objective": ref["objective"] if ref else None,
+            "highs_seconds": ref["seconds"] if ref else None,
+            "rel_objective_error_vs_highs": (abs(sol.primal_objective - ref["objective"]) /
+                                             abs(ref["objective"]) if ref else None)}
+
+
 def bench_adjoint(args):
     """64 staircase LPs (48 blocks each): batched forward to 1e-8 through the
     torch.autograd layer, the adjoint backward, and central finite differences
@@ -492,7 +534,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
                     choices=["sweep512", "large-ref", "large-enh", "staircase96", "adjoint64",
-                             "copy"])
+                             "hybrid48", "copy"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
@@ -519,6 +561,8 @@ def main():
         out = bench_staircase(args)
     elif args.workload == "adjoint64":
         out = bench_adjoint(args)
+    elif args.workload == "hybrid48":
+        out = bench_hybrid(args)
     else:
         out = bench_large(args, "ref" if args.workload == "large-ref" else "enh")
     out.setdefault("higher_is_better", True)

@@ -209,6 +209,63 @@ int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_ad
 int mpdhg_identify_basis(const mpdhg_lp *lp, const double *x, int8_t *col_basis,
                          int8_t *row_basis);
 
+/* ---- FP64 crossover: SimplexSolver::solve_from_basis ------------------ */
+/* Options of the polish stage (model::Options fields it reads,
+ * options.hpp:85-91). */
+typedef struct mpdhg_polish_opts {
+    int64_t max_iterations;  /* iteration_limit: -1 = 10 (m+n) + 10000, as the reference */
+    double primal_tol;       /* primal_feasibility_tol, 1e-6 */
+    double dual_tol;         /* dual_feasibility_tol, 1e-6 */
+    double obj_offset;       /* added to the reported objective (Problem::obj_offset) */
+    int32_t refactor_every;  /* basis updates between refactorisations, 100 */
+    double time_limit_sec;   /* 0 = none */
+} mpdhg_polish_opts;
+void mpdhg_polish_opts_default(mpdhg_polish_opts *opts);
+/* Result of the polish (the model::Solution fields it fills).  Vector
+ * outputs are caller-allocated; NULL skips one. */
+typedef struct mpdhg_polish_result {
+    double *x;              /* [n] */
+    double *y;              /* [m] row duals, simplex sign: reduced cost = c - A^T y */
+    double *reduced_costs;  /* [n] */
+    double *row_activity;   /* [m] A x (model::Solution::slack) */
+    int8_t *col_basis_out;  /* [n] MPDHG_BASIS_* (3 = free nonbasic) */
+    int8_t *row_basis_out;  /* [m] */
+    int32_t status;         /* MPDHG_STATUS_OPTIMAL / INFEASIBLE / UNBOUNDED / ITERATION_LIMIT / TIME_LIMIT */
+    int32_t warm_start;     /* 1: started from the given basis; 0: the slack basis */
+    int64_t iterations;     /* pivots and bound flips (simplex_iterations) */
+    int64_t refactorizations;
+    int64_t repaired;       /* dependent basis columns replaced by slacks */
+    int64_t bound_flips;
+    int64_t degenerate_pivots; /* basis changes with a zero step */
+    int64_t push_steps;     /* primal push steps from x_start (not in iterations) */
+    double objective;       /* obj_offset + c^T x */
+    double primal_infeasibility; /* sum of basic bound violations at exit */
+    double time_sec;
+} mpdhg_polish_result;
+/* SimplexSolver::solve_from_basis (simplex_solver.cpp:266-329), the polish
+ * of GpuSolver::solve_hybrid (gpu_solver.cpp:397-417): an FP64 bounded
+ * primal simplex (composite phase 1, then phase 2; Devex pricing; bound
+ * perturbation) from the basis given by col_basis[n] / row_basis[m]
+ * (MPDHG_BASIS_*, e.g. from mpdhg_identify_basis).  A basis without exactly
+ * m basic entries, or NULL arrays, starts from the slack basis
+ * (primal_simplex.cpp:79-99).  x_start[n] (may be NULL; an extension) is the
+ * point the basis came from: nonbasic variables it puts inside their bounds
+ * start there and a primal push moves them to bounds (crossover) before the
+ * simplex.  Host code; the objective is minimised (sense is not read, as in
+ * the PDHG). */
+int mpdhg_polish(const mpdhg_lp *lp, const int8_t *col_basis, const int8_t *row_basis,
+                 const double *x_start, const mpdhg_polish_opts *opts, mpdhg_polish_result *res);
+/* GpuSolver::solve_hybrid (gpu_solver.cpp:309-417): mpdhg_solve on the
+ * device, mpdhg_identify_basis on its x, mpdhg_polish from that basis (and
+ * that x) and,
+ * if the polish does not end optimal, mpdhg_polish from the slack basis
+ * (the reference's fallback full simplex solve, :413-416; `iterations`
+ * then counts both).  pdhg may be NULL (its vector outputs may be NULL too);
+ * out->time_sec covers the whole call, as time_wall_sec does there. */
+int mpdhg_solve_hybrid(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
+                       const mpdhg_polish_opts *polish_opts, mpdhg_result *pdhg,
+                       mpdhg_polish_result *out);
+
 /* ---- synthetic workloads (owned by the library) ----------------------- */
 typedef struct mpdhg_lp_owned mpdhg_lp_owned;
 /* Random equality LP: nnz_per_col distinct rows per column, values U[-1,1],

@@ -16,6 +16,10 @@
 // "error <what()>" and exit 3.
 // Or: refsimplex --mps-write in.mps out.mps -- parse with the reference reader
 // and write the problem back with the reference writer (io::write_mps).
+// Or: refsimplex --from-basis lp.txt basis.txt -- the LP as above, and a basis
+// (col_basis[n] then row_basis[m], BasisStatus integers); runs
+// SimplexSolver::solve_from_basis with the options GpuSolver::solve_hybrid
+// passes (gpu_solver.cpp:397-402) and prints the solution as above.
 #include <sih/io/mps_io.hpp>
 #include <sih/model/problem.hpp>
 #include <sih/model/options.hpp>
@@ -103,6 +107,7 @@ static void full_dump(const sih::model::Problem &p, const char *path) {
 }
 
 static void solve_and_print(const sih::model::Problem &p);
+static void print_solution(const sih::model::Solution &s);
 
 int main(int argc, char **argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: refsimplex lp.txt | --mps file.mps dump.txt\n"); return 2; }
@@ -130,7 +135,8 @@ int main(int argc, char **argv) {
         solve_and_print(p);
         return 0;
     }
-    std::ifstream in(argv[1]);
+    const bool from_basis = std::string(argv[1]) == "--from-basis" && argc >= 4;
+    std::ifstream in(argv[from_basis ? 2 : 1]);
     long m, n, nnz;
     in >> m >> n >> nnz;
     std::vector<long> cp(n + 1), ri(nnz);
@@ -151,6 +157,20 @@ int main(int argc, char **argv) {
     for (long j = 0; j < n; ++j) p.col_upper()[j] = rd(in);
     for (long i = 0; i < m; ++i) p.row_lower()[i] = rd(in);
     for (long i = 0; i < m; ++i) p.row_upper()[i] = rd(in);
+    if (from_basis) {
+        std::ifstream bin(argv[3]);
+        std::vector<sih::model::BasisStatus> cb(n), rb(m);
+        for (auto &e : cb) { int v; bin >> v; e = static_cast<sih::model::BasisStatus>(v); }
+        for (auto &e : rb) { int v; bin >> v; e = static_cast<sih::model::BasisStatus>(v); }
+        sih::model::Options opts;
+        opts.strategy.algorithm = sih::model::AlgorithmChoice::DualSimplex;
+        opts.strategy.presolve = sih::model::PresolveMode::Off;
+        opts.strategy.enable_scaling = false;
+        opts.log_to_console = false;
+        opts.telemetry_log_path = "";
+        print_solution(sih::simplex::SimplexSolver::solve_from_basis(p, cb, rb, opts));
+        return 0;
+    }
     solve_and_print(p);
     return 0;
 }
@@ -160,7 +180,10 @@ static void solve_and_print(const sih::model::Problem &p) {
     opts.log_to_console = false;
     opts.telemetry_log_path = "";
     opts.strategy.algorithm = sih::model::AlgorithmChoice::DualSimplex;
-    auto s = sih::simplex::SimplexSolver::solve(p, opts);
+    print_solution(sih::simplex::SimplexSolver::solve(p, opts));
+}
+
+static void print_solution(const sih::model::Solution &s) {
     std::printf("status %d\nobjective %.17g\niterations %lld\n", static_cast<int>(s.status),
                 s.primal_objective, static_cast<long long>(s.simplex_iterations));
     auto vec = [](const char *name, const std::vector<double> &x) {

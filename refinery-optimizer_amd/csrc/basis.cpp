@@ -1,6 +1,7 @@
 // Basis identification of GpuSolver::solve_hybrid (gpu_solver.cpp:319-395):
 // the hand-off from an approximate PDHG point to an FP64 simplex warm start.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -64,4 +65,43 @@ extern "C" int mpdhg_identify_basis(const mpdhg_lp *lp, const double *x, int8_t 
     } catch (const std::exception &e) {
         return mpdhg::fail(MPDHG_E_NOMEM, e.what());
     }
+}
+
+// GpuSolver::solve_hybrid (gpu_solver.cpp:309-417): PDHG on the device, the
+// basis hand-off, the FP64 polish from that basis and, when the polish does
+// not end optimal, the same simplex from the slack basis (:413-416).
+extern "C" int mpdhg_solve_hybrid(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
+                                  const mpdhg_polish_opts *popts, mpdhg_result *pdhg,
+                                  mpdhg_polish_result *out) {
+    if (!lp || !out) return mpdhg::fail(MPDHG_E_ARG, "solve_hybrid: NULL lp or result");
+    if (lp->m < 0 || lp->n < 0) return mpdhg::fail(MPDHG_E_ARG, "solve_hybrid: malformed LP");
+    const auto t0 = std::chrono::steady_clock::now();
+    mpdhg_result local{};
+    mpdhg_result *pr = pdhg ? pdhg : &local;
+    double *const user_x = pr->x;
+    std::vector<double> xbuf;
+    try {
+        if (!user_x) {
+            xbuf.resize(static_cast<size_t>(lp->n));
+            pr->x = xbuf.data();
+        }
+        std::vector<int8_t> cb(static_cast<size_t>(lp->n)), rb(static_cast<size_t>(lp->m));
+        int rc = mpdhg_solve(ctx, lp, cfg, pr);
+        if (rc == MPDHG_OK) rc = mpdhg_identify_basis(lp, pr->x, cb.data(), rb.data());
+        const std::vector<double> xs(pr->x, pr->x + lp->n);
+        pr->x = user_x;
+        if (rc != MPDHG_OK) return rc;
+        rc = mpdhg_polish(lp, cb.data(), rb.data(), xs.data(), popts, out);
+        if (rc == MPDHG_OK && out->status != MPDHG_STATUS_OPTIMAL) {
+            const int64_t first = out->iterations;
+            rc = mpdhg_polish(lp, nullptr, nullptr, nullptr, popts, out);
+            out->iterations += first;
+        }
+        if (rc != MPDHG_OK) return rc;
+    } catch (const std::bad_alloc &) {
+        pr->x = user_x;
+        return mpdhg::fail(MPDHG_E_NOMEM, "solve_hybrid: out of host memory");
+    }
+    out->time_sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return MPDHG_OK;
 }

@@ -446,14 +446,38 @@ class GpuSolver:
 
     @staticmethod
     def solve_hybrid(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
-        """PDHG, then the basis hand-off of gpu_solver.cpp:319-395.  The FP64
-        simplex polish that consumes it is not part of this engine: the
-        returned Solution carries the PDHG point with col_basis/row_basis set."""
-        sol = GpuSolver.solve_pdhg(problem, config)
-        cb, rb = identify_basis(problem, sol.x)
-        sol.col_basis = [BasisStatus(int(v)) for v in cb]
-        sol.row_basis = [BasisStatus(int(v)) for v in rb]
-        return sol
+        """gpu_solver.cpp:309-417: PDHG on the GPU, the basis hand-off, the FP64
+        simplex polish from that basis, and the slack-basis simplex when the
+        polish does not end optimal.  The Solution is the polished one: simplex
+        duals (reduced cost = c - A^T y), basis statuses, simplex_iterations
+        counting the pivots, time_wall_sec the whole call.  The PDHG stage's
+        figures are kept in the extension fields (rel_kkt, restarts, ...)."""
+        config = config or GpuPdhgConfig()
+        cfg = config._c()
+        lp, keep = problem._c_view()
+        m, n = problem.num_rows(), problem.num_cols()
+        o = _abi.MpdhgPolishOpts()
+        _abi.lib().mpdhg_polish_opts_default(C.byref(o))
+        o.obj_offset = float(problem.obj_offset)
+        pd = MpdhgResult()
+        out = (np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(m), np.zeros(n, np.int8), np.zeros(m, np.int8))
+        r = _abi.MpdhgPolishResult()
+        dp, bp = C.POINTER(C.c_double), C.POINTER(C.c_int8)
+        r.x, r.y, r.reduced_costs, r.row_activity = (a.ctypes.data_as(dp) for a in out[:4])
+        r.col_basis_out, r.row_basis_out = (a.ctypes.data_as(bp) for a in out[4:])
+        _abi.check(_abi.lib().mpdhg_solve_hybrid(_ctx()._p, C.byref(lp), C.byref(cfg), C.byref(o),
+                                                 C.byref(pd), C.byref(r)), "mpdhg_solve_hybrid")
+        s = _result_to_solution(pd, np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
+        s.status = SolutionStatus(r.status)
+        s.x, s.row_duals, s.reduced_costs, s.slack = out[0], out[1], out[2], out[3]
+        s.col_basis = [BasisStatus(int(v)) for v in out[4]]
+        s.row_basis = [BasisStatus(int(v)) for v in out[5]]
+        s.primal_objective = r.objective
+        s.dual_bound = r.objective
+        s.simplex_iterations = int(r.iterations)
+        s.time_wall_sec = r.time_sec
+        s.time_solver_sec = pd.solve_sec
+        return s
 
     @staticmethod
     def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None):
@@ -514,6 +538,69 @@ def identify_basis(problem: Problem, x) -> tuple:
         cb.ctypes.data_as(C.POINTER(C.c_int8)), rb.ctypes.data_as(C.POINTER(C.c_int8))),
         "mpdhg_identify_basis")
     return cb, rb
+
+
+@dataclass
+class PolishResult:
+    """Result of the FP64 crossover (SimplexSolver::solve_from_basis)."""
+
+    status: SolutionStatus
+    x: np.ndarray
+    row_duals: np.ndarray      # simplex sign: reduced cost = c - A^T y
+    reduced_costs: np.ndarray
+    slack: np.ndarray          # A x
+    col_basis: np.ndarray      # BasisStatus values (int8)
+    row_basis: np.ndarray
+    objective: float
+    iterations: int
+    warm_start: bool
+    refactorizations: int
+    repaired: int
+    bound_flips: int
+    degenerate_pivots: int
+    push_steps: int
+    primal_infeasibility: float
+    time_sec: float
+
+
+def polish(problem: Problem, col_basis=None, row_basis=None, x_start=None, max_iterations: int = -1,
+           primal_tol: float = 1e-6, dual_tol: float = 1e-6, refactor_every: int = 100,
+           time_limit_sec: float = 0.0) -> PolishResult:
+    """FP64 bounded primal simplex from a basis (simplex_solver.cpp:266-329),
+    in libmpdhg (host code).  With no basis, or one without exactly m basic
+    entries, it starts from the slack basis.  x_start (the point the basis
+    came from) adds a primal push of the variables it puts inside their
+    bounds before the simplex."""
+    lp, keep = problem._c_view()
+    m, n = problem.num_rows(), problem.num_cols()
+    o = _abi.MpdhgPolishOpts()
+    _abi.lib().mpdhg_polish_opts_default(C.byref(o))
+    o.max_iterations, o.primal_tol, o.dual_tol = int(max_iterations), float(primal_tol), float(dual_tol)
+    o.obj_offset, o.refactor_every, o.time_limit_sec = float(problem.obj_offset), int(refactor_every), float(time_limit_sec)
+    out = (np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(m), np.zeros(n, np.int8), np.zeros(m, np.int8))
+    r = _abi.MpdhgPolishResult()
+    dp, bp = C.POINTER(C.c_double), C.POINTER(C.c_int8)
+    r.x, r.y, r.reduced_costs, r.row_activity = (a.ctypes.data_as(dp) for a in out[:4])
+    r.col_basis_out, r.row_basis_out = (a.ctypes.data_as(bp) for a in out[4:])
+    cb = rb = None
+    if col_basis is not None and row_basis is not None:
+        cb = np.ascontiguousarray(np.asarray(col_basis, np.int8))
+        rb = np.ascontiguousarray(np.asarray(row_basis, np.int8))
+        if cb.size != n or rb.size != m:
+            raise ValueError("basis sizes do not match the problem")
+    xs = None
+    if x_start is not None:
+        xs = np.ascontiguousarray(np.asarray(x_start, np.float64))
+        if xs.size != n:
+            raise ValueError("x_start size does not match the problem")
+    _abi.check(_abi.lib().mpdhg_polish(C.byref(lp), cb.ctypes.data_as(bp) if cb is not None else None,
+                                       rb.ctypes.data_as(bp) if rb is not None else None,
+                                       xs.ctypes.data_as(dp) if xs is not None else None,
+                                       C.byref(o), C.byref(r)), "mpdhg_polish")
+    return PolishResult(SolutionStatus(r.status), out[0], out[1], out[2], out[3], out[4], out[5],
+                        r.objective, int(r.iterations), bool(r.warm_start), int(r.refactorizations),
+                        int(r.repaired), int(r.bound_flips), int(r.degenerate_pivots),
+                        int(r.push_steps), r.primal_infeasibility, r.time_sec)
 
 
 def solve_gpu_pdhg(problem: Problem, config: Optional[GpuPdhgConfig] = None) -> Solution:
@@ -681,6 +768,6 @@ __all__ = [
     "SparseMatrix", "Triplet", "Problem", "Solution", "SolutionStatus", "BasisStatus",
     "ObjectiveSense", "VariableType", "GpuSolver", "GpuPdhgConfig", "GpuMemoryInfo",
     "solve_gpu_pdhg", "solve_gpu_hybrid", "solve_batch", "identify_basis", "gen_random",
-    "gen_staircase", "read_mps", "write_mps", "MpsReader",
+    "gen_staircase", "read_mps", "write_mps", "MpsReader", "polish", "PolishResult",
     "MPDHG_MODE_REFERENCE", "MPDHG_MODE_ENHANCED", "SIH_INFINITY",
 ]

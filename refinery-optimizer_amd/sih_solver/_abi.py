@@ -128,6 +128,39 @@ class MpdhgLpMeta(C.Structure):
     ]
 
 
+class MpdhgPolishOpts(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int64),
+        ("primal_tol", C.c_double),
+        ("dual_tol", C.c_double),
+        ("obj_offset", C.c_double),
+        ("refactor_every", C.c_int32),
+        ("time_limit_sec", C.c_double),
+    ]
+
+
+class MpdhgPolishResult(C.Structure):
+    _fields_ = [
+        ("x", C.POINTER(C.c_double)),
+        ("y", C.POINTER(C.c_double)),
+        ("reduced_costs", C.POINTER(C.c_double)),
+        ("row_activity", C.POINTER(C.c_double)),
+        ("col_basis_out", C.POINTER(C.c_int8)),
+        ("row_basis_out", C.POINTER(C.c_int8)),
+        ("status", C.c_int32),
+        ("warm_start", C.c_int32),
+        ("iterations", C.c_int64),
+        ("refactorizations", C.c_int64),
+        ("repaired", C.c_int64),
+        ("bound_flips", C.c_int64),
+        ("degenerate_pivots", C.c_int64),
+        ("push_steps", C.c_int64),
+        ("objective", C.c_double),
+        ("primal_infeasibility", C.c_double),
+        ("time_sec", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); exactly the symbols include/mpdhg.h declares.
 _VP = C.c_void_p
 SIGNATURES = {
@@ -153,6 +186,13 @@ SIGNATURES = {
     "mpdhg_lp_view": (C.POINTER(MpdhgLp), [_VP]),
     "mpdhg_lp_row_kind": (C.POINTER(C.c_int8), [_VP]),
     "mpdhg_lp_free": (None, [_VP]),
+    "mpdhg_polish_opts_default": (None, [C.POINTER(MpdhgPolishOpts)]),
+    "mpdhg_polish": (C.c_int, [C.POINTER(MpdhgLp), C.POINTER(C.c_int8), C.POINTER(C.c_int8),
+                               C.POINTER(C.c_double), C.POINTER(MpdhgPolishOpts),
+                               C.POINTER(MpdhgPolishResult)]),
+    "mpdhg_solve_hybrid": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.POINTER(MpdhgConfig),
+                                     C.POINTER(MpdhgPolishOpts), C.POINTER(MpdhgResult),
+                                     C.POINTER(MpdhgPolishResult)]),
     "mpdhg_read_mps": (C.c_int, [C.c_char_p, C.POINTER(_VP)]),
     "mpdhg_lp_meta_view": (C.POINTER(MpdhgLpMeta), [_VP]),
     "mpdhg_write_mps": (C.c_int, [C.c_char_p, C.POINTER(MpdhgLp), C.POINTER(MpdhgLpMeta), C.c_int32]),

@@ -60,6 +60,8 @@ def _structs():
 
     return {"mpdhg_config": _abi.MpdhgConfig, "mpdhg_lp": _abi.MpdhgLp,
             "mpdhg_result": _abi.MpdhgResult, "mpdhg_adjoint_io": _abi.MpdhgAdjointIO,
+            "mpdhg_lp_meta": _abi.MpdhgLpMeta, "mpdhg_polish_opts": _abi.MpdhgPolishOpts,
+            "mpdhg_polish_result": _abi.MpdhgPolishResult,
             "mpdhg_vendor_result": vendor.MpdhgVendorResult}
 
 
@@ -131,6 +133,8 @@ def test_no_device_fails_loudly():
         S.solve_gpu_pdhg(p)
     with pytest.raises(_abi.MpdhgError):
         S.solve_batch([p], S.GpuPdhgConfig.enhanced())
+    with pytest.raises(_abi.MpdhgError):
+        S.solve_gpu_hybrid(p)  # the PDHG stage needs the device; no CPU stand-in
     assert S.GpuSolver.get_vram_info().total_bytes == 0
 
 
