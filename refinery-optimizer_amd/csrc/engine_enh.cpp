@@ -420,7 +420,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                 const int32_t *sv = slots + prev * na;
                 const int running = static_cast<int>(std::count(sv, sv + na, kRunning));
                 finished = (running == 0);
-                if (!finished && 2 * running <= covered) {
+                if (!finished && 10 * running <= 9 * covered) {
                     // Compact: later launches cover only the running instances'
                     // tiles.  Stream order puts the range update after the cycle
                     // already in flight, which still uses the old lists.
