@@ -15,19 +15,29 @@
 //   * basis factorisation: right-looking sparse LU with Markowitz pivot
 //     choice and threshold partial pivoting (|a| >= 0.1 max of its column),
 //     count-bucketed search; singletons cost nothing, so the slack and
-//     triangular parts of a basis factor without fill;
-//   * updates: product-form eta file, refactorised every `refactor` pivots;
-//   * pricing: largest |d_j| (the reference's rule);
+//     triangular parts of a basis factor without fill.  A column whose
+//     remaining entries fall below 1e-7 of its largest original entry is
+//     taken as dependent;
+//   * updates: product-form eta file, refactorised every `refactor_every`
+//     pivots;
+//   * pricing: Devex reference weights, largest d_j^2 / w_j (the reference
+//     prices by the largest |d_j|);
 //   * phase 1: composite, on the sum of the basic variables' bound
 //     violations (the cost is -1 below, +1 above, recomputed each
 //     iteration); the ratio test stops at the first breakpoint;
 //   * ratio test: two-pass (Harris) with the primal tolerance, taking the
 //     largest |alpha| among the near-ties, and the entering variable's own
 //     bound flip;
+//   * degeneracy: every non-fixed finite bound is expanded by a small
+//     variable-dependent amount during the solve and restored at the end,
+//     where optimality is confirmed again;
 //   * a singular warm basis is repaired: each dependent column is replaced
 //     by the slack of a row the factorisation left unpivoted.  (The
 //     reference falls back to the slack basis instead, primal_simplex.cpp:
 //     140-144; both reach an optimal vertex, the repair in fewer pivots.)
+//   * crossover from a point x0 (an extension; the hybrid passes the PDHG
+//     point): the slack basis with every column at x0, a primal push of the
+//     columns inside their bounds, then the simplex.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
