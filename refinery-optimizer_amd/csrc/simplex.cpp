@@ -68,6 +68,11 @@ double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A basis the factorisation could not make nonsingular, even after repair.
+struct FactorError : std::runtime_error {
+    FactorError() : std::runtime_error("basis factorisation failed after repair") {}
+};
+
 // Columns of [A  -I]: j < n structural (CSC of A), j >= n the row variable j-n.
 struct Cols {
     int64_t m, n;
@@ -456,10 +461,13 @@ class Simplex {
         return warm;
     }
 
+    // The pivoting loop; a factorisation that fails mid-solve ends it with
+    // MPDHG_STATUS_NUMERICAL_FAILURE.
     int run(double t0);
     void results(mpdhg_polish_result &r) const;
 
   private:
+    int iterate(double t0);
     static double finite_lo(double v) { return (std::isinf(v) || v <= -kBig) ? -HUGE_VAL : v; }
     static double finite_hi(double v) { return (std::isinf(v) || v >= kBig) ? HUGE_VAL : v; }
     bool has_lo(int64_t v) const { return std::isfinite(lb_[static_cast<size_t>(v)]); }
@@ -624,7 +632,7 @@ class Simplex {
                 ++repairs_;
             }
         }
-        if (!bad.empty()) throw std::runtime_error("basis factorisation failed after repair");
+        if (!bad.empty()) throw FactorError();
         basic_values();
     }
 
@@ -756,6 +764,14 @@ class Simplex {
 };
 
 int Simplex::run(double t0) {
+    try {
+        return iterate(t0);
+    } catch (const FactorError &) {
+        return status_ = MPDHG_STATUS_NUMERICAL_FAILURE;
+    }
+}
+
+int Simplex::iterate(double t0) {
     const double tp = o_.primal_tol, td = o_.dual_tol;
     const int64_t cap = o_.max_iterations > 0 ? o_.max_iterations : 10 * N_ + 10000;
     std::vector<double> cb(static_cast<size_t>(m_)), alpha;
@@ -961,10 +977,19 @@ int mpdhg_polish(const mpdhg_lp *lp, const int8_t *col_basis, const int8_t *row_
         return mpdhg::fail(MPDHG_E_ARG, "polish: refactor_every >= 1 and positive tolerances required");
     try {
         const double t0 = mpdhg::now();
-        mpdhg::Simplex s(*lp, o);
-        s.start(col_basis, row_basis, x_start);
-        s.run(t0);
-        s.results(*res);
+        auto solve = [&](const int8_t *cb, const int8_t *rb, const double *xs) {
+            mpdhg::Simplex s(*lp, o);
+            s.start(cb, rb, xs);
+            s.run(t0);
+            s.results(*res);
+        };
+        try {
+            solve(col_basis, row_basis, x_start);
+        } catch (const mpdhg::FactorError &) {
+            // a start basis that stays singular after repair: the slack basis,
+            // as the reference does for a singular warm basis
+            solve(nullptr, nullptr, nullptr);
+        }
         res->time_sec = mpdhg::now() - t0;
     } catch (const std::bad_alloc &) {
         return mpdhg::fail(MPDHG_E_NOMEM, "polish: out of host memory");
