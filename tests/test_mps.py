@@ -152,3 +152,11 @@ def test_mps_lp_feeds_the_solver_view(tmp_path):
     q = _read_text(tmp_path, CASES["toy01_lp_simple"])
     lp, keep = q._c_view()
     assert (lp.m, lp.n, lp.nnz) == (2, 2, 4)
+
+
+def test_empty_file_reads_as_an_empty_problem(tmp_path):
+    path = tmp_path / "empty.mps"
+    path.write_text("")
+    p = S.read_mps(str(path))
+    assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (0, 0, 0)
+    assert p.name == "problem" and p.sense == S.ObjectiveSense.Minimize

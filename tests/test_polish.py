@@ -229,3 +229,37 @@ def test_crossover_start_point_size_is_checked():
     p = _problem(CASES[0])
     with pytest.raises(ValueError):
         S.polish(p, x_start=np.zeros(p.num_cols() + 1))
+
+
+def _lp(m, n, trip, c, l, u, rl, ru):
+    p = S.Problem("edge")
+    p.resize(m, n)
+    p.A = S.SparseMatrix.from_triplets(m, n, [S.Triplet(*t) for t in trip])
+    p.c, p.col_lower, p.col_upper = (np.array(v, float) for v in (c, l, u))
+    p.row_lower, p.row_upper = np.array(rl, float), np.array(ru, float)
+    return p
+
+
+def test_polish_without_rows():
+    """m = 0: every column sits at the bound its cost prefers."""
+    p = _lp(0, 3, [], [1.0, -2.0, 0.0], [0.0, -1.0, 2.0], [5.0, 4.0, 7.0], [], [])
+    r = S.polish(p)
+    assert r.status == S.SolutionStatus.Optimal
+    assert r.x.tolist() == [0.0, 4.0, 2.0] and r.objective == -8.0
+
+
+def test_polish_unbounded_and_infeasible_edges():
+    # min -x, x >= 0 free above, one row x - y = 0 with y >= 0: unbounded
+    p = _lp(1, 2, [(0, 0, 1.0), (0, 1, -1.0)], [-1.0, 0.0], [0.0, 0.0], [np.inf, np.inf], [0.0], [0.0])
+    assert S.polish(p).status == S.SolutionStatus.Unbounded
+    # x + y = 5 with x, y in [0, 1]: infeasible
+    q = _lp(1, 2, [(0, 0, 1.0), (0, 1, 1.0)], [1.0, 1.0], [0.0, 0.0], [1.0, 1.0], [5.0], [5.0])
+    assert S.polish(q).status == S.SolutionStatus.Infeasible
+
+
+def test_polish_free_and_fixed_columns():
+    # min x0 + x1 s.t. x0 - x1 = 2, x0 free, x1 fixed at 3  ->  x0 = 5
+    p = _lp(1, 2, [(0, 0, 1.0), (0, 1, -1.0)], [1.0, 1.0], [-np.inf, 3.0], [np.inf, 3.0], [2.0], [2.0])
+    r = S.polish(p)
+    assert r.status == S.SolutionStatus.Optimal
+    assert np.allclose(r.x, [5.0, 3.0]) and abs(r.objective - 8.0) < 1e-12
