@@ -118,7 +118,8 @@ def bench_sweep(args):
     nnz = sum(p.num_nonzeros() for p in probs)
     log(f"generated 512 LPs ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
     cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=args.max_iterations,
-                                   launch_graph=0 if args.no_graph else 1, spmv_tune=args.spmv_tune)
+                                   launch_graph=0 if args.no_graph else 1, spmv_tune=args.spmv_tune,
+                                   spmv_block=args.spmv_block, spmv_chunk=args.spmv_chunk)
     for w in range(args.warmup):
         S.solve_batch(probs, cfg)
         log(f"warmup step {w} done")
@@ -544,6 +545,10 @@ def main():
     ap.add_argument("--spmv-tune", type=int, default=1, choices=(0, 1, 2),
                     help="tile-schedule search (mpdhg_config.spmv_tune): 1 = time the schedules on "
                          "large batches, 0 = the rule (deterministic choice, used by profiling runs)")
+    ap.add_argument("--spmv-block", type=int, default=0, choices=(0, 128, 256, 512),
+                    help="sweep512: force the tile workgroup size of both directions (0 = auto)")
+    ap.add_argument("--spmv-chunk", type=int, default=0, choices=(0, 1024, 2048, 4096, 8192),
+                    help="sweep512: force the tile chunk of both directions (0 = auto)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying recorded check cycles "
                          "(the kernels are the same; rocprofv3's kernel tracing faults on graph "
