@@ -151,33 +151,77 @@ void Factor::factorize(const Cols &A, const std::vector<int64_t> &head, std::vec
     eta_r_.clear(); eta_beg_.assign(1, 0); eta_idx_.clear(); eta_piv_.clear(); eta_val_.clear();
     bad_pos.clear();
     free_rows.clear();
+    // Slack positions first: a row variable's column is -e_i, a singleton,
+    // so it pivots on row i with no multipliers, and the structural entries
+    // of row i become its U row.  Only the structural columns on the
+    // remaining rows enter the Markowitz phase.
+    std::vector<int> srow_step(static_cast<size_t>(m), -1);
+    for (int k = 0; k < m; ++k) {
+        const int64_t v = head[static_cast<size_t>(k)];
+        if (v < A.n) continue;
+        const int i = static_cast<int>(v - A.n);
+        srow_step[static_cast<size_t>(i)] = static_cast<int>(p_.size());
+        p_.push_back(i);
+        q_.push_back(k);
+        piv_.push_back(-1.0);
+    }
+    const int S = static_cast<int>(p_.size());
+    lbeg_.assign(static_cast<size_t>(S) + 1, 0);
+    urow_beg_.assign(static_cast<size_t>(S) + 1, 0);
+    for (int k = 0; k < m; ++k) {
+        const int64_t v = head[static_cast<size_t>(k)];
+        if (v >= A.n) continue;
+        for (int64_t t = A.ptr[v]; t < A.ptr[v + 1]; ++t) {
+            const int st = srow_step[static_cast<size_t>(A.idx[t])];
+            if (st >= 0 && A.val[t] != 0.0) ++urow_beg_[static_cast<size_t>(st) + 1];
+        }
+    }
+    for (int st = 0; st < S; ++st) urow_beg_[static_cast<size_t>(st) + 1] += urow_beg_[static_cast<size_t>(st)];
+    uidx_.resize(static_cast<size_t>(urow_beg_[static_cast<size_t>(S)]));
+    uval_.resize(uidx_.size());
+    std::vector<int> ucur(urow_beg_.begin(), urow_beg_.end() - 1);
     // active submatrix: values by column, patterns by row
     std::vector<std::vector<std::pair<int, double>>> col(static_cast<size_t>(m));
     std::vector<std::vector<int>> row(static_cast<size_t>(m));
+    std::vector<double> orig_max(static_cast<size_t>(m), 0.0);
     for (int k = 0; k < m; ++k) {
         const int64_t v = head[static_cast<size_t>(k)];
+        if (v >= A.n) continue;
         auto &c = col[static_cast<size_t>(k)];
-        if (v >= A.n) {
-            c.emplace_back(static_cast<int>(v - A.n), -1.0);
-        } else {
-            for (int64_t t = A.ptr[v]; t < A.ptr[v + 1]; ++t)
-                if (A.val[t] != 0.0) c.emplace_back(static_cast<int>(A.idx[t]), A.val[t]);
+        for (int64_t t = A.ptr[v]; t < A.ptr[v + 1]; ++t) {
+            if (A.val[t] == 0.0) continue;
+            const int i = static_cast<int>(A.idx[t]);
+            orig_max[static_cast<size_t>(k)] = std::max(orig_max[static_cast<size_t>(k)], std::fabs(A.val[t]));
+            const int st = srow_step[static_cast<size_t>(i)];
+            if (st >= 0) {
+                const int e = ucur[static_cast<size_t>(st)]++;
+                uidx_[static_cast<size_t>(e)] = k;
+                uval_[static_cast<size_t>(e)] = A.val[t];
+            } else {
+                c.emplace_back(i, A.val[t]);
+            }
         }
         for (const auto &e : c) row[static_cast<size_t>(e.first)].push_back(k);
     }
     Buckets cb, rb;
     cb.init(m, m);
     rb.init(m, m);
-    std::vector<char> col_live(static_cast<size_t>(m), 1), row_live(static_cast<size_t>(m), 1);
-    for (int k = 0; k < m; ++k) cb.insert(k, static_cast<int>(col[static_cast<size_t>(k)].size()));
-    for (int i = 0; i < m; ++i) rb.insert(i, static_cast<int>(row[static_cast<size_t>(i)].size()));
+    std::vector<char> col_live(static_cast<size_t>(m), 0), row_live(static_cast<size_t>(m), 0);
+    int live = 0;
+    for (int k = 0; k < m; ++k)
+        if (head[static_cast<size_t>(k)] < A.n) {
+            cb.insert(k, static_cast<int>(col[static_cast<size_t>(k)].size()));
+            col_live[static_cast<size_t>(k)] = 1;
+            ++live;
+        }
+    for (int i = 0; i < m; ++i)
+        if (srow_step[static_cast<size_t>(i)] < 0) {
+            rb.insert(i, static_cast<int>(row[static_cast<size_t>(i)].size()));
+            row_live[static_cast<size_t>(i)] = 1;
+        }
     std::vector<int> slot(static_cast<size_t>(m), -1);
     // a column whose remaining entries have all shrunk below kLuDep times its
-    // original largest entry is dependent on the pivoted ones
-    std::vector<double> orig_max(static_cast<size_t>(m), 0.0);
-    for (int k = 0; k < m; ++k)
-        for (const auto &e : col[static_cast<size_t>(k)])
-            orig_max[static_cast<size_t>(k)] = std::max(orig_max[static_cast<size_t>(k)], std::fabs(e.second));
+    // original largest entry (orig_max) is dependent on the pivoted ones
     auto col_max = [&](int k) {
         double mx = 0.0;
         for (const auto &e : col[static_cast<size_t>(k)]) mx = std::max(mx, std::fabs(e.second));
@@ -189,7 +233,6 @@ void Factor::factorize(const Cols &A, const std::vector<int64_t> &head, std::vec
         return 0.0;
     };
     std::vector<std::pair<int, double>> lcol, urow;
-    int live = m;
     while (live > 0) {
         // dependent columns: no entry left
         while (cb.head[0] >= 0) {
