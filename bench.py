@@ -275,7 +275,8 @@ def bench_staircase(args):
 
 
 def bench_hybrid(args):
-    """GpuSolver::solve_hybrid on a 48-block staircase LP (19,200 x 43,200):
+    """GpuSolver::solve_hybrid on a 48-block staircase LP (19,200 x 43,200; --blocks
+    12 or 96 for the other sizes with HiGHS fixtures):
     enhanced PDHG on the GPU to 1e-8, the basis hand-off, and the FP64
     crossover (primal push from the PDHG point, then the simplex) on the
     host.  Reported beside the HiGHS optimum and HiGHS's own solve time,
@@ -283,7 +284,7 @@ def bench_hybrid(args):
     O(m^3) and cannot run at this size)."""
     import sih_solver as S
 
-    T, R, Cb, seed, tol = 48, 400, 900, 1, 1e-8
+    T, R, Cb, seed, tol = args.blocks, 400, 900, 1, 1e-8
     p = S.gen_staircase(T, R, Cb, seed)
     log(f"hybrid staircase T={T}: m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()}")
     cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=400000, spmv_tune=args.spmv_tune)
@@ -549,6 +550,8 @@ def main():
                     help="sweep512: force the tile workgroup size of both directions (0 = auto)")
     ap.add_argument("--spmv-chunk", type=int, default=0, choices=(0, 1024, 2048, 4096, 8192),
                     help="sweep512: force the tile chunk of both directions (0 = auto)")
+    ap.add_argument("--blocks", type=int, default=48, choices=(12, 48, 96),
+                    help="hybrid48: time blocks of the staircase LP (HiGHS fixtures exist for these)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying recorded check cycles "
                          "(the kernels are the same; rocprofv3's kernel tracing faults on graph "
