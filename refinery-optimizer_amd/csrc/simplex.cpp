@@ -94,8 +94,9 @@ class Factor {
     void ftran(std::vector<double> &v) const;
     // B^T y = c: c indexed by basis position in, y indexed by row out.
     void btran(std::vector<double> &v) const;
-    // Position r now holds the column whose B^-1 image is alpha.
-    void update(int r, const std::vector<double> &alpha);
+    // Position r now holds the column whose B^-1 image is alpha (nonzero at
+    // the positions nz, ascending).
+    void update(int r, const std::vector<double> &alpha, const std::vector<int> &nz);
     int updates() const { return static_cast<int>(eta_r_.size()); }
 
   private:
@@ -107,6 +108,10 @@ class Factor {
     std::vector<int> lbeg_, urow_beg_;
     std::vector<int> lidx_, uidx_;
     std::vector<double> lval_, uval_;
+    // U again by columns (the entries of step k's pivot column in the rows of
+    // earlier steps), for a column-oriented FTRAN that skips zero unknowns
+    std::vector<int> ucol_beg_, ucol_row_;
+    std::vector<double> ucol_val_;
     // eta file
     std::vector<int> eta_r_, eta_beg_, eta_idx_;
     std::vector<double> eta_piv_, eta_val_;
@@ -344,6 +349,31 @@ void Factor::factorize(const Cols &A, const std::vector<int64_t> &head, std::vec
     }
     for (int i = 0; i < m; ++i)
         if (row_live[static_cast<size_t>(i)]) free_rows.push_back(i);
+    // U by columns: the entry (position q', u) of step j's row lies in the
+    // column of the step that pivoted q'.  Only a complete factorisation is
+    // ever solved with; an incomplete one (dependent positions, which have
+    // no step) is repaired and factorised again, so it gets no column form.
+    if (!bad_pos.empty()) {
+        ucol_beg_.clear();
+        ucol_row_.clear();
+        ucol_val_.clear();
+        return;
+    }
+    const int K = static_cast<int>(p_.size());
+    std::vector<int> step_of_pos(static_cast<size_t>(m), -1);
+    for (int k = 0; k < K; ++k) step_of_pos[static_cast<size_t>(q_[k])] = k;
+    ucol_beg_.assign(static_cast<size_t>(K) + 1, 0);
+    for (size_t e = 0; e < uidx_.size(); ++e) ++ucol_beg_[static_cast<size_t>(step_of_pos[static_cast<size_t>(uidx_[e])]) + 1];
+    for (int k = 0; k < K; ++k) ucol_beg_[static_cast<size_t>(k) + 1] += ucol_beg_[static_cast<size_t>(k)];
+    ucol_row_.resize(uidx_.size());
+    ucol_val_.resize(uidx_.size());
+    std::vector<int> cur(ucol_beg_.begin(), ucol_beg_.end() - 1);
+    for (int j = 0; j < K; ++j)
+        for (int e = urow_beg_[j]; e < urow_beg_[j + 1]; ++e) {
+            const int c = cur[static_cast<size_t>(step_of_pos[static_cast<size_t>(uidx_[e])])]++;
+            ucol_row_[static_cast<size_t>(c)] = p_[j];
+            ucol_val_[static_cast<size_t>(c)] = uval_[e];
+        }
 }
 
 void Factor::ftran(std::vector<double> &v) const {
@@ -353,11 +383,14 @@ void Factor::ftran(std::vector<double> &v) const {
         if (t == 0.0) continue;
         for (int e = lbeg_[k]; e < lbeg_[k + 1]; ++e) v[static_cast<size_t>(lidx_[e])] -= lval_[e] * t;
     }
-    w_.assign(static_cast<size_t>(m_), 0.0);
+    // U x = w by columns, last step first: each unknown, once known, is
+    // removed from the rows of the earlier steps; a zero unknown costs one test
+    w_.resize(static_cast<size_t>(m_));
     for (int k = K - 1; k >= 0; --k) {
-        double s = v[static_cast<size_t>(p_[k])];
-        for (int e = urow_beg_[k]; e < urow_beg_[k + 1]; ++e) s -= uval_[e] * w_[static_cast<size_t>(uidx_[e])];
-        w_[static_cast<size_t>(q_[k])] = s / piv_[k];
+        const double xk = v[static_cast<size_t>(p_[k])] / piv_[k];
+        w_[static_cast<size_t>(q_[k])] = xk;
+        if (xk == 0.0) continue;
+        for (int e = ucol_beg_[k]; e < ucol_beg_[k + 1]; ++e) v[static_cast<size_t>(ucol_row_[e])] -= ucol_val_[e] * xk;
     }
     v.swap(w_);
     for (size_t t = 0; t < eta_r_.size(); ++t) {
@@ -392,11 +425,11 @@ void Factor::btran(std::vector<double> &v) const {
     v.swap(w_);
 }
 
-void Factor::update(int r, const std::vector<double> &alpha) {
+void Factor::update(int r, const std::vector<double> &alpha, const std::vector<int> &nz) {
     eta_r_.push_back(r);
     eta_piv_.push_back(alpha[static_cast<size_t>(r)]);
-    for (int i = 0; i < m_; ++i)
-        if (i != r && alpha[static_cast<size_t>(i)] != 0.0) {
+    for (int i : nz)
+        if (i != r) {
             eta_idx_.push_back(i);
             eta_val_.push_back(alpha[static_cast<size_t>(i)]);
         }
@@ -572,8 +605,9 @@ class Simplex {
             }
             column(v, alpha);
             f_.ftran(alpha);
+            nonzeros(alpha);
             double tmax = own;
-            for (int k = 0; k < m_; ++k) {
+            for (const int k : nz_) {
                 const double a = alpha[static_cast<size_t>(k)];
                 if (std::fabs(a) < kPivTol) continue;
                 const double rate = -dir * a;
@@ -588,7 +622,7 @@ class Simplex {
             int r = -1;
             double rmag = 0.0, tstep = own;
             bool to_lower = false;
-            for (int k = 0; k < m_; ++k) {
+            for (const int k : nz_) {
                 const double a = alpha[static_cast<size_t>(k)];
                 if (std::fabs(a) < kPivTol) continue;
                 const double rate = -dir * a;
@@ -606,7 +640,7 @@ class Simplex {
             }
             ++pushes_;
             if (r < 0 || own <= tstep) {
-                for (int k = 0; k < m_; ++k)
+                for (const int k : nz_)
                     x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] -= dir * own * alpha[static_cast<size_t>(k)];
                 if (!has_lo(v) && !has_hi(v)) {
                     x_[u] = 0.0;
@@ -618,7 +652,7 @@ class Simplex {
                 continue;
             }
             const double t = std::max(0.0, tstep);
-            for (int k = 0; k < m_; ++k)
+            for (const int k : nz_)
                 x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] -= dir * t * alpha[static_cast<size_t>(k)];
             x_[u] += dir * t;
             const int64_t leave = head_[static_cast<size_t>(r)];
@@ -629,7 +663,7 @@ class Simplex {
             head_[static_cast<size_t>(r)] = v;
             pos_[u] = r;
             st_[u] = kBasic;
-            f_.update(r, alpha);
+            f_.update(r, alpha, nz_);
             if (f_.updates() >= o_.refactor_every) refactor();
         }
         refactor();
@@ -693,6 +727,14 @@ class Simplex {
             if (pos_[static_cast<size_t>(n_ + i)] < 0) rhs[static_cast<size_t>(i)] += x_[static_cast<size_t>(n_ + i)];
         f_.ftran(rhs);
         for (int k = 0; k < m_; ++k) x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] = rhs[static_cast<size_t>(k)];
+    }
+
+    // The positions where an FTRAN result is nonzero, ascending: the ratio
+    // tests, the updates of the basic values and the eta file visit only these.
+    void nonzeros(const std::vector<double> &a) {
+        nz_.clear();
+        for (int k = 0; k < m_; ++k)
+            if (a[static_cast<size_t>(k)] != 0.0) nz_.push_back(k);
     }
 
     void column(int64_t v, std::vector<double> &out) const {
@@ -797,6 +839,7 @@ class Simplex {
 
     std::vector<double> w_, rho_, row_, lb0_, ub0_, xs_;
     std::vector<int8_t> prio_;
+    std::vector<int> nz_;
     int64_t pushes_ = 0;
     bool perturbed_ = false;
     int64_t degenerate_ = 0;
@@ -879,11 +922,12 @@ int Simplex::iterate(double t0) {
         ++iters_;
         column(q, alpha);
         f_.ftran(alpha);
+        nonzeros(alpha);
         // ratio test: pass 1 with the tolerance, pass 2 the largest |alpha|
         const size_t uq = static_cast<size_t>(q);
         const double flip = (has_lo(q) && has_hi(q)) ? ub_[uq] - lb_[uq] : HUGE_VAL;
         double tmax = flip;
-        for (int k = 0; k < m_; ++k) {
+        for (const int k : nz_) {
             const double a = alpha[static_cast<size_t>(k)];
             if (std::fabs(a) < kPivTol) continue;
             const double rate = -dir * a;  // d x_B[k] / d t
@@ -903,7 +947,7 @@ int Simplex::iterate(double t0) {
         int r = -1;
         double rmag = 0.0, tstep = 0.0;
         bool to_lower = false;
-        for (int k = 0; k < m_; ++k) {
+        for (const int k : nz_) {
             const double a = alpha[static_cast<size_t>(k)];
             if (std::fabs(a) < kPivTol) continue;
             const double rate = -dir * a;
@@ -929,7 +973,7 @@ int Simplex::iterate(double t0) {
             // the entering variable reaches its other bound first
             if (!std::isfinite(flip)) return status_ = MPDHG_STATUS_NUMERICAL_FAILURE;
             const double t = flip;
-            for (int k = 0; k < m_; ++k)
+            for (const int k : nz_)
                 x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] -= dir * t * alpha[static_cast<size_t>(k)];
             x_[uq] = dir > 0 ? ub_[uq] : lb_[uq];
             st_[uq] = dir > 0 ? kUpper : kLower;
@@ -939,7 +983,7 @@ int Simplex::iterate(double t0) {
         const double t = std::max(0.0, tstep);
         if (t <= 1e-12) ++degenerate_;
         devex(r, q, alpha[static_cast<size_t>(r)]);
-        for (int k = 0; k < m_; ++k)
+        for (const int k : nz_)
             x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] -= dir * t * alpha[static_cast<size_t>(k)];
         x_[uq] += dir * t;
         const int64_t leave = head_[static_cast<size_t>(r)];
@@ -950,7 +994,7 @@ int Simplex::iterate(double t0) {
         head_[static_cast<size_t>(r)] = q;
         pos_[uq] = r;
         st_[uq] = kBasic;
-        f_.update(r, alpha);
+        f_.update(r, alpha, nz_);
         if (f_.updates() >= o_.refactor_every) refactor();
     }
 }
