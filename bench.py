@@ -168,12 +168,12 @@ def bench_sweep(args):
     }
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
-        # every 16th LP of the sweep with m <= 4000 (bounded CPU time: the
-        # largest LPs alone would take minutes on the host cores)
-        cs = [p for p in probs[::16] if p.num_rows() <= 4000][: args.cpu_lps]
+        # every 16th LP of the sweep with m <= 8000, about 10-15 s on 16 cores
+        # (the largest LPs alone would take minutes on the host cores)
+        cs = [p for p in probs[::16] if p.num_rows() <= 8000][: args.cpu_lps]
         ok, dt = cpu_port_sample(cs, tol, cores)
         out["cpu_baseline"] = {"value": ok / dt, "unit": "LPs/s", "cores": cores, "kind": "port",
-                               "sample": f"{len(cs)} LPs of the sweep (every 16th with m <= 4000), CPU port of the "
+                               "sample": f"{len(cs)} LPs of the sweep (every 16th with m <= 8000), CPU port of the "
                                          f"same algorithm, {cores} LPs in parallel, {dt:.1f}s, "
                                          f"{ok} converged"}
     return out
