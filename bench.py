@@ -33,6 +33,11 @@ FETCH_SIZE and WRITE_SIZE counters (two rocprofv3 --pmc passes over
 `bench.py --max-iterations 128`, every instance active), scaled by what the
 counters read for the copy workload; the figures are read from
 profiles/pmc_traffic_sweep512.md, which records the commands.
+
+`vs_fp64_vertex` (sweep512) states the error of the batched results against
+FP64 optimal vertices: after the timed solves, the crossover (mpdhg_polish,
+host code) starts from each sampled LP's PDHG point and reaches a vertex;
+the relative objective difference is reported (max and median).
 """
 from __future__ import annotations
 
@@ -108,6 +113,39 @@ def cpu_port_sample(probs, tol, threads):
     return sum(ok), time.perf_counter() - t
 
 
+def vertex_check(probs, sols, tol):
+    """Objective error of the batched PDHG results against FP64 optimal vertices:
+    the crossover (mpdhg_polish from each PDHG point, host code, outside the
+    timed region) on a sample of the staircase-family LPs.  (Random-family
+    bases fill in almost completely under LU, which makes their crossover too
+    slow for a default run.)"""
+    import numpy as np
+
+    import sih_solver as S
+
+    from sih_solver import workloads as W
+
+    family = [sp[0] for sp in W.sweep_specs(len(probs))]
+    picks = [q for q in range(1, len(probs), 8)
+             if family[q] == "staircase" and probs[q].num_rows() <= 8000][:24]
+    errs, t0, opt = [], time.perf_counter(), 0
+    for q in picks:
+        p, s = probs[q], sols[q]
+        cb, rb = S.identify_basis(p, s.x)
+        r = S.polish(p, cb, rb, s.x)
+        if r.status != S.SolutionStatus.Optimal:
+            continue
+        opt += 1
+        errs.append(abs(s.primal_objective - r.objective) / max(1.0, abs(r.objective)))
+    dt = time.perf_counter() - t0
+    log(f"vertex check: {opt}/{len(picks)} FP64 vertices in {dt:.1f}s, objective error "
+        f"max {max(errs):.2e}, median {float(np.median(errs)):.2e}")
+    return {"sample": f"every 8th LP of the staircase family with m <= 8000 ({len(picks)} LPs), FP64 "
+                      "crossover from the batched PDHG point", "vertices": opt,
+            "max_rel_objective_error": max(errs), "median_rel_objective_error": float(np.median(errs)),
+            "pdhg_tol": tol, "seconds": dt}
+
+
 def bench_sweep(args):
     import sih_solver as S
     from sih_solver import workloads as W
@@ -148,6 +186,7 @@ def bench_sweep(args):
     if plain_rate:
         log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s; "
             f"batched {conv / dev:.1f} LPs/s ({conv / dev / plain_rate:.2f}x)")
+    vertex = None if args.no_vertex else vertex_check(probs, sols, tol)
     capped = args.max_iterations < 100000
     out = {
         "metric": METRIC, "value": conv / dev, "unit": "LPs/s", "ms_per_step": 1e3 * dev,
@@ -165,6 +204,7 @@ def bench_sweep(args):
             "seconds": t_plain},
         "batched_over_plain": (conv / dev) / plain_rate if plain_rate else None,
         "roofline": rf, "spmv": spmv,
+        "vs_fp64_vertex": vertex,
     }
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
@@ -543,6 +583,8 @@ def main():
     ap.add_argument("--cpu-lps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-LP-per-call comparison")
+    ap.add_argument("--no-vertex", action="store_true",
+                    help="skip the FP64-vertex objective check of the sweep's results")
     ap.add_argument("--spmv-tune", type=int, default=1, choices=(0, 1, 2),
                     help="tile-schedule search (mpdhg_config.spmv_tune): 1 = time the schedules on "
                          "large batches, 0 = the rule (deterministic choice, used by profiling runs)")
