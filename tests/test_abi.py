@@ -270,3 +270,28 @@ def test_scenario_generator_shocks_costs_and_zeroes_one_capacity():
     changed = np.flatnonzero(sc.row_upper != base.row_upper)
     assert len(changed) == 1 and sc.row_upper[changed[0]] == 0.0
     assert np.array_equal(sc.A.csc_values(), base.A.csc_values())
+
+
+def test_crossover_and_mps_entry_points_reject_bad_arguments(tmp_path):
+    L = _abi.lib()
+    res = _abi.MpdhgPolishResult()
+    assert L.mpdhg_polish(None, None, None, None, None, C.byref(res)) == -1  # NULL lp
+    p = S.gen_random(20, 50, 4, 1)
+    lp, keep = p._c_view()
+    o = _abi.MpdhgPolishOpts()
+    L.mpdhg_polish_opts_default(C.byref(o))
+    assert (o.max_iterations, o.primal_tol, o.dual_tol, o.refactor_every) == (-1, 1e-6, 1e-6, 100)
+    o.primal_tol = 0.0
+    assert L.mpdhg_polish(C.byref(lp), None, None, None, C.byref(o), C.byref(res)) == -1
+    assert L.mpdhg_solve_hybrid(None, None, None, None, None, C.byref(res)) == -1
+    assert L.mpdhg_read_mps(None, None) == -1
+    out = C.c_void_p()
+    assert L.mpdhg_read_mps(str(tmp_path / "none.mps").encode(), C.byref(out)) == -1
+    assert "Could not open MPS file" in L.mpdhg_last_error().decode()
+    assert L.mpdhg_write_mps(None, C.byref(lp), None, 1) == -1
+    # a NULL meta writes a nameless minimisation LP (rows R0.., columns C0..)
+    path = tmp_path / "anon.mps"
+    assert L.mpdhg_write_mps(str(path).encode(), C.byref(lp), None, 1) == 0
+    q = S.read_mps(str(path))
+    assert q.name == "problem" and q.row_names[0] == "R0" and q.col_names[0] == "C0"
+    assert (q.num_rows(), q.num_cols()) == (20, 50)
