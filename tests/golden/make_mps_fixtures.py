@@ -195,6 +195,27 @@ COLUMNS
 RHS
     RHS       R1        inf
 """,
+    "long_names_int_bounds": """NAME          LONG_NAMES_AND_INTEGER_BOUNDS
+ROWS
+ N  OBJECTIVE_ROW
+ L  CAPACITY_CONSTRAINT_1
+ G  DEMAND_CONSTRAINT_22
+COLUMNS
+    MARKER                 'MARKER'                 'INTORG'
+    PRODUCTION_LEVEL_A  OBJECTIVE_ROW  2.5   CAPACITY_CONSTRAINT_1  1.0
+    PRODUCTION_LEVEL_A  DEMAND_CONSTRAINT_22  1.0
+    MARKER                 'MARKER'                 'INTEND'
+    INVENTORY_HOLDING_B  OBJECTIVE_ROW  0.125  DEMAND_CONSTRAINT_22  -1.0
+    X  CAPACITY_CONSTRAINT_1  1e-3
+RHS
+    RHS  CAPACITY_CONSTRAINT_1  1234.5678901234  DEMAND_CONSTRAINT_22  -7.25e-5
+BOUNDS
+ LO BND  PRODUCTION_LEVEL_A  -3
+ UP BND  PRODUCTION_LEVEL_A  40
+ MI BND  INVENTORY_HOLDING_B
+ UP BND  INVENTORY_HOLDING_B  12.000000000001
+ENDATA
+""",
     "error_number": """NAME          BADNUM
 ROWS
  N  OBJ
