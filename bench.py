@@ -600,8 +600,8 @@ def main():
                          "replays in this ROCm release, so profiling runs use this)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one, and the "
-              "engine has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path (how the batch and a "
+              "single large LP would shard is a design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "sweep512":
         out = bench_sweep(args)
