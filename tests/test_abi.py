@@ -295,3 +295,19 @@ def test_crossover_and_mps_entry_points_reject_bad_arguments(tmp_path):
     q = S.read_mps(str(path))
     assert q.name == "problem" and q.row_names[0] == "R0" and q.col_names[0] == "C0"
     assert (q.num_rows(), q.num_cols()) == (20, 50)
+
+
+def test_integration_shim_compiles_against_the_reference_headers(tmp_path):
+    """The C++ shim INTEGRATION.md gives for GpuSolver (solve_pdhg, solve_hybrid,
+    queries) compiles against the reference's own headers and include/mpdhg.h."""
+    import re
+    ref_inc = Path("/root/reference/core/include")
+    if not (ref_inc / "sih" / "gpu" / "gpu_solver.hpp").exists():
+        pytest.skip("reference headers not present")
+    text = (ROOT / "INTEGRATION.md").read_text()
+    shim = re.findall(r"```cpp\n(.*?)```", text, re.S)[0]
+    src = tmp_path / "shim.cpp"
+    src.write_text(shim)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", str(ref_inc), "-I", str(ROOT / "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
