@@ -689,6 +689,7 @@ class Simplex {
     // Factorise the basis, repairing dependent columns with slacks; then
     // recompute the basic values from the nonbasic ones.
     void refactor() {
+        d_valid_ = false;
         std::vector<int> bad, freer;
         for (int pass = 0; pass < 3; ++pass) {
             f_.factorize(A_, head_, bad, freer);
@@ -843,6 +844,7 @@ class Simplex {
     int64_t pushes_ = 0;
     bool perturbed_ = false;
     int64_t degenerate_ = 0;
+    bool d_valid_ = false;  // d_ and y_ match the current basis (phase 2)
     bool warm_ = false;
     int64_t iters_ = 0, refactors_ = 0, repairs_ = 0, flips_ = 0;
     int status_ = MPDHG_STATUS_UNKNOWN;
@@ -882,7 +884,11 @@ int Simplex::iterate(double t0) {
             const size_t v = static_cast<size_t>(head_[static_cast<size_t>(k)]);
             cb[static_cast<size_t>(k)] = phase2 ? c_[v] : (x_[v] < lb_[v] - tp ? -1.0 : (x_[v] > ub_[v] + tp ? 1.0 : 0.0));
         }
-        reduced(cb, phase2);
+        // phase 2 keeps the reduced costs up to date across pivots (below);
+        // phase 1, and the first phase-2 iteration after a factorisation,
+        // compute them afresh
+        if (!(phase2 && d_valid_)) reduced(cb, phase2);
+        d_valid_ = phase2;
         // pricing: the largest d_j^2 / w_j over the dual infeasible variables
         // (Devex reference weights w); a fixed variable never enters
         int64_t q = -1;
@@ -983,6 +989,21 @@ int Simplex::iterate(double t0) {
         const double t = std::max(0.0, tstep);
         if (t <= 1e-12) ++degenerate_;
         devex(r, q, alpha[static_cast<size_t>(r)]);
+        if (d_valid_) {
+            // y += th rho and d_j -= th alpha_rj (alpha_rj = a_j^T rho, -rho_i for the
+            // row variable i), th = d_q / alpha_rq: d_q becomes 0 and the leaving
+            // variable's -th
+            const double th = d_[uq] / alpha[static_cast<size_t>(r)];
+            for (int i = 0; i < m_; ++i) y_[static_cast<size_t>(i)] += th * rho_[static_cast<size_t>(i)];
+            for (int64_t v = 0; v < N_; ++v) {
+                const size_t u = static_cast<size_t>(v);
+                if (st_[u] == kBasic || v == q) continue;
+                const double a = v < n_ ? row_[u] : -rho_[static_cast<size_t>(v - n_)];
+                if (a != 0.0) d_[u] -= th * a;
+            }
+            d_[uq] = 0.0;
+            d_[static_cast<size_t>(head_[static_cast<size_t>(r)])] = -th;
+        }
         for (const int k : nz_)
             x_[static_cast<size_t>(head_[static_cast<size_t>(k)])] -= dir * t * alpha[static_cast<size_t>(k)];
         x_[uq] += dir * t;
