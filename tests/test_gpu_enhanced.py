@@ -172,3 +172,65 @@ def test_graph_replay_equals_eager_launches(need_gpu):
         assert s.simplex_iterations == t.simplex_iterations and s.restarts == t.restarts, p.name
         assert np.array_equal(s.x, t.x) and np.array_equal(s.row_duals, t.row_duals), p.name
         assert s.rel_kkt == t.rel_kkt and s.status == t.status, p.name
+
+
+def _edge_lp():
+    """random_lp(60, 100, 0.1, 1) widened by an empty row (bounds [-1, 1]) and
+    two empty columns (cost 0.7 on [0, 5], cost -0.4 on [-2, 3]), with 8 of
+    its columns made free.  HiGHS finds it bounded and feasible."""
+    base, dense = random_lp(60, 100, 0.1, 1)
+    m, n = 61, 102
+    a = np.zeros((m, n))
+    a[:60, :100] = dense
+    free = np.random.default_rng(101).choice(100, 8, replace=False)
+    l = np.concatenate([base.col_lower, [0.0, -2.0]])
+    u = np.concatenate([base.col_upper, [5.0, 3.0]])
+    l[free], u[free] = -math.inf, math.inf
+    ptr, rows, vals = [0], [], []
+    for j in range(n):
+        nz = np.nonzero(a[:, j])[0]
+        rows += nz.tolist()
+        vals += a[nz, j].tolist()
+        ptr.append(len(rows))
+    p = S.Problem("edge_free_empty")
+    p.resize(m, n)
+    p.A = S.SparseMatrix.from_csc(m, n, np.array(ptr), np.array(rows, np.int64), np.array(vals))
+    p.c = np.concatenate([base.c, [0.7, -0.4]])
+    p.col_lower, p.col_upper = l, u
+    p.row_lower = np.concatenate([base.row_lower, [-1.0]])
+    p.row_upper = np.concatenate([base.row_upper, [1.0]])
+    return p
+
+
+def test_edge_instances_in_one_batch(need_gpu):
+    # Instances with m = 0 or n = 0 are resolved on the host (Optimal, no
+    # iteration, as gpu_solver.cpp:77-80 does); the others iterate on the
+    # device in the same batch.  The edge LP's first check cycle equals the
+    # CPU port bit for bit, it converges to HiGHS's optimum, and every
+    # instance of the batch equals its single solve.
+    edge = _edge_lp()
+    cols_only, rows_only = S.Problem("m0"), S.Problem("n0")
+    cols_only.resize(0, 4)
+    rows_only.resize(3, 0)
+    probs = [cols_only, edge, rows_only, S.gen_random(200, 480, 8, 41)]
+
+    sol = S.solve_gpu_pdhg(edge, enh(tol=0.0, max_iterations=64, max_refinements=0))
+    ref = port(edge, sol, tol=0.0, max_iterations=64, check_frequency=64, max_refinements=0)
+    assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sol.row_duals, ref["y"])
+
+    cfg = enh(tol=1e-6, max_iterations=200000)
+    sols = S.solve_batch(probs, cfg)
+    assert sols[0].status == sols[2].status == S.SolutionStatus.Optimal
+    assert sols[0].simplex_iterations == sols[2].simplex_iterations == 0
+    lp = O.LpArrays.of(edge)
+    s = sols[1]
+    assert s.status == S.SolutionStatus.Optimal
+    assert O.kkt(lp, s.x, s.row_duals)["kkt"] <= 1e-6 * 1.0001
+    assert s.x[100] == 0.0 and s.x[101] == 3.0  # empty columns sit at their cost's bound
+    h, _ = _highs(lp)
+    assert abs(s.primal_objective - h.fun) <= 1e-5 * (1 + abs(h.fun))
+    for p, t in zip(probs[1:], sols[1:]):
+        if p.num_rows() and p.num_cols():
+            one = S.solve_gpu_pdhg(p, cfg)
+            assert one.simplex_iterations == t.simplex_iterations, p.name
+            assert np.array_equal(one.x, t.x) and np.array_equal(one.row_duals, t.row_duals), p.name
