@@ -101,6 +101,9 @@ extern "C" int mpdhg_solve_hybrid(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdh
     } catch (const std::bad_alloc &) {
         pr->x = user_x;
         return mpdhg::fail(MPDHG_E_NOMEM, "solve_hybrid: out of host memory");
+    } catch (const std::exception &e) {
+        pr->x = user_x;
+        return mpdhg::fail(MPDHG_E_STATE, std::string("solve_hybrid: ") + e.what());
     }
     out->time_sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return MPDHG_OK;

@@ -536,8 +536,29 @@ int mpdhg_write_mps(const char *path, const mpdhg_lp *lp, const mpdhg_lp_meta *m
     if (lp->m < 0 || lp->n < 0 || (lp->n > 0 && (!lp->col_ptr || !lp->c || !lp->col_lower || !lp->col_upper)) ||
         (lp->m > 0 && (!lp->row_lower || !lp->row_upper)) || (lp->nnz > 0 && (!lp->row_ind || !lp->vals)))
         return mpdhg::fail(MPDHG_E_ARG, "write_mps: malformed LP");
+    if (lp->n > 0) {
+        if (lp->col_ptr[0] != 0 || lp->col_ptr[lp->n] != lp->nnz)
+            return mpdhg::fail(MPDHG_E_ARG, "write_mps: col_ptr must start at 0 and end at nnz");
+        for (int64_t j = 0; j < lp->n; ++j)
+            if (lp->col_ptr[j + 1] < lp->col_ptr[j]) return mpdhg::fail(MPDHG_E_ARG, "write_mps: col_ptr not monotone");
+    } else if (lp->nnz != 0) {
+        return mpdhg::fail(MPDHG_E_ARG, "write_mps: nonzeros without columns");
+    }
+    for (int64_t k = 0; k < lp->nnz; ++k)
+        if (lp->row_ind[k] < 0 || lp->row_ind[k] >= lp->m)
+            return mpdhg::fail(MPDHG_E_ARG, "write_mps: row index out of range");
     if (meta && meta->has_quadratic && meta->q_nnz > 0 && (!meta->q_col_ptr || !meta->q_row_ind || !meta->q_vals))
         return mpdhg::fail(MPDHG_E_ARG, "write_mps: quadratic arrays missing");
+    if (meta && meta->has_quadratic && meta->q_nnz > 0) {
+        if (meta->q_col_ptr[0] != 0 || meta->q_col_ptr[lp->n] != meta->q_nnz)
+            return mpdhg::fail(MPDHG_E_ARG, "write_mps: q_col_ptr must start at 0 and end at q_nnz");
+        for (int64_t j = 0; j < lp->n; ++j)
+            if (meta->q_col_ptr[j + 1] < meta->q_col_ptr[j])
+                return mpdhg::fail(MPDHG_E_ARG, "write_mps: q_col_ptr not monotone");
+        for (int64_t k = 0; k < meta->q_nnz; ++k)
+            if (meta->q_row_ind[k] < 0 || meta->q_row_ind[k] >= lp->n)
+                return mpdhg::fail(MPDHG_E_ARG, "write_mps: Q row index out of range");
+    }
     try {
         mpdhg::write_mps(path, *lp, meta);
     } catch (const std::bad_alloc &) {

@@ -1070,10 +1070,13 @@ void mpdhg_polish_opts_default(mpdhg_polish_opts *o) {
 int mpdhg_polish(const mpdhg_lp *lp, const int8_t *col_basis, const int8_t *row_basis,
                  const double *x_start, const mpdhg_polish_opts *opts, mpdhg_polish_result *res) {
     if (!lp || !res) return mpdhg::fail(MPDHG_E_ARG, "polish: NULL argument");
-    if (lp->m < 0 || lp->n < 0 || lp->m > (int64_t{1} << 30) || !lp->col_ptr ||
+    if (lp->m < 0 || lp->n < 0 || lp->nnz < 0 || lp->m > (int64_t{1} << 30) || lp->n > (int64_t{1} << 30) ||
+        !lp->col_ptr ||
         (lp->nnz > 0 && (!lp->row_ind || !lp->vals)) || (lp->n > 0 && (!lp->c || !lp->col_lower || !lp->col_upper)) ||
         (lp->m > 0 && (!lp->row_lower || !lp->row_upper)))
         return mpdhg::fail(MPDHG_E_ARG, "polish: malformed LP");
+    if (lp->col_ptr[0] != 0 || lp->col_ptr[lp->n] != lp->nnz)
+        return mpdhg::fail(MPDHG_E_ARG, "polish: col_ptr must start at 0 and end at nnz");
     for (int64_t j = 0; j < lp->n; ++j)
         if (lp->col_ptr[j + 1] < lp->col_ptr[j]) return mpdhg::fail(MPDHG_E_ARG, "polish: col_ptr not monotone");
     for (int64_t t = 0; t < lp->col_ptr[lp->n]; ++t)

@@ -283,12 +283,31 @@ def test_crossover_and_mps_entry_points_reject_bad_arguments(tmp_path):
     assert (o.max_iterations, o.primal_tol, o.dual_tol, o.refactor_every) == (-1, 1e-6, 1e-6, 100)
     o.primal_tol = 0.0
     assert L.mpdhg_polish(C.byref(lp), None, None, None, C.byref(o), C.byref(res)) == -1
+    o.primal_tol = 1e-6
+    # col_ptr must span exactly nnz entries: a short count is refused before any read
+    lp.nnz -= 1
+    assert L.mpdhg_polish(C.byref(lp), None, None, None, C.byref(o), C.byref(res)) == -1
+    assert "col_ptr must start at 0 and end at nnz" in L.mpdhg_last_error().decode()
+    lp.nnz += 1
+    assert L.mpdhg_polish(C.byref(lp), None, None, None, C.byref(o), C.byref(res)) == 0
     assert L.mpdhg_solve_hybrid(None, None, None, None, None, C.byref(res)) == -1
     assert L.mpdhg_read_mps(None, None) == -1
     out = C.c_void_p()
     assert L.mpdhg_read_mps(str(tmp_path / "none.mps").encode(), C.byref(out)) == -1
     assert "Could not open MPS file" in L.mpdhg_last_error().decode()
     assert L.mpdhg_write_mps(None, C.byref(lp), None, 1) == -1
+    bad = tmp_path / "bad.mps"
+    lp.nnz -= 1
+    assert L.mpdhg_write_mps(str(bad).encode(), C.byref(lp), None, 1) == -1
+    assert "col_ptr must start at 0 and end at nnz" in L.mpdhg_last_error().decode()
+    lp.nnz += 1
+    rows = np.ctypeslib.as_array(lp.row_ind, shape=(lp.nnz,))
+    saved = int(rows[3])
+    rows[3] = 20  # one past the last row
+    assert L.mpdhg_write_mps(str(bad).encode(), C.byref(lp), None, 1) == -1
+    assert "row index out of range" in L.mpdhg_last_error().decode()
+    rows[3] = saved
+    assert not bad.exists()
     # a NULL meta writes a nameless minimisation LP (rows R0.., columns C0..)
     path = tmp_path / "anon.mps"
     assert L.mpdhg_write_mps(str(path).encode(), C.byref(lp), None, 1) == 0
