@@ -17,6 +17,10 @@ namespace mpdhg {
 
 namespace {
 
+// Stream bytes per iteration above which the fused kernels load the matrix
+// stream non-temporally: the size of MI355X's last-level (Infinity) cache.
+constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
+
 struct EnhHost {
     std::vector<int> cptr, cidx, rptr, ridx;
     std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
@@ -275,6 +279,19 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         d_status.upload(run, st);
     }
     TileSet rts = RD.set(), cts = CD.set();
+    // Index and value stream of the listed instances, both directions, per
+    // iteration: above kNtStreamBytes it cannot stay in the caches between
+    // products, so the fused kernels read it non-temporally.
+    auto stream_bytes = [&](auto &&listed) {
+        double b = 0.0;
+        for (int a = 0; a < na; ++a)
+            if (listed(a)) {
+                const int64_t z1 = (a + 1 < na) ? L.nnz_off[static_cast<size_t>(a) + 1] : L.nnz;
+                b += 16.0 * static_cast<double>(z1 - L.nnz_off[static_cast<size_t>(a)]);
+            }
+        return b;
+    };
+    rts.stream_nt = cts.stream_nt = stream_bytes([](int) { return true; }) > kNtStreamBytes;
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -437,6 +454,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     keep_ct.back().upload(nct, st);
                     rts = make_tileset(keep_rt.back(), static_cast<int>(nrt.size()), RD.sc, RD.gather);
                     cts = make_tileset(keep_ct.back(), static_cast<int>(nct.size()), CD.sc, CD.gather);
+                    rts.stream_nt = cts.stream_nt =
+                        stream_bytes([&](int a) { return sv[a] == kRunning; }) > kNtStreamBytes;
                     covered = running;
                     ++lists;
                     host_keep_rt.push_back(std::move(nrt));

@@ -88,8 +88,11 @@ __global__ __launch_bounds__(256) void k_tile_run(Tile *__restrict__ tiles, int 
 // the summation order never depends on the layout or on TN.  `lds` must
 // hold stage_bytes(TN).  The layout is a template parameter (one per tile
 // set) so that each instantiation carries one addressing path: two paths in
-// one kernel cost a third more registers and occupancy with it.
-template <int B, int TN, bool G, typename V, typename X, typename A>
+// one kernel cost a third more registers and occupancy with it.  NT reads
+// the index and value stream with non-temporal loads; it is a template
+// parameter because a uniform branch between the two load forms is merged
+// by the compiler into one form, without the hint.
+template <int B, int TN, bool G, typename V, typename X, typename A, bool NT = false>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw) {
@@ -110,16 +113,28 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
         // pair to the same LDS slot: no per-entry branches, every load issued.
         // Addresses as a uniform base plus an unsigned 32-bit offset, so that
         // every load needs one offset register rather than a 64-bit address.
+        // A stream larger than the caches is read once per product:
+        // non-temporal loads keep it from displacing the gathered vectors in
+        // L2.  A stream that fits stays cached for the next product.
         const int *__restrict__ ib = idx + c0;
         const V *__restrict__ vb = val + c0;
         const uint32_t last = static_cast<uint32_t>(c1 - 1 - c0);
         int id[PT];
         V v[PT];
+        if constexpr (NT) {
 #pragma unroll
-        for (int q = 0; q < PT; ++q) {
-            const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
-            id[q] = ib[o];
-            v[q] = vb[o];
+            for (int q = 0; q < PT; ++q) {
+                const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
+                id[q] = __builtin_nontemporal_load(ib + o);
+                v[q] = __builtin_nontemporal_load(vb + o);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
+                id[q] = ib[o];
+                v[q] = vb[o];
+            }
         }
         X g[PT];
         if (G) {

@@ -101,10 +101,10 @@ struct PrimalArgs {
 
 // Iteration it = *A.itb + off: the iteration number lives on the device, so
 // one recorded check cycle (a HIP graph) replays for every cycle.
-template <int B, int TN, bool G>
-__global__ __launch_bounds__(B) void k_enh_primal(const Tile *__restrict__ tiles, int ntiles,
-                                                       int chunk, const EnhInst *__restrict__ st,
-                                                       PrimalArgs A, int off, int check) {
+template <int B, int TN, bool G, bool NT>
+__device__ __forceinline__ void enh_primal_body(const Tile *__restrict__ tiles, int ntiles, int chunk,
+                                                const EnhInst *__restrict__ st, const PrimalArgs &A,
+                                                int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -127,7 +127,8 @@ __global__ __launch_bounds__(B) void k_enh_primal(const Tile *__restrict__ tiles
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot_rows<B, TN, G, float, float, float>(tr, A.cidx, A.cval, restart ? A.yp : A.y, lds);
+    const float g = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.cidx, A.cval, restart ? A.yp : A.y,
+                                                                     lds);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -176,10 +177,10 @@ struct DualArgs {
     float rho;
 };
 
-template <int B, int TN, bool G>
-__global__ __launch_bounds__(B) void k_enh_dual(const Tile *__restrict__ tiles, int ntiles,
-                                                     int chunk, const EnhInst *__restrict__ st,
-                                                     DualArgs A, int off, int check) {
+template <int B, int TN, bool G, bool NT>
+__device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, int ntiles, int chunk,
+                                              const EnhInst *__restrict__ st, const DualArgs &A,
+                                              int off, int check) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(B) void k_enh_dual(const Tile *__restrict__ tiles, 
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<B, TN, G, float, float, float>(tr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.ridx, A.rval, A.xbar, lds);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -233,6 +234,21 @@ __global__ __launch_bounds__(B) void k_enh_dual(const Tile *__restrict__ tiles, 
         put_group(A.gr + kGrDya2 * A.grn, tl, group_sum_d(e2));
     }
 }
+
+// The fused kernels, with the matrix stream read through the caches (k_enh_*)
+// or non-temporally (k_enh_*_nt, for streams larger than the caches).
+#define MPDHG_ENH_KERNEL(NAME, BODY, ARGS, NT)                                                   \
+    template <int B, int TN, bool G>                                                             \
+    __global__ __launch_bounds__(B) void NAME(const Tile *__restrict__ tiles, int ntiles,       \
+                                              int chunk, const EnhInst *__restrict__ st, ARGS A, \
+                                              int off, int check) {                              \
+        BODY<B, TN, G, NT>(tiles, ntiles, chunk, st, A, off, check);                             \
+    }
+MPDHG_ENH_KERNEL(k_enh_primal, enh_primal_body, PrimalArgs, false)
+MPDHG_ENH_KERNEL(k_enh_primal_nt, enh_primal_body, PrimalArgs, true)
+MPDHG_ENH_KERNEL(k_enh_dual, enh_dual_body, DualArgs, false)
+MPDHG_ENH_KERNEL(k_enh_dual_nt, enh_dual_body, DualArgs, true)
+#undef MPDHG_ENH_KERNEL
 
 // ---- FP64 checks on the original (unscaled) LP ----------------------------
 __device__ __forceinline__ bool fin(double v) { return isfinite(v); }
@@ -589,8 +605,12 @@ void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhP
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
                       a.xbar, a.xbase, a.dc, a.xchk, a.gc, a.gcn, a.itb, a.rho};
     if (!cts.count) return;
-    if (e0)
+    if (e0 && cts.stream_nt)
+        MPDHG_LAUNCH_TN_EV(cts, e0, e1, dev::k_enh_primal_nt, st, A, off, check);
+    else if (e0)
         MPDHG_LAUNCH_TN_EV(cts, e0, e1, dev::k_enh_primal, st, A, off, check);
+    else if (cts.stream_nt)
+        MPDHG_LAUNCH_TN(cts, dev::k_enh_primal_nt, st, A, off, check);
     else
         MPDHG_LAUNCH_TN(cts, dev::k_enh_primal, st, A, off, check);
 }
@@ -599,8 +619,12 @@ void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDua
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
                     a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
     if (!rts.count) return;
-    if (e0)
+    if (e0 && rts.stream_nt)
+        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual_nt, st, A, off, check);
+    else if (e0)
         MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual, st, A, off, check);
+    else if (rts.stream_nt)
+        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual_nt, st, A, off, check);
     else
         MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
 }

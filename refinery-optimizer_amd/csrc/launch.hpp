@@ -16,6 +16,7 @@ struct TileSet {
     int block = 256;      // threads per workgroup (rows per tile at most)
     int tn = 4096;        // nonzeros per LDS chunk; (block, tn) is one of kSchedules
     bool gather = false;  // every tile's chunks in gather order (Tile::base >= 0)
+    bool stream_nt = false;  // index/value stream read non-temporally (fused enhanced kernels)
 };
 
 namespace launch {

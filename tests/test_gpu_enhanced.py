@@ -234,3 +234,20 @@ def test_edge_instances_in_one_batch(need_gpu):
             one = S.solve_gpu_pdhg(p, cfg)
             assert one.simplex_iterations == t.simplex_iterations, p.name
             assert np.array_equal(one.x, t.x) and np.array_equal(one.row_duals, t.row_duals), p.name
+
+
+def test_stream_cache_policy_changes_speed_only(need_gpu):
+    # A batch whose matrix stream exceeds the last-level cache (256 MiB per
+    # iteration, both directions) runs the fused kernels that load the stream
+    # non-temporally; one LP alone runs the cached kernels.  Only the cache
+    # policy differs, so every LP of the batch equals its single solve.
+    big = S.gen_random(1_000_000, 2_400_000, 8, 51)  # 19.2M nonzeros: 307 MB of stream
+    small = [S.gen_random(300, 720, 8, 52), S.gen_staircase(2, 40, 90, 53), toy01()]
+    cfg = enh(tol=1e-6, max_iterations=192)
+    sols = S.solve_batch([big] + small, cfg)
+    assert 16.0 * big.num_nonzeros() > 256 * 2**20
+    for p, s in zip(small, sols[1:]):
+        one = S.solve_gpu_pdhg(p, cfg)
+        assert one.simplex_iterations == s.simplex_iterations and one.restarts == s.restarts, p.name
+        assert np.array_equal(one.x, s.x) and np.array_equal(one.row_duals, s.row_duals), p.name
+        assert one.rel_kkt == s.rel_kkt, p.name
