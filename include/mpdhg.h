@@ -100,6 +100,14 @@ typedef struct mpdhg_config {
                                      when the batch holds >= 262144 nonzeros (smaller batches use
                                      the rule); 2 = always; 0 = never (the rule: the largest
                                      workgroup that still gives >= 1024 tiles). */
+    int32_t cpu_fallback;         /* 0 (default): with no HIP device visible, mpdhg_solve /
+                                     mpdhg_solve_batch / mpdhg_solve_hybrid return
+                                     MPDHG_E_NODEV.  1: they do what GpuSolver::solve_pdhg does
+                                     without a device (gpu_solver.cpp:66-71): print a notice to
+                                     stderr and solve each LP with the host FP64 simplex
+                                     (mpdhg_polish from the slack basis); status, iterations
+                                     (simplex pivots), x, duals and objective then come from the
+                                     simplex, the duals converted to this result's dual_sign. */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite

@@ -3,9 +3,11 @@
 // code plus a thread-local message.
 #include <hip/hip_runtime_api.h>
 
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "mpdhg.h"
@@ -60,6 +62,8 @@ int check_config(const mpdhg_config *cfg) {
         return mpdhg::fail(MPDHG_E_ARG, "spmv_tune must be 0, 1 or 2");
     if (cfg->launch_graph != 0 && cfg->launch_graph != 1)
         return mpdhg::fail(MPDHG_E_ARG, "launch_graph must be 0 or 1");
+    if (cfg->cpu_fallback != 0 && cfg->cpu_fallback != 1)
+        return mpdhg::fail(MPDHG_E_ARG, "cpu_fallback must be 0 or 1");
     return MPDHG_OK;
 }
 
@@ -80,6 +84,58 @@ int make_ctx(int device, mpdhg_ctx **out) {
         *out = c;
         return MPDHG_OK;
     });
+}
+
+int visible_devices() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    return n;
+}
+
+// GpuSolver::solve_pdhg without a device (gpu_solver.cpp:66-71): a notice on
+// stderr, then the host FP64 simplex on each LP (mpdhg_polish from the slack
+// basis).  The simplex's duals carry the simplex sign (reduced cost =
+// c - A^T y_s), so the PDHG-sign duals are -y_s and the reduced costs are the
+// same in both conventions.
+int host_fallback(const mpdhg_lp *lps, int count, const mpdhg_config *cfg, mpdhg_result *res) {
+    std::fprintf(stderr, "[mpdhg] No HIP device found. Falling back to the CPU simplex.\n");
+    mpdhg_polish_opts po;
+    mpdhg_polish_opts_default(&po);
+    for (int q = 0; q < count; ++q) {
+        const double t0 = mpdhg::now_sec();
+        const mpdhg_lp &lp = lps[q];
+        mpdhg_result &r = res[q];
+        if (lp.m < 0 || lp.n < 0) return mpdhg::fail(MPDHG_E_ARG, "malformed LP");
+        const size_t m = static_cast<size_t>(lp.m), n = static_cast<size_t>(lp.n);
+        std::vector<double> x(n), ys(m), rc(n), ax(m);
+        mpdhg_polish_result pr;
+        std::memset(&pr, 0, sizeof(pr));
+        pr.x = x.data();
+        pr.y = ys.data();
+        pr.reduced_costs = rc.data();
+        pr.row_activity = ax.data();
+        const int rc_code = mpdhg_polish(&lp, nullptr, nullptr, nullptr, &po, &pr);
+        if (rc_code != MPDHG_OK) return rc_code;
+        int rc2 = guarded([&] {
+            mpdhg::HostLp h;
+            mpdhg::prepare_structure(lp, h);
+            std::vector<double> y(m);
+            for (size_t i = 0; i < m; ++i) y[i] = -ys[i];
+            mpdhg::host_kkt(lp, h, x, y, r);
+            const double sg = cfg->dual_sign ? -1.0 : 1.0;
+            if (r.x) std::memcpy(r.x, x.data(), n * sizeof(double));
+            if (r.y)
+                for (size_t i = 0; i < m; ++i) r.y[i] = sg * y[i];
+            return MPDHG_OK;
+        });
+        if (rc2) return rc2;
+        r.status = pr.status;
+        r.iterations = pr.iterations;
+        r.primal_objective = pr.objective;
+        r.time_wall_sec = mpdhg::now_sec() - t0;
+        r.solve_sec = pr.time_sec;
+    }
+    return MPDHG_OK;
 }
 
 void clear_result(mpdhg_result *r) {
@@ -164,13 +220,14 @@ int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdh
     if (count < 0 || (count > 0 && (!lps || !res))) return mpdhg::fail(MPDHG_E_ARG, "bad batch");
     int rc = check_config(cfg);
     if (rc) return rc;
+    for (int q = 0; q < count; ++q) clear_result(&res[q]);
+    if (!ctx && cfg->cpu_fallback && visible_devices() <= 0) return host_fallback(lps, count, cfg, res);
     mpdhg_ctx *tmp = nullptr;
     if (!ctx) {
         rc = make_ctx(0, &tmp);
         if (rc) return rc;
         ctx = tmp;
     }
-    for (int q = 0; q < count; ++q) clear_result(&res[q]);
     rc = guarded([&] {
         MPDHG_HIP(hipSetDevice(ctx->c.device));
         if (cfg->mode == MPDHG_MODE_REFERENCE)

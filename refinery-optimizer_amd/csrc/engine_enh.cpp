@@ -351,10 +351,13 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             ctx.events.push_back(e);
         }
     }
-    // Samples are the first non-check iterations after the first cycle, while
-    // every instance of the batch still runs, so that bytes_*_launch (computed
-    // for the whole batch) matches what each sampled launch streams.
+    // Samples are the first non-check iterations after the first cycle on the
+    // full tile lists.  bytes_*_launch is computed for the whole batch, so a
+    // sample counts only if every instance was still iterating when it ran:
+    // its iteration index is kept and checked against each instance's stop
+    // once the loop is over.
     int used = 0;
+    std::vector<int> samp_it(static_cast<size_t>(nsamp));
     Pinned pin(2 * static_cast<size_t>(na) * sizeof(int32_t));
     int32_t *slots = static_cast<int32_t *>(pin.p);
     hipEvent_t poll_ev[2];
@@ -377,16 +380,25 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     };
     // Full cycles replay a HIP graph recorded from issue_cycle(F); it is
     // recorded again after each tile-list compaction (new lists and grids).
+    // A replaced graph may still be executing (the host only waits on the
+    // cycle before it), so it is retired, not destroyed: retired graphs are
+    // destroyed after the final stream synchronisation.
     struct CycleGraph {
         hipGraph_t g = nullptr;
         hipGraphExec_t e = nullptr;
-        void reset() {
-            if (e) (void)hipGraphExecDestroy(e);
-            if (g) (void)hipGraphDestroy(g);
+        std::vector<std::pair<hipGraph_t, hipGraphExec_t>> retired;
+        void retire() {
+            if (e || g) retired.emplace_back(g, e);
             e = nullptr;
             g = nullptr;
         }
-        ~CycleGraph() { reset(); }
+        ~CycleGraph() {
+            retire();
+            for (auto &r : retired) {
+                if (r.second) (void)hipGraphExecDestroy(r.second);
+                if (r.first) (void)hipGraphDestroy(r.first);
+            }
+        }
     } graph;
     int lists = 0, graph_lists = -1;  // tile-list version, and the one the graph was recorded for
     try {
@@ -401,13 +413,13 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     hipEvent_t *e = samp ? ctx.events.data() + 4 * used : nullptr;
                     launch::enh_primal(st, cts, d_inst.get(), P, i, check, e ? e[0] : nullptr, e ? e[1] : nullptr);
                     launch::enh_dual(st, rts, d_inst.get(), D, i, check, e ? e[2] : nullptr, e ? e[3] : nullptr);
-                    if (samp) ++used;
+                    if (samp) samp_it[static_cast<size_t>(used++)] = it + i;
                 }
                 launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
                 launch::enh_advance(st, d_itb.get(), len);
             } else if (cfg.launch_graph && len == F) {
                 if (graph_lists != lists) {
-                    graph.reset();
+                    graph.retire();
                     MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     try {
                         issue_cycle(F);
@@ -473,14 +485,6 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     (void)hipEventDestroy(poll_ev[0]);
     (void)hipEventDestroy(poll_ev[1]);
     const double loop_sec = now_sec() - t_loop;
-    double ms_csc = 0, ms_csr = 0;
-    for (int s = 0; s < used; ++s) {
-        float a = 0, b = 0;
-        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(4 * s)], ctx.events[static_cast<size_t>(4 * s + 1)]));
-        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(4 * s + 2)], ctx.events[static_cast<size_t>(4 * s + 3)]));
-        ms_csc += a;
-        ms_csr += b;
-    }
 
     // ---- results: the best point of each instance ----
     std::vector<double> xb, yb, lb, ab;
@@ -490,6 +494,22 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     d_axbest.download(ab, R, st);
     MPDHG_HIP(hipMemcpyAsync(hs.data(), d_inst.get(), hs.size() * sizeof(EnhInst), hipMemcpyDeviceToHost, st));
     MPDHG_HIP(hipStreamSynchronize(st));
+    // kernel timings of the samples taken while every instance iterated
+    // (an instance that stopped after `iterations` iterations ran none past it)
+    int first_stop = it;
+    for (const EnhInst &s : hs)
+        if (s.status != kRunning) first_stop = std::min(first_stop, static_cast<int>(s.iterations));
+    double ms_csc = 0, ms_csr = 0;
+    int valid = 0;
+    for (int s = 0; s < used; ++s) {
+        if (samp_it[static_cast<size_t>(s)] >= first_stop) continue;
+        float a = 0, b = 0;
+        MPDHG_HIP(hipEventElapsedTime(&a, ctx.events[static_cast<size_t>(4 * s)], ctx.events[static_cast<size_t>(4 * s + 1)]));
+        MPDHG_HIP(hipEventElapsedTime(&b, ctx.events[static_cast<size_t>(4 * s + 2)], ctx.events[static_cast<size_t>(4 * s + 3)]));
+        ms_csc += a;
+        ms_csr += b;
+        ++valid;
+    }
     const double sg = cfg.dual_sign ? -1.0 : 1.0;
     for (int a = 0; a < na; ++a) {
         const int q = act[static_cast<size_t>(a)];
@@ -524,7 +544,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         r.solve_sec = loop_sec;
         r.kernel_ms_spmv_csc = ms_csc;
         r.kernel_ms_spmv_csr = ms_csr;
-        r.kernel_launches_csc = r.kernel_launches_csr = used;
+        r.kernel_launches_csc = r.kernel_launches_csr = valid;
         // K1 streams CSC idx+val, gathers y once, reads c,l,u,x,x0 and writes x,xbar;
         // K2 streams CSR idx+val, gathers xbar once, reads rl,ru,ybar,y,y0, writes y.
         r.bytes_csc_launch = 8.0 * L.nnz + 4.0 * (L.cols + 1) + 4.0 * L.rows + 4.0 * 7 * L.cols;

@@ -6,9 +6,10 @@ py/sih_solver/__init__.py:81-91), so code written against
 ``sih_solver.solve_gpu_pdhg`` runs unchanged.  All arithmetic happens in
 libmpdhg.so (HIP kernels for gfx950 behind the C ABI of include/mpdhg.h).
 
-Deviation from the reference: with no HIP device visible, ``solve_pdhg``
-raises instead of running a CPU dual simplex (gpu_solver.cpp:66-71);
-``GpuSolver.is_cuda_available()`` tells the caller beforehand.
+With no HIP device visible, ``solve_pdhg`` raises by default; the reference
+runs a CPU dual simplex instead (gpu_solver.cpp:66-71), which
+``GpuPdhgConfig(cpu_fallback=1)`` reproduces with the library's host FP64
+simplex.  ``GpuSolver.is_cuda_available()`` tells the caller beforehand.
 """
 from __future__ import annotations
 
@@ -325,6 +326,7 @@ class GpuPdhgConfig:
     launch_graph: int = 1  # replay each full check cycle as a recorded HIP graph
     spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
     spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
+    cpu_fallback: int = 0  # 1: with no HIP device, solve on the host FP64 simplex (gpu_solver.cpp:66-71)
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
@@ -398,6 +400,14 @@ def _ctx() -> _Ctx:
     return _default_ctx
 
 
+def _solve_ctx(config: GpuPdhgConfig):
+    """The context a solve runs on; none when no device is visible and the
+    configuration asks for the host fallback (the library then runs it)."""
+    if config.cpu_fallback and not GpuSolver.is_cuda_available():
+        return None
+    return _ctx()._p
+
+
 def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None) -> List[Solution]:
     """Solve many LPs in one device-resident batch (extension of the reference API)."""
     config = config or GpuPdhgConfig()
@@ -417,7 +427,7 @@ def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = N
         res[q].y = o[1].ctypes.data_as(C.POINTER(C.c_double))
         res[q].reduced_costs = o[2].ctypes.data_as(C.POINTER(C.c_double))
         res[q].row_activity = o[3].ctypes.data_as(C.POINTER(C.c_double))
-    _abi.check(_abi.lib().mpdhg_solve_batch(_ctx()._p, lps, nb, C.byref(cfg), res),
+    _abi.check(_abi.lib().mpdhg_solve_batch(_solve_ctx(config), lps, nb, C.byref(cfg), res),
                "mpdhg_solve_batch")
     return [_result_to_solution(res[q], *outs[q]) for q in range(nb)]
 
@@ -465,7 +475,7 @@ class GpuSolver:
         dp, bp = C.POINTER(C.c_double), C.POINTER(C.c_int8)
         r.x, r.y, r.reduced_costs, r.row_activity = (a.ctypes.data_as(dp) for a in out[:4])
         r.col_basis_out, r.row_basis_out = (a.ctypes.data_as(bp) for a in out[4:])
-        _abi.check(_abi.lib().mpdhg_solve_hybrid(_ctx()._p, C.byref(lp), C.byref(cfg), C.byref(o),
+        _abi.check(_abi.lib().mpdhg_solve_hybrid(_solve_ctx(config), C.byref(lp), C.byref(cfg), C.byref(o),
                                                  C.byref(pd), C.byref(r)), "mpdhg_solve_hybrid")
         s = _result_to_solution(pd, np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
         s.status = SolutionStatus(r.status)

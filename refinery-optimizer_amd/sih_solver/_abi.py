@@ -40,6 +40,7 @@ class MpdhgConfig(C.Structure):
         ("launch_graph", C.c_int32),
         ("spmv_block", C.c_int32),
         ("spmv_tune", C.c_int32),
+        ("cpu_fallback", C.c_int32),
     ]
 
 

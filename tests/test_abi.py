@@ -138,6 +138,31 @@ def test_no_device_fails_loudly():
     assert S.GpuSolver.get_vram_info().total_bytes == 0
 
 
+def test_no_device_cpu_fallback_when_asked():
+    # cpu_fallback = 1 reproduces GpuSolver::solve_pdhg without a device
+    # (gpu_solver.cpp:66-71): the host FP64 simplex solves the LP.  toy01's
+    # optimum is x* = (1.5, 2.5), objective -6.5 (toy_suite_metadata.json:5-12);
+    # both rows are tight with simplex duals (-0.5, -0.5), i.e. (0.5, 0.5) in
+    # the PDHG sign (reduced cost = c + A^T y = 0 on both basic columns).
+    if S.GpuSolver.is_cuda_available():
+        pytest.skip("a HIP device is visible")
+    from lpfix import toy01
+
+    p = toy01()
+    for dual_sign, y_expect in ((0, [0.5, 0.5]), (1, [-0.5, -0.5])):
+        s = S.solve_gpu_pdhg(p, S.GpuPdhgConfig(cpu_fallback=1, dual_sign=dual_sign))
+        assert s.status == S.SolutionStatus.Optimal
+        assert np.allclose(s.x, [1.5, 2.5], atol=1e-12) and math.isclose(s.primal_objective, -6.5)
+        assert np.allclose(s.row_duals, y_expect, atol=1e-12)
+        assert np.allclose(s.reduced_costs, 0.0, atol=1e-12) and np.allclose(s.slack, [4.0, 9.0])
+        assert s.rel_kkt < 1e-12
+    q = S.gen_random(30, 72, 4, 3)
+    a, b = S.solve_batch([q, p], S.GpuPdhgConfig.enhanced(cpu_fallback=1))
+    assert a.status == b.status == S.SolutionStatus.Optimal and a.rel_kkt < 1e-9
+    h = S.solve_gpu_hybrid(p, S.GpuPdhgConfig(cpu_fallback=1))
+    assert h.status == S.SolutionStatus.Optimal and math.isclose(h.primal_objective, -6.5)
+
+
 # ---- generator restatement ---------------------------------------------------
 M64 = (1 << 64) - 1
 

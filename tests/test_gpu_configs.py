@@ -1,0 +1,115 @@
+"""The measured workloads at their full sizes on the GPU.
+
+* One large random equality LP (m=500,000, n=1,200,000, 10 nonzeros per
+  column, 12M nonzeros), reference mode: 50 iterations (two checks) equal
+  the oracle's FP32 restatement of gpu_solver.cpp:227-271 bit for bit, with
+  the searched tile schedules and with a forced (512, 8192) schedule whose
+  CSR chunks fall back to storage order (n = 1.2M exceeds the packed index
+  word's 2^19 entries at 8192-nonzero chunks) while its CSC chunks stay in
+  gather order.  Enhanced mode reaches 1e-4 relative KKT, recomputed in FP64
+  by the oracle from the returned (x, y).
+* The 96-block refinery staircase LP (38,400 x 86,400, 508,143 nonzeros) to 1e-8: the oracle's
+  FP64 KKT of the returned point is <= 1e-8 and the objective equals the
+  committed HiGHS optimum to 1e-7 relative.
+* The 512-LP scenario sweep, batched, to 1e-6: every LP converges, the
+  oracle's FP64 KKT recomputed from each returned (x, y) is <= 1e-6 and agrees
+  with the engine's own figure, and the objectives of a two-family sample
+  (7 random-equality LPs, 12 staircase LPs) equal HiGHS's to 10 x 1e-6
+  relative (tests/golden/highs_sweep.json).
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sih_solver as S
+from sih_solver import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).parent / "golden"
+
+
+def _ref_cfg(**kw):
+    c = S.GpuPdhgConfig()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+@pytest.fixture(scope="module")
+def large_lp():
+    p = S.gen_random(500_000, 1_200_000, 10, 1)
+    assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (500_000, 1_200_000, 12_000_000)
+    return p
+
+
+def test_large_lp_reference_mode_bitexact(need_gpu, large_lp):
+    p = large_lp
+    lp = O.LpArrays.of(p)
+    r = O.pdhg_ref(lp, max_iterations=50, tol_primal=np.float32(0.0), tol_dual=np.float32(0.0),
+                   check_frequency=25)
+    assert r["iterations"] == 51 and r["checks"] == 2
+    for block, chunk in ((0, 0), (512, 8192)):
+        c = _ref_cfg(max_iterations=50, tol_primal=0.0, tol_dual=0.0, spmv_block=block,
+                     spmv_chunk=chunk)
+        sol = S.solve_gpu_pdhg(p, c)
+        label = f"schedule ({sol.csr_block},{sol.csr_chunk}) / ({sol.csc_block},{sol.csc_chunk})"
+        if block:
+            assert (sol.csr_block, sol.csr_chunk, sol.csc_block, sol.csc_chunk) == (block, chunk, block, chunk)
+        assert int(sol.status) == r["status"] == int(S.SolutionStatus.IterationLimit), label
+        assert sol.simplex_iterations == r["iterations"], label
+        assert np.array_equal(sol.x, r["x"]), label
+        assert np.array_equal(sol.row_duals, r["y"]), label
+        assert sol.ref_primal == r["last_primal"] and sol.ref_dual == r["last_dual"], label
+        assert sol.primal_objective == r["objective"], label
+
+
+def test_large_lp_enhanced_mode_to_1e4(need_gpu, large_lp):
+    p = large_lp
+    sol = S.solve_gpu_pdhg(p, S.GpuPdhgConfig.enhanced(tol=1e-4, max_iterations=40000))
+    assert sol.status == S.SolutionStatus.Optimal, sol.rel_kkt
+    k = O.kkt(O.LpArrays.of(p), sol.x, sol.row_duals)
+    assert k["kkt"] <= 1e-4 * 1.0001, k
+    assert math.isclose(k["kkt"], sol.rel_kkt, rel_tol=1e-6), (k["kkt"], sol.rel_kkt)
+
+
+def test_staircase_96_blocks_to_1e8_matches_highs_fixture(need_gpu):
+    fx = json.loads((GOLDEN / "highs_staircase.json").read_text())
+    ref = next(c for c in fx if (c["T"], c["R"], c["C"], c["seed"]) == (96, 400, 900, 1))
+    p = S.gen_staircase(96, 400, 900, 1)
+    assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (ref["m"], ref["n"], ref["nnz"])
+    s = S.solve_gpu_pdhg(p, S.GpuPdhgConfig.enhanced(tol=1e-8, max_iterations=400000))
+    assert s.status == S.SolutionStatus.Optimal and s.rel_kkt <= 1e-8
+    k = O.kkt(O.LpArrays.of(p), s.x, s.row_duals)
+    assert k["kkt"] <= 1e-8 * 1.0001, k
+    assert abs(s.primal_objective - ref["objective"]) <= 1e-7 * abs(ref["objective"])
+
+
+def test_scenario_sweep_512_batched_to_1e6(need_gpu):
+    tol = 1e-6
+    specs = W.sweep_specs(512)
+    probs = W.scenario_sweep(512)
+    assert sum(p.num_nonzeros() for p in probs) > 40_000_000
+    sols = S.solve_batch(probs, S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=100000))
+    worst = 0.0
+    for q, (p, s) in enumerate(zip(probs, sols)):
+        label = (q, specs[q][0], p.num_rows())
+        assert s.status == S.SolutionStatus.Optimal, (label, s.rel_kkt)
+        k = O.kkt(O.LpArrays.of(p), s.x, s.row_duals)
+        assert k["kkt"] <= tol * 1.0001, (label, k)
+        assert math.isclose(k["kkt"], s.rel_kkt, rel_tol=1e-6, abs_tol=1e-13), (label, k["kkt"], s.rel_kkt)
+        worst = max(worst, k["kkt"])
+    fx = json.loads((GOLDEN / "highs_sweep.json").read_text())
+    assert {c["kind"] for c in fx} == {"random", "staircase"}
+    for c in fx:
+        q = c["index"]
+        p, s = probs[q], sols[q]
+        assert specs[q][0] == c["kind"]
+        assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (c["m"], c["n"], c["nnz"]), q
+        assert abs(s.primal_objective - c["objective"]) <= 10 * tol * (1 + abs(c["objective"])), \
+            (q, c["kind"], s.primal_objective, c["objective"])
+    print(f"sweep: 512/512 at <= {tol:g}, worst recomputed KKT {worst:.3e}")
