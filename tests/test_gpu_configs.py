@@ -101,7 +101,12 @@ def test_scenario_sweep_512_batched_to_1e6(need_gpu):
         assert s.status == S.SolutionStatus.Optimal, (label, s.rel_kkt)
         k = O.kkt(O.LpArrays.of(p), s.x, s.row_duals)
         assert k["kkt"] <= tol * 1.0001, (label, k)
-        assert math.isclose(k["kkt"], s.rel_kkt, rel_tol=1e-6, abs_tol=1e-13), (label, k["kkt"], s.rel_kkt)
+        # The gap term divides |pobj - dobj| by (1 + |pobj| + |dobj|); both
+        # objectives are FP64 sums of thousands of terms summed in different
+        # orders here and on the device, so near the optimum (where they
+        # cancel) the two KKT figures agree to ~1e-10 absolute, not to
+        # relative precision.
+        assert math.isclose(k["kkt"], s.rel_kkt, rel_tol=1e-6, abs_tol=1e-10), (label, k["kkt"], s.rel_kkt)
         worst = max(worst, k["kkt"])
     fx = json.loads((GOLDEN / "highs_sweep.json").read_text())
     assert {c["kind"] for c in fx} == {"random", "staircase"}
