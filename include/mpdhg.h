@@ -173,6 +173,28 @@ void mpdhg_destroy(mpdhg_ctx *ctx);
 int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
                 mpdhg_result *res);
 
+/* No-allocation variant of mpdhg_solve.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code running under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  Every
+ * device buffer of the solve is carved from a scratch array of
+ * MPDHG_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends; the call makes no hipMalloc, hipFree, pinned-host
+ * allocation, event or graph (host heap memory is used for the host-side
+ * CSR build).  ctx must come from mpdhg_create (which sets the scratch up on
+ * its device).  The solve is mpdhg_solve's, launched kernel by kernel, with
+ * the tile schedules chosen by rule (spmv_tune and launch_graph are ignored,
+ * profile is off), which changes no result: every result equals
+ * mpdhg_solve's bit for bit.  An LP whose buffers could exceed the scratch is
+ * rejected with MPDHG_E_ARG before any device work; reference mode needs about
+ * 32 B per nonzero and 60 B per row and column, enhanced mode about 64 B per
+ * nonzero and 100 B per row and column (so LPs up to ~1.5M and ~0.7M nonzeros
+ * fit).  Calls on one GPU share its scratch and leave their buffers in it:
+ * calls that overlap on one GPU (from several threads or contexts) race and
+ * must be serialised by the caller. */
+#define MPDHG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
+                        mpdhg_result *res);
+
 /* Batched solve (extension): `count` LPs of any sizes in one device-resident
  * sweep; every launch covers all still-running instances.  Results are
  * per instance, identical to mpdhg_solve on each LP in reference mode. */

@@ -74,6 +74,7 @@ int make_ctx(int device, mpdhg_ctx **out) {
     if (device < 0 || device >= n) return mpdhg::fail(MPDHG_E_ARG, "device index out of range");
     return guarded([&] {
         MPDHG_HIP(hipSetDevice(device));
+        mpdhg::noalloc_setup(device);  // scratch of the no-allocation solve, once per device
         auto *c = new mpdhg_ctx();
         c->c.device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking);
@@ -262,6 +263,36 @@ int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_ad
 int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg, mpdhg_result *res) {
     if (!lp || !res) return mpdhg::fail(MPDHG_E_ARG, "NULL lp or result");
     return mpdhg_solve_batch(ctx, lp, 1, cfg, res);
+}
+
+int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg, mpdhg_result *res) {
+    if (!lp || !res) return mpdhg::fail(MPDHG_E_ARG, "NULL lp or result");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return mpdhg::fail(MPDHG_E_ARG, "malformed LP");
+    const size_t need = mpdhg::noalloc_bytes(cfg->mode, lp->m, lp->n, lp->nnz, cfg->spmv_layout == 0);
+    if (need > MPDHG_NOALLOC_SCRATCH_BYTES)
+        return mpdhg::fail(MPDHG_E_ARG, "LP too large for the no-allocation solve: up to " +
+                                            std::to_string(need) + " bytes of scratch needed, " +
+                                            std::to_string(MPDHG_NOALLOC_SCRATCH_BYTES) + " held");
+    if (!ctx) return mpdhg::fail(MPDHG_E_ARG, "the no-allocation solve needs a context (mpdhg_create)");
+    clear_result(res);
+    mpdhg_config c = *cfg;
+    c.profile = 0;
+    c.launch_graph = 0;
+    c.spmv_tune = 0;
+    return guarded([&] {
+        MPDHG_HIP(hipSetDevice(ctx->c.device));
+        mpdhg::Arena arena;
+        arena.base = static_cast<char *>(mpdhg::noalloc_scratch(ctx->c.device));
+        arena.cap = MPDHG_NOALLOC_SCRATCH_BYTES;
+        mpdhg::ArenaScope scope(&arena);
+        if (c.mode == MPDHG_MODE_REFERENCE)
+            mpdhg::solve_reference(ctx->c, lp, 1, c, res);
+        else
+            mpdhg::solve_enhanced(ctx->c, lp, 1, c, res);
+        return MPDHG_OK;
+    });
 }
 
 }  // extern "C"

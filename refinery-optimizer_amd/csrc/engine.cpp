@@ -15,6 +15,11 @@ double now_sec() {
         .count();
 }
 
+Arena *&thread_arena() {
+    thread_local Arena *a = nullptr;
+    return a;
+}
+
 void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess)
         throw Error(e == hipErrorOutOfMemory ? MPDHG_E_NOMEM : MPDHG_E_HIP,
@@ -339,6 +344,10 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     DBuf<float> d_cval_s, d_rval_s, d_c, d_l, d_u, d_tau, d_rl, d_ru, d_sigma;
     DBuf<float> d_x(C), d_xbar(C), d_xbest(C), d_y(R);
     DBuf<RefInst> d_inst;
+    // Under an arena (the no-allocation entry point) the solve allocates
+    // nothing beyond it: no schedule search, no profiling events, no graph,
+    // and status polls that wait on the stream instead of on events.
+    const bool noalloc = arena_active();
     d_cptr.upload(hh.cptr, st); d_cidx_s.upload(hh.cidx, st); d_cval_s.upload(hh.cval, st);
     d_rptr.upload(hh.rptr, st); d_ridx_s.upload(hh.ridx, st); d_rval_s.upload(hh.rval, st);
     d_c.upload(hh.c, st); d_l.upload(hh.l, st); d_u.upload(hh.u, st); d_tau.upload(hh.tau, st);
@@ -350,7 +359,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     const DirSource RS{d_ridx_s.get(), d_rval_s.get(), nullptr}, CS{d_cidx_s.get(), d_cval_s.get(), nullptr};
     DevDir RD, CD;
     double tune_ms = 0.0;
-    if (tune_wanted(cfg, L.nnz)) {
+    if (tune_wanted(cfg, L.nnz) && !noalloc) {
         std::vector<TuneTrial> trials;
         DBuf<double> d_gtmp(static_cast<size_t>(L.cgroups));
         tune_ms += tune_dir(RP, true, lay, RS, d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, false,
@@ -379,7 +388,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
 
     // ---- profiling samples: HIP events around the fused SpMV launches ----
     const int K = cfg.max_iterations, F = cfg.check_frequency;
-    const int nsamp = cfg.profile ? std::min(256, std::max(K, 0)) : 0;
+    const int nsamp = (cfg.profile && !noalloc) ? std::min(256, std::max(K, 0)) : 0;
     if (nsamp) event_pool(ctx, static_cast<size_t>(4 * nsamp));
     int used = 0;
 
@@ -395,7 +404,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         hipEvent_t e[2] = {nullptr, nullptr};
         ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
     } ev;
-    for (auto &v : ev.e) MPDHG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    if (!noalloc)
+        for (auto &v : ev.e) MPDHG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
     // The iteration count lives on the device (advanced by the last launch of
     // each check cycle), so one recorded cycle -- F fused pairs, the check,
     // the decisions -- replays unchanged as a HIP graph for every full cycle.
@@ -440,7 +450,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         const int len = std::min(F, K - done);
         if (nsamp && used < nsamp) {
             issue_cycle(len, true);  // profiled cycles: eager, with events around the fused pairs
-        } else if (cfg.launch_graph && len == F) {
+        } else if (cfg.launch_graph && len == F && !noalloc) {
             if (!graph.e) {
                 MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                 try {
@@ -462,6 +472,14 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         MPDHG_HIP(hipGetLastError());
         const int cur = nchk & 1;
         MPDHG_HIP(hipMemcpyAsync(snaps[cur], d_inst.get(), hs.size() * sizeof(RefInst), hipMemcpyDeviceToHost, st));
+        if (noalloc) {  // this check's status, at once
+            MPDHG_HIP(hipStreamSynchronize(st));
+            ++nchk;
+            bool any = false;
+            for (int a = 0; a < na; ++a) any = any || (snaps[cur][a].status == kRunning);
+            if (!any) break;
+            continue;
+        }
         MPDHG_HIP(hipEventRecord(ev.e[cur], st));
         if (nchk++ > 0) {
             MPDHG_HIP(hipEventSynchronize(ev.e[cur ^ 1]));

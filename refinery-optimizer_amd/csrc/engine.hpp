@@ -25,7 +25,32 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define MPDHG_HIP(call) ::mpdhg::hip_check((call), #call)
 
-// Owning device buffer.
+// A bump allocator over caller-provided device memory.  While one is set on
+// the calling thread (ArenaScope), every DBuf takes its memory from it and
+// never frees it, and the solves avoid everything else that allocates:
+// pinned host buffers, events, HIP graphs and the schedule search.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes) {
+        const size_t off = (used + 255) & ~static_cast<size_t>(255);
+        if (off > cap || bytes > cap - off)
+            throw Error(MPDHG_E_NOMEM, "scratch exhausted: " + std::to_string(off + bytes) +
+                                           " bytes needed, " + std::to_string(cap) + " held");
+        used = off + bytes;
+        return base + off;
+    }
+};
+Arena *&thread_arena();
+inline bool arena_active() { return thread_arena() != nullptr; }
+struct ArenaScope {
+    explicit ArenaScope(Arena *a) { thread_arena() = a; }
+    ~ArenaScope() { thread_arena() = nullptr; }
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+};
+
+// Owning device buffer (or, under an ArenaScope, a span of the arena).
 template <typename T>
 class DBuf {
   public:
@@ -33,15 +58,29 @@ class DBuf {
     explicit DBuf(size_t n) { resize(n); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), owned_(o.owned_) { o.p_ = nullptr; o.n_ = 0; }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            owned_ = o.owned_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void resize(size_t n) {
         release();
         if (n == 0) n = 1;
+        if (Arena *a = thread_arena()) {
+            p_ = static_cast<T *>(a->take(n * sizeof(T)));
+            n_ = n;
+            owned_ = false;
+            return;
+        }
+        owned_ = true;
         hipError_t e = hipMalloc(reinterpret_cast<void **>(&p_), n * sizeof(T));
         if (e != hipSuccess) {
             p_ = nullptr;
@@ -64,19 +103,31 @@ class DBuf {
 
   private:
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_ && owned_) (void)hipFree(p_);
         p_ = nullptr;
         n_ = 0;
     }
     T *p_ = nullptr;
     size_t n_ = 0;
+    bool owned_ = true;
 };
 
-// Pinned host memory (hipHostMalloc), for asynchronous device-to-host polls.
+// Pinned host memory (hipHostMalloc), for asynchronous device-to-host polls;
+// ordinary host memory under an ArenaScope, whose solves poll synchronously.
 struct Pinned {
     void *p = nullptr;
-    explicit Pinned(size_t bytes) { MPDHG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
-    ~Pinned() { if (p) (void)hipHostFree(p); }
+    std::vector<unsigned char> plain;
+    explicit Pinned(size_t bytes) {
+        if (arena_active()) {
+            plain.resize(bytes ? bytes : 1);
+            p = plain.data();
+        } else {
+            MPDHG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault));
+        }
+    }
+    ~Pinned() {
+        if (p && plain.empty()) (void)hipHostFree(p);
+    }
     Pinned(const Pinned &) = delete;
     Pinned &operator=(const Pinned &) = delete;
 };
@@ -216,5 +267,16 @@ void host_kkt(const mpdhg_lp &lp, const HostLp &h, const std::vector<double> &x,
               const std::vector<double> &y, mpdhg_result &r);
 
 double now_sec();
+
+// The no-allocation solve's device memory (noalloc.hip).  noalloc_setup
+// looks up, with `device` current, the device address of that device's copy
+// of the library's scratch array (once per device; called at context
+// creation); noalloc_scratch returns it, or throws if the lookup failed.
+void noalloc_setup(int device);
+void *noalloc_scratch(int device);
+// Upper bound of the scratch bytes a solve of one LP of this size needs
+// (every device buffer of the mode, with the gather-order copies when
+// `gather`; tiles counted as one per 16-row group), checked before the solve.
+size_t noalloc_bytes(int mode, int64_t m, int64_t n, int64_t nnz, bool gather);
 
 }  // namespace mpdhg

@@ -234,6 +234,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     DBuf<double> d_xbase(C), d_xchk(C), d_lam(C), d_xbest(C), d_lambest(C);
     DBuf<double> d_ybase(R), d_ychk(R), d_ax(R), d_ybest(R), d_axbest(R);
     DBuf<EnhInst> d_inst;
+    // Under an arena (the no-allocation entry point) the solve allocates
+    // nothing beyond it: no schedule search, no profiling events, no graph,
+    // and status polls that wait on the stream instead of on events.
+    const bool noalloc = arena_active();
     // per-group partials of the decision sums and maxima (quantity-major)
     const int gcn = static_cast<int>(L.cgroups), grn = static_cast<int>(L.rgroups);
     DBuf<double> d_gc(static_cast<size_t>(kGcCount) * L.cgroups), d_gr(static_cast<size_t>(kGrCount) * L.rgroups);
@@ -246,7 +250,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const DirSource CS{d_cidx_s.get(), d_cval_s.get(), d_cval64_s.get()};
     DevDir RD, CD;
     double tune_ms = 0.0;
-    if (tune_wanted(cfg, L.nnz)) {
+    if (tune_wanted(cfg, L.nnz) && !noalloc) {
         std::vector<TuneTrial> trials;
         tune_ms += tune_dir(RP, true, lay, RS, d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true, true,
                             d_x.get(), d_y.get(), nullptr, st, RD, trials);
@@ -343,7 +347,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                         d_ybest.get(), d_axbest.get(), d_xbase.get(), d_ybase.get(), d_cs.get(),
                         d_ls.get(), d_us.get(), d_x.get(), d_x0.get(), d_xp.get(), d_rls.get(),
                         d_rus.get(), d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_itb.get()};
-    const int nsamp = cfg.profile ? 256 : 0;
+    const int nsamp = (cfg.profile && !noalloc) ? 256 : 0;
     if (nsamp) {
         while (ctx.events.size() < static_cast<size_t>(4 * nsamp + 4)) {
             hipEvent_t e;
@@ -360,9 +364,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     std::vector<int> samp_it(static_cast<size_t>(nsamp));
     Pinned pin(2 * static_cast<size_t>(na) * sizeof(int32_t));
     int32_t *slots = static_cast<int32_t *>(pin.p);
-    hipEvent_t poll_ev[2];
-    MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
-    MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    if (!noalloc) {
+        MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
+        MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
+    }
     int it = 0, cycle = 0, covered = na;
     bool finished = false;
     std::vector<DBuf<Tile>> keep_rt, keep_ct;  // compacted tile lists, alive until the end
@@ -417,7 +423,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                 }
                 launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
                 launch::enh_advance(st, d_itb.get(), len);
-            } else if (cfg.launch_graph && len == F) {
+            } else if (cfg.launch_graph && len == F && !noalloc) {
                 if (graph_lists != lists) {
                     graph.retire();
                     MPDHG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -442,10 +448,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             const int slot = cycle & 1;
             MPDHG_HIP(hipMemcpyAsync(slots + slot * na, d_status.get(), static_cast<size_t>(na) * sizeof(int32_t),
                                      hipMemcpyDeviceToHost, st));
-            MPDHG_HIP(hipEventRecord(poll_ev[slot], st));
-            if (cycle > 0) {  // inspect the previous cycle while this one runs
-                const int prev = slot ^ 1;
-                MPDHG_HIP(hipEventSynchronize(poll_ev[prev]));
+            if (noalloc) MPDHG_HIP(hipStreamSynchronize(st));  // and inspect this cycle at once
+            else MPDHG_HIP(hipEventRecord(poll_ev[slot], st));
+            if (noalloc || cycle > 0) {  // else inspect the previous cycle while this one runs
+                const int prev = noalloc ? slot : slot ^ 1;
+                if (!noalloc) MPDHG_HIP(hipEventSynchronize(poll_ev[prev]));
                 const int32_t *sv = slots + prev * na;
                 const int running = static_cast<int>(std::count(sv, sv + na, kRunning));
                 finished = (running == 0);
@@ -478,12 +485,12 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         }
         MPDHG_HIP(hipStreamSynchronize(st));
     } catch (...) {
-        (void)hipEventDestroy(poll_ev[0]);
-        (void)hipEventDestroy(poll_ev[1]);
+        for (hipEvent_t e : poll_ev)
+            if (e) (void)hipEventDestroy(e);
         throw;
     }
-    (void)hipEventDestroy(poll_ev[0]);
-    (void)hipEventDestroy(poll_ev[1]);
+    for (hipEvent_t e : poll_ev)
+        if (e) (void)hipEventDestroy(e);
     const double loop_sec = now_sec() - t_loop;
 
     // ---- results: the best point of each instance ----

@@ -176,6 +176,8 @@ SIGNATURES = {
                               C.POINTER(MpdhgResult)]),
     "mpdhg_solve_batch": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgConfig),
                                     C.POINTER(MpdhgResult)]),
+    "mpdhg_solve_noalloc": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.POINTER(MpdhgConfig),
+                                      C.POINTER(MpdhgResult)]),
     "mpdhg_adjoint_batch": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgAdjointIO),
                                       C.c_double, C.c_double, C.c_int]),
     "mpdhg_identify_basis": (C.c_int, [C.POINTER(MpdhgLp), C.POINTER(C.c_double),
