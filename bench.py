@@ -8,10 +8,14 @@ capacity zeroed), ~40.6M nonzeros in total.  A step is one batched solve of
 all 512 LPs to FP64 relative KKT <= 1e-6 (FP32 iterates, FP64 checks and
 iterative refinement).  value = LPs that reached 1e-6 / device-resident
 solve time of the batch.  The same engine solving the LPs one per launch
-sequence ("plain") is timed on every 8th LP of the sweep and reported beside
-it, as is the CPU port of the same algorithm on a bounded sample.
+sequence ("plain") is timed on every 16th LP of each generator family and
+reported beside it, as is the CPU port of the same algorithm on a bounded
+two-family sample (one core and all host cores).
 
 Other workloads:
+  cfg0       the CPU-sized random equality LP m=1,000 n=2,500 8 nnz/column:
+             reference mode to the reference's own stop (P, D <= 1e-4) on
+             the GPU beside the same algorithm in FP64 on one host core.
   large-ref  one random equality LP m=500,000 n=1,200,000 10 nnz/column,
              reference-mode PDHG; a step is one PDHG iteration.
   large-enh  the same LP, enhanced mode to 1e-4 relative KKT.
@@ -37,7 +41,8 @@ profiles/pmc_traffic_sweep512.md, which records the commands.
 `vs_fp64_vertex` (sweep512) states the error of the batched results against
 FP64 optimal vertices: after the timed solves, the crossover (mpdhg_polish,
 host code) starts from each sampled LP's PDHG point and reaches a vertex;
-the relative objective difference is reported (max and median).
+the relative objective difference is reported (max and median, overall and
+per family).
 """
 from __future__ import annotations
 
@@ -119,17 +124,47 @@ def cpu_port_sample(probs, tol, threads):
         return r["kkt"] <= tol
 
     t = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        ok = list(ex.map(one, probs))
+    if threads == 1:
+        ok = [one(p) for p in probs]
+    else:
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            ok = list(ex.map(one, probs))
     return sum(ok), time.perf_counter() - t
+
+
+def host_cores():
+    """Cores this process may use (at most 16, the host share of one GPU) and the host's CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(16, avail), {"nproc": os.cpu_count(), "affinity": avail, "model": model}
+
+
+# The sweep alternates its two generator families (random equality LPs at
+# even indices, staircase LPs at odd ones), so every 16th LP of each family
+# samples both families over their whole size ranges.
+def strata(count=512, step=16):
+    return sorted(list(range(0, count, step)) + list(range(1, count, step)))
+
+
+# Size caps of the vertex check: the crossover's LU fills in on random-family
+# bases (m = 1,914 takes ~6 s on the host), while staircase bases stay sparse.
+VERTEX_M_MAX = {"random": 1200, "staircase": 8000}
 
 
 def vertex_check(probs, sols, tol):
     """Objective error of the batched PDHG results against FP64 optimal vertices:
     the crossover (mpdhg_polish from each PDHG point, host code, outside the
-    timed region) on a sample of the staircase-family LPs.  (Random-family
-    bases fill in almost completely under LU, which makes their crossover too
-    slow for a default run.)"""
+    timed region) on every 8th LP of each family up to the family's size cap."""
     import numpy as np
 
     import sih_solver as S
@@ -137,9 +172,9 @@ def vertex_check(probs, sols, tol):
     from sih_solver import workloads as W
 
     family = [sp[0] for sp in W.sweep_specs(len(probs))]
-    picks = [q for q in range(1, len(probs), 8)
-             if family[q] == "staircase" and probs[q].num_rows() <= 8000][:24]
-    errs, t0, opt = [], time.perf_counter(), 0
+    picks = [q for q in strata(len(probs), 8) if probs[q].num_rows() <= VERTEX_M_MAX[family[q]]]
+    errs = {"random": [], "staircase": []}
+    t0, opt = time.perf_counter(), 0
     for q in picks:
         p, s = probs[q], sols[q]
         cb, rb = S.identify_basis(p, s.x)
@@ -147,14 +182,19 @@ def vertex_check(probs, sols, tol):
         if r.status != S.SolutionStatus.Optimal:
             continue
         opt += 1
-        errs.append(abs(s.primal_objective - r.objective) / max(1.0, abs(r.objective)))
+        errs[family[q]].append(abs(s.primal_objective - r.objective) / max(1.0, abs(r.objective)))
     dt = time.perf_counter() - t0
+    allv = errs["random"] + errs["staircase"]
     log(f"vertex check: {opt}/{len(picks)} FP64 vertices in {dt:.1f}s, objective error "
-        f"max {max(errs):.2e}, median {float(np.median(errs)):.2e}")
-    return {"sample": f"every 8th LP of the staircase family with m <= 8000 ({len(picks)} LPs), FP64 "
+        f"max {max(allv):.2e}, median {float(np.median(allv)):.2e}")
+    fam = {k: {"lps": len(v), "max_rel_objective_error": max(v) if v else None,
+               "median_rel_objective_error": float(np.median(v)) if v else None}
+           for k, v in errs.items()}
+    return {"sample": f"every 8th LP of each family, random family m <= {VERTEX_M_MAX['random']}, "
+                      f"staircase family m <= {VERTEX_M_MAX['staircase']} ({len(picks)} LPs), FP64 "
                       "crossover from the batched PDHG point", "vertices": opt,
-            "max_rel_objective_error": max(errs), "median_rel_objective_error": float(np.median(errs)),
-            "pdhg_tol": tol, "seconds": dt}
+            "max_rel_objective_error": max(allv), "median_rel_objective_error": float(np.median(allv)),
+            "per_family": fam, "pdhg_tol": tol, "seconds": dt}
 
 
 def bench_sweep(args):
@@ -185,17 +225,23 @@ def bench_sweep(args):
     dev = sum(times) / len(times)
     wall = sum(walls) / len(walls)
     rf, spmv = roofline(sols[0], enh_kernels(nnz))
-    # plain: the same engine, one LP per launch sequence, on every 8th LP
-    sample = probs[::8] if not args.no_plain else []
+    # plain: the same engine, one LP per launch sequence, on every 16th LP of
+    # each family (64 LPs, half of each family)
+    family = [sp[0] for sp in W.sweep_specs(len(probs))]
+    sample = strata(len(probs), 16) if not args.no_plain else []
     cfg.profile = False
-    t_plain, ok_plain = 0.0, 0
-    for p in sample:
-        s1 = S.solve_gpu_pdhg(p, cfg)
-        t_plain += s1.time_solver_sec
-        ok_plain += int(s1.status == S.SolutionStatus.Optimal and s1.rel_kkt <= tol)
+    fam_t = {"random": 0.0, "staircase": 0.0}
+    fam_ok = {"random": 0, "staircase": 0}
+    for q in sample:
+        s1 = S.solve_gpu_pdhg(probs[q], cfg)
+        fam_t[family[q]] += s1.time_solver_sec
+        fam_ok[family[q]] += int(s1.status == S.SolutionStatus.Optimal and s1.rel_kkt <= tol)
+    t_plain, ok_plain = sum(fam_t.values()), sum(fam_ok.values())
     plain_rate = ok_plain / t_plain if t_plain > 0 and ok_plain else None
     if plain_rate:
-        log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s; "
+        log(f"plain: {ok_plain}/{len(sample)} in {t_plain:.3f}s -> {plain_rate:.1f} LPs/s "
+            f"(random {fam_ok['random'] / max(fam_t['random'], 1e-9):.1f}, staircase "
+            f"{fam_ok['staircase'] / max(fam_t['staircase'], 1e-9):.1f}); "
             f"batched {conv / dev:.1f} LPs/s ({conv / dev / plain_rate:.2f}x)")
     vertex = None if args.no_vertex else vertex_check(probs, sols, tol)
     capped = args.max_iterations < 100000
@@ -210,24 +256,50 @@ def bench_sweep(args):
         "converged": conv,
         "wall_lps_per_s": conv / wall,
         "plain": None if plain_rate is None else {
-            "lps_per_s": plain_rate, "sample": f"every 8th LP ({len(sample)} LPs), one solve "
+            "lps_per_s": plain_rate, "sample": f"every 16th LP of each family ({len(sample)} LPs, "
+                                               "half random-equality, half staircase), one solve "
                                                "call each", "converged": ok_plain,
-            "seconds": t_plain},
+            "seconds": t_plain,
+            "per_family": {k: {"lps_per_s": fam_ok[k] / fam_t[k] if fam_t[k] > 0 else None,
+                               "converged": fam_ok[k], "seconds": fam_t[k]} for k in fam_t}},
         "batched_over_plain": (conv / dev) / plain_rate if plain_rate else None,
         "roofline": rf, "spmv": spmv,
         "vs_fp64_vertex": vertex,
     }
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        # every 16th LP of the sweep with m <= 8000, about 10-15 s on 16 cores
-        # (the largest LPs alone would take minutes on the host cores)
-        cs = [p for p in probs[::16] if p.num_rows() <= 8000][: args.cpu_lps]
-        ok, dt = cpu_port_sample(cs, tol, cores)
-        out["cpu_baseline"] = {"value": ok / dt, "unit": "LPs/s", "cores": cores, "kind": "port",
-                               "sample": f"{len(cs)} LPs of the sweep (every 16th with m <= 8000), CPU port of the "
-                                         f"same algorithm, {cores} LPs in parallel, {dt:.1f}s, "
-                                         f"{ok} converged"}
+        out["cpu_baseline"] = cpu_baseline_sweep(probs, family, cfg, tol, args)
     return out
+
+
+def cpu_baseline_sweep(probs, family, cfg, tol, args):
+    """The CPU port of the same algorithm on a bounded two-family sample of the
+    sweep: every 16th LP of each family with m <= 2,000 (24 LPs, ~30 s on one
+    core), timed on one core and on all the host cores (LPs in parallel), with
+    the batched GPU solve of the same LPs beside it.  (The largest sweep LPs
+    alone would take minutes each on the host.)"""
+    import sih_solver as S
+
+    idx = [q for q in strata(len(probs), 16) if probs[q].num_rows() <= 2000][: args.cpu_lps]
+    cs = [probs[q] for q in idx]
+    nf = {k: sum(1 for q in idx if family[q] == k) for k in ("random", "staircase")}
+    cores, host = host_cores()
+    ok1, dt1 = cpu_port_sample(cs, tol, 1)
+    okn, dtn = cpu_port_sample(cs, tol, cores)
+    g = S.solve_batch(cs, cfg)
+    okg = sum(1 for x in g if x.status == S.SolutionStatus.Optimal and x.rel_kkt <= tol)
+    log(f"cpu port: {len(cs)} LPs ({nf['random']} random, {nf['staircase']} staircase) "
+        f"{ok1 / dt1:.2f} LPs/s on 1 core ({dt1:.1f}s), {okn / dtn:.2f} LPs/s on {cores} cores "
+        f"({dtn:.1f}s); GPU batched {okg / g[0].time_solver_sec:.1f} LPs/s on the same LPs")
+    return {"value": okn / dtn, "unit": "LPs/s", "cores": cores, "kind": "port",
+            "sample": f"{len(cs)} LPs of the sweep: every 16th LP of each family with m <= 2000 "
+                      f"({nf['random']} random-equality, {nf['staircase']} staircase), CPU port of the "
+                      f"same algorithm (FP32 iterates, FP64 checks) to 1e-6, {cores} LPs in parallel: "
+                      f"{dtn:.1f}s, {okn} converged",
+            "single_core": {"value": ok1 / dt1, "unit": "LPs/s", "seconds": dt1, "converged": ok1},
+            "gpu_same_sample": {"value": okg / g[0].time_solver_sec, "unit": "LPs/s",
+                                "seconds": g[0].time_solver_sec, "converged": okg,
+                                "note": "batched solve of the same LPs, device-resident loop"},
+            "host": host}
 
 
 def bench_large(args, mode):
@@ -238,10 +310,12 @@ def bench_large(args, mode):
     log(f"generated m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()} "
         f"in {time.perf_counter() - t:.1f}s")
     if mode == "ref":
-        cfg = S.GpuPdhgConfig(max_iterations=max(args.warmup, 1), tol_primal=0.0, tol_dual=0.0)
+        cfg = S.GpuPdhgConfig(max_iterations=max(args.warmup, 1), tol_primal=0.0, tol_dual=0.0,
+                              spmv_bands=args.spmv_bands)
         S.solve_gpu_pdhg(p, cfg)
         cfg = S.GpuPdhgConfig(max_iterations=args.steps, tol_primal=0.0, tol_dual=0.0,
-                              spmv_tune=args.spmv_tune, launch_graph=0 if args.no_graph else 1)
+                              spmv_tune=args.spmv_tune, launch_graph=0 if args.no_graph else 1,
+                              spmv_bands=args.spmv_bands)
         sol = S.solve_gpu_pdhg(p, cfg)
         # kernel timings from a separate run whose first cycles launch eagerly with events
         cfg.profile, cfg.max_iterations = True, min(args.steps, 100)
@@ -254,8 +328,10 @@ def bench_large(args, mode):
         # the same iterations with rocSPARSE products (bench/cusparse_pdhg_baseline.cu counterpart)
         from sih_solver import vendor
 
+        out["csr_bands"] = sol.csr_bands
         out["vendor_rocsparse"] = {}
-        for name, mode_v in (("transpose_op", vendor.TRANSPOSE_OP), ("explicit_csc", vendor.EXPLICIT_CSC)):
+        for name, mode_v in () if args.no_vendor else (("transpose_op", vendor.TRANSPOSE_OP),
+                                                       ("explicit_csc", vendor.EXPLICIT_CSC)):
             v = vendor.run(p, args.steps, mode_v)
             out["vendor_rocsparse"][name] = {"us_per_iter": v.us_per_iter, "us_spmv_t": v.us_spmv_t,
                                              "us_spmv": v.us_spmv}
@@ -276,6 +352,68 @@ def bench_large(args, mode):
                                       "to 1e-4 rel KKT, step = one iteration"}}
     out["roofline"] = rf
     out["spmv"] = spmv
+    return out
+
+
+def bench_cfg0(args):
+    """The CPU-sized random equality LP (m=1,000, n=2,500, 8 nonzeros per
+    column, seed 0).  The reference's algorithm to its own stop (P, D <= 1e-4,
+    gpu_solver.cpp:252-270) on the GPU in reference mode (FP32), beside the
+    FP64 CPU path: the same algorithm in double on one host core (the
+    oracle's restatement; the reference has no CPU PDHG of its own).  The
+    enhanced mode to 1e-4 relative KKT is reported too."""
+    import numpy as np
+
+    import sih_solver as S
+
+    p = S.gen_random(1000, 2500, 8, 0)
+    K = 100000
+    cfg = S.GpuPdhgConfig(max_iterations=K)
+    for _ in range(args.warmup):
+        S.solve_gpu_pdhg(p, cfg)
+    ts = []
+    for _ in range(max(args.steps, 1)):
+        sol = S.solve_gpu_pdhg(p, cfg)
+        ts.append(sol.time_solver_sec)
+    dev = float(np.median(ts))
+    log(f"cfg0 GPU reference mode: status {int(sol.status)} in {sol.simplex_iterations} iterations, "
+        f"{dev * 1e3:.1f} ms (P {sol.ref_primal:.2e}, D {sol.ref_dual:.2e}, rel KKT {sol.rel_kkt:.2e})")
+    ecfg = S.GpuPdhgConfig.enhanced(tol=1e-4)
+    S.solve_gpu_pdhg(p, ecfg)
+    enh = S.solve_gpu_pdhg(p, ecfg)
+    log(f"cfg0 GPU enhanced mode: rel KKT {enh.rel_kkt:.2e} in {enh.simplex_iterations} iterations, "
+        f"{enh.time_solver_sec * 1e3:.1f} ms")
+    out = {"metric": "seconds to the reference's stop (P, D <= 1e-4), config-0 LP",
+           "value": dev, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * dev,
+           "dtype": "fp32",
+           "config": {"workload": "cfg0: random equality LP m=1000 n=2500 8 nnz/col seed 0, "
+                                  "reference-mode PDHG to P, D <= 1e-4; step = one solve"},
+           "status": int(sol.status), "iterations": sol.simplex_iterations,
+           "ref_primal": sol.ref_primal, "ref_dual": sol.ref_dual, "rel_kkt": sol.rel_kkt,
+           "enhanced": {"seconds": enh.time_solver_sec, "iterations": enh.simplex_iterations,
+                        "rel_kkt": enh.rel_kkt, "status": int(enh.status), "tol": 1e-4}}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        lp = O.LpArrays.of(p)
+        O.pdhg_ref(lp, max_iterations=K, fp64=True)  # warm-up
+        runs = []
+        for _ in range(5):
+            t = time.perf_counter()
+            r = O.pdhg_ref(lp, max_iterations=K, fp64=True)
+            runs.append(time.perf_counter() - t)
+        k = O.kkt(lp, r["x"], r["y"])
+        cpu = float(np.median(runs))
+        log(f"cfg0 CPU FP64 path: status {r['status']} in {r['iterations']} iterations, "
+            f"{cpu * 1e3:.1f} ms median of 5 (rel KKT {k['kkt']:.2e})")
+        _, host = host_cores()
+        out["cpu_baseline"] = {"value": cpu, "unit": "s", "cores": 1, "kind": "port",
+                               "sample": "the whole config-0 solve: the reference algorithm in FP64 on "
+                                         "one host core (int32 CSC/CSR, sequential row sums) to P, D <= "
+                                         f"1e-4; median of 5 after a warm-up; status {r['status']}, "
+                                         f"{r['iterations']} iterations, rel KKT {k['kkt']:.2e}",
+                               "host": host}
     return out
 
 
@@ -587,7 +725,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
-                    choices=["sweep512", "large-ref", "large-enh", "staircase96", "adjoint64",
+                    choices=["sweep512", "cfg0", "large-ref", "large-enh", "staircase96", "adjoint64",
                              "hybrid48", "copy"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
@@ -604,6 +742,10 @@ def main():
                     help="sweep512: force the tile workgroup size of both directions (0 = auto)")
     ap.add_argument("--spmv-chunk", type=int, default=0, choices=(0, 1024, 2048, 4096, 8192),
                     help="sweep512: force the tile chunk of both directions (0 = auto)")
+    ap.add_argument("--spmv-bands", type=int, default=0,
+                    help="large-ref: column bands of the A xbar product (mpdhg_config.spmv_bands: "
+                         "0 = rule, 1 = none, k = k bands)")
+    ap.add_argument("--no-vendor", action="store_true", help="large-ref: skip the rocSPARSE baseline")
     ap.add_argument("--blocks", type=int, default=48, choices=(12, 48, 96),
                     help="hybrid48: time blocks of the staircase LP (HiGHS fixtures exist for these)")
     ap.add_argument("--no-graph", action="store_true",
@@ -619,6 +761,8 @@ def main():
         out = bench_sweep(args)
     elif args.workload == "copy":
         out = bench_copy(args)
+    elif args.workload == "cfg0":
+        out = bench_cfg0(args)
     elif args.workload == "staircase96":
         out = bench_staircase(args)
     elif args.workload == "adjoint64":

@@ -108,6 +108,13 @@ typedef struct mpdhg_config {
                                      (mpdhg_polish from the slack basis); status, iterations
                                      (simplex pivots), x, duals and objective then come from the
                                      simplex, the duals converted to this result's dual_sign. */
+    int32_t spmv_bands;           /* reference mode, one LP: split the A xbar product (CSR) into
+                                     this many column bands, one launch each, so that each band's
+                                     slice of xbar stays in an XCD's 4 MB L2 (rows carry their
+                                     running sums from band to band in storage order: results do
+                                     not change).  0 (default) = by rule (bands of at most 3 MiB
+                                     of xbar once xbar exceeds 3 MiB), 1 = no bands, k = k bands
+                                     (at most 16). */
 } mpdhg_config;
 
 /* A model::Problem's LP data (problem.hpp:37-137) in CSC form.  Infinite
@@ -151,6 +158,7 @@ typedef struct mpdhg_result {
     int32_t csr_block, csr_chunk;  /* tile schedule the solve used for A x (CSR) ... */
     int32_t csc_block, csc_chunk;  /* ... and for A^T y (CSC) */
     double tune_ms;                /* host + device time of the schedule search (0: rule) */
+    int32_t csr_bands;             /* column bands of the A xbar product (1: none) */
 } mpdhg_result;
 
 typedef struct mpdhg_ctx mpdhg_ctx;
@@ -182,8 +190,9 @@ int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
  * allocation, event or graph (host heap memory is used for the host-side
  * CSR build).  ctx must come from mpdhg_create (which sets the scratch up on
  * its device).  The solve is mpdhg_solve's, launched kernel by kernel, with
- * the tile schedules chosen by rule (spmv_tune and launch_graph are ignored,
- * profile is off), which changes no result: every result equals
+ * the tile schedules chosen by rule and no column bands (spmv_tune,
+ * spmv_bands and launch_graph are ignored, profile is off), which changes no
+ * result: every result equals
  * mpdhg_solve's bit for bit.  An LP whose buffers could exceed the scratch is
  * rejected with MPDHG_E_ARG before any device work; reference mode needs about
  * 32 B per nonzero and 60 B per row and column, enhanced mode about 64 B per

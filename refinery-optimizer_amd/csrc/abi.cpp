@@ -64,6 +64,8 @@ int check_config(const mpdhg_config *cfg) {
         return mpdhg::fail(MPDHG_E_ARG, "launch_graph must be 0 or 1");
     if (cfg->cpu_fallback != 0 && cfg->cpu_fallback != 1)
         return mpdhg::fail(MPDHG_E_ARG, "cpu_fallback must be 0 or 1");
+    if (cfg->spmv_bands < 0 || cfg->spmv_bands > mpdhg::kMaxBands)
+        return mpdhg::fail(MPDHG_E_ARG, "spmv_bands must be in [0, 16]");
     return MPDHG_OK;
 }
 
@@ -147,6 +149,7 @@ void clear_result(mpdhg_result *r) {
     r->reduced_costs = rc;
     r->row_activity = ra;
     r->status = MPDHG_STATUS_UNKNOWN;
+    r->csr_bands = 1;
 }
 
 }  // namespace
@@ -281,6 +284,7 @@ int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *
     c.profile = 0;
     c.launch_graph = 0;
     c.spmv_tune = 0;
+    c.spmv_bands = 1;
     return guarded([&] {
         MPDHG_HIP(hipSetDevice(ctx->c.device));
         mpdhg::Arena arena;

@@ -167,6 +167,16 @@ TileSet make_tileset(const DBuf<Tile> &d, int count, Schedule sc, bool gather);
 constexpr int64_t kTilesWanted = 1024;
 Schedule choose_schedule(int64_t nnz, int64_t rows, int forced_block, int forced_tn);
 
+// Column bands of a reference-mode CSR product (mpdhg_config::spmv_bands):
+// by rule, bands of at most kBandBytes of the gathered vector once it
+// exceeds kBandMinBytes (an XCD's L2 holds 4 MB).  Measured on the 12M-
+// nonzero LP (xbar 4.8 MB): 2 bands 65.7 us per product, none 75.6, 3 bands
+// 67.4, 4 bands 71.5 (each pass rereads the row pointers and the running sums).
+constexpr int kMaxBands = 16;
+constexpr int64_t kBandBytes = 3 << 20;
+constexpr int64_t kBandMinBytes = 3 << 20;
+int band_count(int forced, int64_t vec_len);
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;

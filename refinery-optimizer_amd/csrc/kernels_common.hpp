@@ -92,10 +92,14 @@ __global__ __launch_bounds__(256) void k_tile_run(Tile *__restrict__ tiles, int 
 // the index and value stream with non-temporal loads; it is a template
 // parameter because a uniform branch between the two load forms is merged
 // by the compiler into one form, without the hint.
+// `init` starts each lane's accumulator: a row split into column bands
+// (products over one band of columns per launch) continues its sequential
+// FMA chain from the sum the previous band stored, so the summation order is
+// still the row's storage order.
 template <int B, int TN, bool G, typename V, typename X, typename A, bool NT = false>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
-                                           void *lds_raw) {
+                                           void *lds_raw, A init = A(0)) {
     static_assert(valid_schedule(B, TN), "tile schedule");
     constexpr int kBits = ilog2(TN);
     constexpr uint32_t kMask = static_cast<uint32_t>(TN) - 1u;
@@ -106,7 +110,7 @@ __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__rest
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     const int tid = static_cast<int>(threadIdx.x);
     const int k0 = tr.k0, k1 = tr.k1, lo = tr.lo, hi = tr.hi;
-    A sum = A(0);
+    A sum = init;
     for (int c0 = lo; c0 < hi; c0 += TN) {
         const int c1 = min(hi, c0 + TN);
         // Lanes past the chunk end re-read its last entry and rewrite the same

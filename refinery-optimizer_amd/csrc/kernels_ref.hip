@@ -66,6 +66,43 @@ __global__ __launch_bounds__(B) void k_ref_dual(
     }
 }
 
+// cuda_pdhg.cu:14-30 (A xbar) over one band of columns: band pass `first`
+// starts each row's sum at 0, later passes continue it from hpart; every pass
+// but the `last` stores the running sums in hpart, and the last one applies
+// the Moreau dual step (:82-103) of k_ref_dual.  Each band's nonzeros of a
+// row are the row's next columns in storage order, so the sums equal
+// k_ref_dual's bit for bit.  rptr / ridx / rval: the band's own CSR.
+template <int B, int TN, bool G>
+__global__ __launch_bounds__(B) void k_ref_dual_band(
+    const Tile *__restrict__ tiles, int ntiles, int chunk, const RefInst *__restrict__ st,
+    const int *__restrict__ rptr, const int *__restrict__ ridx, const float *__restrict__ rval,
+    const float *__restrict__ xbar, float *__restrict__ hpart, int first, int last,
+    const float *__restrict__ rl, const float *__restrict__ ru, const float *__restrict__ sigma,
+    float *__restrict__ y) {
+    __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
+    const int t = xcd_tile(blockIdx.x, chunk);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    if (!tl.run || st[tl.inst].status != kRunning) return;
+    const TileRows tr = tile_rows(tl, rptr);
+    const int i = tl.begin + static_cast<int>(threadIdx.x);
+    const bool mine = i < tl.end;
+    const float h0 = (mine && !first) ? hpart[i] : 0.0f;
+    const float h = tile_dot_rows<B, TN, G, float, float, float>(tr, ridx, rval, xbar, lds, h0);
+    if (!mine) return;
+    if (!last) {
+        hpart[i] = h;
+        return;
+    }
+    const float sg = sigma[i];
+    const float w = h + (y[i] / sg);
+    float p = w;
+    const float lo = rl[i], hi = ru[i];
+    if (p < lo) p = lo;
+    if (p > hi) p = hi;
+    y[i] = sg * (w - p);
+}
+
 // cuda_pdhg.cu:114-129 row residuals of A x, max-reduced per tile (the
 // reference copies m floats to the host and reduces there, :249-255).
 template <int B, int TN, bool G>
@@ -204,6 +241,18 @@ void ref_dual(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rp
         MPDHG_LAUNCH_TN_EV(ts, e0, e1, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
     else
         MPDHG_LAUNCH_TN(ts, dev::k_ref_dual, st, rptr, ridx, rval, xbar, rl, ru, sigma, y);
+}
+
+void ref_dual_band(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
+                   const int *ridx, const float *rval, const float *xbar, float *hpart, int first,
+                   int last, const float *rl, const float *ru, const float *sigma, float *y,
+                   hipEvent_t e0, hipEvent_t e1) {
+    if (e0 || e1)
+        MPDHG_LAUNCH_TN_EV(ts, e0, e1, dev::k_ref_dual_band, st, rptr, ridx, rval, xbar, hpart, first,
+                           last, rl, ru, sigma, y);
+    else
+        MPDHG_LAUNCH_TN(ts, dev::k_ref_dual_band, st, rptr, ridx, rval, xbar, hpart, first, last, rl,
+                        ru, sigma, y);
 }
 
 void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,

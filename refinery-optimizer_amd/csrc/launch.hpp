@@ -30,6 +30,12 @@ void ref_dual(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rp
               const int *ridx, const float *rval, const float *xbar, const float *rl,
               const float *ru, const float *sigma, float *y, hipEvent_t e0 = nullptr,
               hipEvent_t e1 = nullptr);
+// A xbar over one column band (k_ref_dual_band): first / last band pass of
+// the product; e0 / e1 (either may be null) at the kernel's own start / end.
+void ref_dual_band(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
+                   const int *ridx, const float *rval, const float *xbar, float *hpart, int first,
+                   int last, const float *rl, const float *ru, const float *sigma, float *y,
+                   hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void ref_rowres(hipStream_t s, const TileSet &ts, const RefInst *st, const int *rptr,
                 const int *ridx, const float *rval, const float *x, const float *rl,
                 const float *ru, float *part);

@@ -290,6 +290,7 @@ class Solution:
     csc_block: int = 0  # and for A^T y
     csc_chunk: int = 0
     tune_ms: float = 0.0  # time of the schedule search
+    csr_bands: int = 1  # column bands of the A xbar product
 
     def is_optimal(self) -> bool:
         return self.status == SolutionStatus.Optimal
@@ -327,6 +328,7 @@ class GpuPdhgConfig:
     spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
     spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
     cpu_fallback: int = 0  # 1: with no HIP device, solve on the host FP64 simplex (gpu_solver.cpp:66-71)
+    spmv_bands: int = 0  # reference mode, one LP: column bands of A xbar (0 rule, 1 none, k bands)
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
@@ -366,7 +368,7 @@ def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
                  "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
                  "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
                  "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight",
-                 "csr_block", "csr_chunk", "csc_block", "csc_chunk", "tune_ms"):
+                 "csr_block", "csr_chunk", "csc_block", "csc_chunk", "tune_ms", "csr_bands"):
         setattr(s, name, getattr(r, name))
     return s
 

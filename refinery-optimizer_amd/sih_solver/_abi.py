@@ -41,6 +41,7 @@ class MpdhgConfig(C.Structure):
         ("spmv_block", C.c_int32),
         ("spmv_tune", C.c_int32),
         ("cpu_fallback", C.c_int32),
+        ("spmv_bands", C.c_int32),
     ]
 
 
@@ -94,6 +95,7 @@ class MpdhgResult(C.Structure):
         ("csc_block", C.c_int32),
         ("csc_chunk", C.c_int32),
         ("tune_ms", C.c_double),
+        ("csr_bands", C.c_int32),
     ]
 
 

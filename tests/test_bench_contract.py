@@ -41,3 +41,17 @@ def test_committed_traffic_matches_the_counter_table():
         fetch, write = (float(v.replace(",", "")) for v in row.groups())
         assert abs(b.measured_traffic(k) - (2.0 * fetch + write) * 1024) < 1024, k
     assert b.measured_traffic("k_no_such_kernel") is None
+
+
+def test_comparison_samples_draw_both_families_alike():
+    # The sweep alternates its generator families, so the plain, CPU and
+    # vertex samples are stratified: every k-th LP of each family.
+    import sih_solver  # noqa: F401  (conftest puts the package on the path)
+    from sih_solver import workloads as W
+
+    b = _bench()
+    fam = [sp[0] for sp in W.sweep_specs(512)]
+    for step, per in ((16, 32), (8, 64)):
+        idx = b.strata(512, step)
+        assert len(idx) == 2 * per and len(set(idx)) == len(idx)
+        assert sum(fam[q] == "random" for q in idx) == sum(fam[q] == "staircase" for q in idx) == per

@@ -6,7 +6,8 @@
   the searched tile schedules and with a forced (512, 8192) schedule whose
   CSR chunks fall back to storage order (n = 1.2M exceeds the packed index
   word's 2^19 entries at 8192-nonzero chunks) while its CSC chunks stay in
-  gather order.  Enhanced mode reaches 1e-4 relative KKT, recomputed in FP64
+  gather order; with the A xbar product in the rule's 2 column bands, in 3,
+  and in none.  Enhanced mode reaches 1e-4 relative KKT, recomputed in FP64
   by the oracle from the returned (x, y).
 * The 96-block refinery staircase LP (38,400 x 86,400, 508,143 nonzeros) to 1e-8: the oracle's
   FP64 KKT of the returned point is <= 1e-8 and the objective equals the
@@ -53,13 +54,16 @@ def test_large_lp_reference_mode_bitexact(need_gpu, large_lp):
     r = O.pdhg_ref(lp, max_iterations=50, tol_primal=np.float32(0.0), tol_dual=np.float32(0.0),
                    check_frequency=25)
     assert r["iterations"] == 51 and r["checks"] == 2
-    for block, chunk in ((0, 0), (512, 8192)):
+    for block, chunk, bands in ((0, 0, 0), (512, 8192, 0), (0, 0, 1), (256, 2048, 3)):
         c = _ref_cfg(max_iterations=50, tol_primal=0.0, tol_dual=0.0, spmv_block=block,
-                     spmv_chunk=chunk)
+                     spmv_chunk=chunk, spmv_bands=bands)
         sol = S.solve_gpu_pdhg(p, c)
-        label = f"schedule ({sol.csr_block},{sol.csr_chunk}) / ({sol.csc_block},{sol.csc_chunk})"
+        label = (f"schedule ({sol.csr_block},{sol.csr_chunk}) / ({sol.csc_block},{sol.csc_chunk}), "
+                 f"{sol.csr_bands} bands")
         if block:
             assert (sol.csr_block, sol.csr_chunk, sol.csc_block, sol.csc_chunk) == (block, chunk, block, chunk)
+        # by rule, xbar (4.8 MB) is split into bands of at most 3 MiB
+        assert sol.csr_bands == (bands if bands else 2), label
         assert int(sol.status) == r["status"] == int(S.SolutionStatus.IterationLimit), label
         assert sol.simplex_iterations == r["iterations"], label
         assert np.array_equal(sol.x, r["x"]), label

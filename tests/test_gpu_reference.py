@@ -64,7 +64,16 @@ def test_random_equality_lp_m1000_bitexact(need_gpu):
     p = S.gen_random(1000, 2500, 8, 0)
     c = cfg()  # reference defaults: 4000 iterations, check every 25
     sol = S.solve_gpu_pdhg(p, c)
-    assert_bitexact(sol, oracle_of(p, c), "random m=1000")
+    assert sol.csr_bands == 1  # xbar of 10 kB: no column bands by rule
+    r = oracle_of(p, c)
+    assert_bitexact(sol, r, "random m=1000")
+    # The A xbar product split into column bands (each row's sum carried from
+    # band to band in storage order): the same bits, including bands that
+    # hold no nonzero of some rows and 16 bands of ~156 columns.
+    for bands in (2, 5, 16):
+        sb = S.solve_gpu_pdhg(p, cfg(spmv_bands=bands))
+        assert sb.csr_bands == bands
+        assert_bitexact(sb, r, f"random m=1000, {bands} bands")
 
 
 def _mixed_batch():
