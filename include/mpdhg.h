@@ -108,6 +108,12 @@ typedef struct mpdhg_config {
                                      (mpdhg_polish from the slack basis); status, iterations
                                      (simplex pivots), x, duals and objective then come from the
                                      simplex, the duals converted to this result's dual_sign. */
+    int32_t vertex_check;         /* 0 (default): off.  k > 0: after the solve, every LP with at
+                                     most k rows is taken from its PDHG point to an FP64 optimal
+                                     vertex by the host crossover (mpdhg_identify_basis, then
+                                     mpdhg_polish from that basis and point; 60 s per LP at most),
+                                     and its result carries the vertex's status and objective and
+                                     the PDHG objective's relative error against it. */
     int32_t spmv_bands;           /* reference mode, one LP: split the A xbar product (CSR) into
                                      this many column bands, one launch each, so that each band's
                                      slice of xbar stays in an XCD's 4 MB L2 (rows carry their
@@ -159,6 +165,10 @@ typedef struct mpdhg_result {
     int32_t csc_block, csc_chunk;  /* ... and for A^T y (CSC) */
     double tune_ms;                /* host + device time of the schedule search (0: rule) */
     int32_t csr_bands;             /* column bands of the A xbar product (1: none) */
+    int32_t vertex_status;         /* vertex_check: MPDHG_STATUS_* of the crossover (UNKNOWN: not run) */
+    double vertex_objective;       /* vertex_check: c^T x at the FP64 vertex (NaN: not run) */
+    double vertex_rel_error;       /* vertex_check: |primal_objective - vertex_objective| /
+                                      max(1, |vertex_objective|) (NaN unless the vertex is optimal) */
 } mpdhg_result;
 
 typedef struct mpdhg_ctx mpdhg_ctx;
@@ -209,6 +219,24 @@ int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *
  * per instance, identical to mpdhg_solve on each LP in reference mode. */
 int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count,
                       const mpdhg_config *cfg, mpdhg_result *res);
+
+/* Warm start of one LP (extension): the point a scenario re-solve starts from,
+ * usually the solution of the base LP (py/refinery/reoptimization.py:31-79
+ * re-solves each perturbation from the previous solution's basis). */
+typedef struct mpdhg_warm_start {
+    const double *x;       /* [n] primal start, or NULL: clamp(0, l, u) */
+    const double *y;       /* [m] row duals in the config's dual_sign, or NULL: 0 */
+    double primal_weight;  /* enhanced: omega to start from; <= 0: ||c|| / ||b|| of the scaled LP */
+    double step_size;      /* enhanced: eta; <= 0: the power iteration (every LP of the batch
+                              must give one for the power iteration to be skipped) */
+} mpdhg_warm_start;
+/* mpdhg_solve_batch with a warm start per LP (warm may be NULL: all cold).
+ * Reference mode starts from x (in FP32, clamped to the bounds) and y
+ * instead of (clamp(0, l, u), 0) and keeps gpu_solver.cpp:227-271 otherwise;
+ * enhanced mode starts its iterates and its Halpern anchor at the scaled x
+ * and y, with the given primal weight and step size. */
+int mpdhg_solve_warm(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdhg_config *cfg,
+                     const mpdhg_warm_start *warm, mpdhg_result *res);
 
 /* Adjoint of the PDHG fixed point (extension; no reference counterpart).
  * For L(x*) with upstream gradient w = dL/dx* at a solution (x, y) of `lp`:
@@ -320,6 +348,32 @@ int mpdhg_gen_staircase(int32_t T, int32_t R, int32_t C, uint64_t seed, mpdhg_lp
  * zero_capacity and the LP has capacity rows, one of them gets ru = 0. */
 int mpdhg_gen_scenario(const mpdhg_lp_owned *base, uint64_t seed, double shock_sigma,
                        int32_t zero_capacity, mpdhg_lp_owned **out);
+/* Scenario perturbations of a staircase LP, restating the perturbations of
+ * py/refinery/parameterized_generator.py:28-133 on the staircase model
+ * (crude, sale and inventory columns; balance, blend and capacity rows).
+ * Each multiplier is 1 + clip(N(0, std), lo, hi) with the reference's clips.
+ * Costs and bounds of other LPs only take the lognormal shock. */
+#define MPDHG_REGIME_NORMAL 0
+#define MPDHG_REGIME_HIGH_CRUDE 1     /* crude costs x 1.30 */
+#define MPDHG_REGIME_DIESEL_SURGE 2   /* sale prices x 1.30; every 4th product ("diesel") x 1.25
+                                         more and its demand cap x 1.60/1.25 (the reference raises
+                                         every cap by 1.25 and diesel caps by 1.60; the common 1.25
+                                         is left out) */
+#define MPDHG_REGIME_TURNAROUND 3     /* the first capacity row of each block x 0.65 */
+#define MPDHG_REGIME_SOUR_CRUDE 4     /* crude costs x 1.10, first crude x 0.85, second x 1.25 */
+#define MPDHG_REGIME_SWEET_SHORTAGE 5 /* the second crude's supply x 0.35 */
+typedef struct mpdhg_scenario_opts {
+    double shock_sigma;    /* every cost x exp(shock_sigma N(0,1)) (the sweep's price shock) */
+    double price_std;      /* crude costs clip [-0.30, 0.40], sale prices clip [-0.25, 0.35] */
+    double supply_std;     /* crude column upper bounds, clip [-0.40, 0.50] */
+    double capacity_std;   /* capacity rows' upper bounds, clip [-0.25, 0.30] */
+    double demand_std;     /* sale column upper bounds (demand caps), clip [-0.20, 0.25] */
+    int32_t regime;        /* MPDHG_REGIME_* */
+    int32_t zero_capacity; /* 1: one random capacity row gets ru = 0 (a unit outage) */
+} mpdhg_scenario_opts;
+void mpdhg_scenario_opts_default(mpdhg_scenario_opts *opts); /* all zero: normal regime, no change */
+int mpdhg_gen_scenario_ex(const mpdhg_lp_owned *base, uint64_t seed, const mpdhg_scenario_opts *opts,
+                          mpdhg_lp_owned **out);
 /* Borrowed view of an owned LP (valid until mpdhg_lp_free). */
 const mpdhg_lp *mpdhg_lp_view(const mpdhg_lp_owned *lp);
 /* Row kinds of a staircase LP (0 balance, 1 blend, 2 capacity), else NULL. */
@@ -363,6 +417,47 @@ const mpdhg_lp_meta *mpdhg_lp_meta_view(const mpdhg_lp_owned *lp);
  * the reference, does not change the output. */
 int mpdhg_write_mps(const char *path, const mpdhg_lp *lp, const mpdhg_lp_meta *meta,
                     int32_t free_format);
+
+/* ---- solution checker: checker::check_solution (checker/checker.cpp:9-268) */
+/* The candidate a checker call verifies (the model::Solution fields it reads).
+ * Duals in the checker's sign: reduced cost = c - A^T y, the negative of the
+ * PDHG sign (solve with dual_sign = 1, or negate y).  A NULL vector stands
+ * for an empty one. */
+typedef struct mpdhg_check_input {
+    int32_t status;              /* MPDHG_STATUS_* */
+    const double *x;             /* [n] */
+    const double *row_duals;     /* [m] */
+    const double *reduced_costs; /* [n] */
+    double primal_objective;
+    const double *ray;           /* Farkas ray [m] (Infeasible) or ray [n] (Unbounded) */
+    int64_t ray_len;
+} mpdhg_check_input;
+/* checker::CheckResult (checker.hpp:9-34); summary holds the reference's text. */
+typedef struct mpdhg_check_result {
+    int32_t is_primal_feasible;
+    double max_row_violation, max_col_bound_violation, max_primal_residual;
+    int32_t is_dual_feasible;
+    double max_dual_residual;
+    int32_t is_complementary;
+    double max_complementarity_slack;
+    int32_t is_integrality_satisfied;
+    double max_integrality_violation;
+    double evaluated_objective, objective_discrepancy;
+    int32_t is_certificate_valid;
+    double certificate_violation;
+    int32_t all_checks_passed;
+    char summary[256];
+} mpdhg_check_result;
+/* Independent FP64 verification of a candidate solution of an LP, with the
+ * reference's semantics and arithmetic order (A x by rows, A^T y by columns,
+ * in storage order): primal row and bound violations, integrality, the
+ * objective (with meta's offset and quadratic term), and for continuous
+ * problems with duals the stationarity c + Qx - A^T y - s and the
+ * complementarity |y_i| min(|Ax_i - rl_i|, |ru_i - Ax_i|); Farkas and
+ * unbounded-ray certificates for those statuses.  meta may be NULL (no
+ * offset, continuous, no Q).  Host code. */
+int mpdhg_check_solution(const mpdhg_lp *lp, const mpdhg_lp_meta *meta, const mpdhg_check_input *sol,
+                         double tol_primal, double tol_dual, double tol_int, mpdhg_check_result *out);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,8 @@ class RefCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("tol_primal", C.c_double),
                 ("tol_dual", C.c_double), ("check_frequency", C.c_int),
                 ("step_tau", C.c_double), ("step_sigma", C.c_double),
-                ("enable_restart", C.c_int)]
+                ("enable_restart", C.c_int), ("x0", C.POINTER(C.c_double)),
+                ("y0", C.POINTER(C.c_double))]
 
 
 class RefOut(C.Structure):
@@ -89,10 +90,17 @@ class LpArrays:
 
 def pdhg_ref(lp: LpArrays, max_iterations=4000, tol_primal=1e-4, tol_dual=1e-4,
              check_frequency=25, step_tau=0.0, step_sigma=0.0, enable_restart=True,
-             fp64=False):
-    """Reference-mode PDHG on the CPU (gpu_solver.cpp:60-307).  Returns a dict."""
+             fp64=False, x0=None, y0=None):
+    """Reference-mode PDHG on the CPU (gpu_solver.cpp:60-307).  Returns a dict.
+    x0 / y0: an optional start point (PDHG dual sign) in place of (clamp(0), 0)."""
     cfg = RefCfg(max_iterations, tol_primal, tol_dual, check_frequency, step_tau, step_sigma,
                  int(enable_restart))
+    keep = []
+    for name, v in (("x0", x0), ("y0", y0)):
+        if v is not None:
+            a = np.ascontiguousarray(v, np.float64)
+            keep.append(a)
+            setattr(cfg, name, _p(a, C.c_double))
     out = RefOut()
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)

@@ -39,6 +39,10 @@ typedef struct orc_ref_cfg {
     int check_frequency;
     double step_tau, step_sigma;
     int enable_restart;
+    /* Optional start point (an extension: the reference always starts from
+     * clamp(0, l, u) and y = 0): x0[n] cast to R and clamped to the bounds in
+     * place of 0, y0[m] (PDHG sign) cast to R in place of 0.  NULL: cold. */
+    const double *x0, *y0;
 } orc_ref_cfg;
 
 typedef struct orc_ref_out {

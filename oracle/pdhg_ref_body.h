@@ -115,12 +115,15 @@ int ORC_NAME(orc_pdhg_ref)(int m, int n, const int *col_ptr, const int *row_ind,
     R mac, mab;
     ORC_NAME(orc_ref_setup)(m, n, col_ptr, vals_d, rp, csr_v, c, rl, ru,
                             (R)cfg->step_tau, (R)cfg->step_sigma, tau, sigma, &mac, &mab);
-    /* :192-196 initial point */
+    /* :192-196 initial point (or the optional start point, clamped alike) */
     for (int j = 0; j < n; ++j) {
-        x[j] = ORC_NAME(smax)(hl[j], ORC_NAME(smin)(hu[j], (R)0));
+        const R x0 = cfg->x0 ? (R)cfg->x0[j] : (R)0;
+        x[j] = ORC_NAME(smax)(hl[j], ORC_NAME(smin)(hu[j], x0));
         xbar[j] = x[j];
         best[j] = x[j];
     }
+    if (cfg->y0)
+        for (int i = 0; i < m; ++i) y[i] = (R)cfg->y0[i];
     R best_res = (R)1e30;
     int iter, converged = 0;
     const int K = cfg->max_iterations, F = cfg->check_frequency;

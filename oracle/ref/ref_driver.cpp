@@ -20,6 +20,13 @@
 // (col_basis[n] then row_basis[m], BasisStatus integers); runs
 // SimplexSolver::solve_from_basis with the options GpuSolver::solve_hybrid
 // passes (gpu_solver.cpp:397-402) and prints the solution as above.
+// Or: refsimplex --check lp.txt sol.txt -- the LP as above followed by
+// "sense S offset O", "vtypes" and n integers, "q NNZ" with Q's CSC arrays
+// (NNZ 0: no Q); the solution file holds "status S", "objective V", then
+// "x", "row_duals", "reduced_costs", "ray" each as a count and values, and
+// "tols tp td ti".  Runs checker::check_solution and prints every
+// CheckResult field (doubles in %a) and the summary.
+#include <sih/checker/checker.hpp>
 #include <sih/io/mps_io.hpp>
 #include <sih/model/problem.hpp>
 #include <sih/model/options.hpp>
@@ -136,7 +143,8 @@ int main(int argc, char **argv) {
         return 0;
     }
     const bool from_basis = std::string(argv[1]) == "--from-basis" && argc >= 4;
-    std::ifstream in(argv[from_basis ? 2 : 1]);
+    const bool check = std::string(argv[1]) == "--check" && argc >= 4;
+    std::ifstream in(argv[(from_basis || check) ? 2 : 1]);
     long m, n, nnz;
     in >> m >> n >> nnz;
     std::vector<long> cp(n + 1), ri(nnz);
@@ -157,6 +165,62 @@ int main(int argc, char **argv) {
     for (long j = 0; j < n; ++j) p.col_upper()[j] = rd(in);
     for (long i = 0; i < m; ++i) p.row_lower()[i] = rd(in);
     for (long i = 0; i < m; ++i) p.row_upper()[i] = rd(in);
+    if (check) {
+        std::string tag;
+        int sense;
+        in >> tag >> sense >> tag;
+        const double offset = rd(in);
+        p.set_sense(static_cast<sih::model::ObjectiveSense>(sense));
+        p.set_obj_offset(offset);
+        in >> tag;
+        for (long j = 0; j < n; ++j) { int t; in >> t; p.var_types()[j] = static_cast<sih::model::VariableType>(t); }
+        long qnz;
+        in >> tag >> qnz;
+        if (qnz > 0) {
+            std::vector<long> qp(n + 1), qi(qnz);
+            std::vector<double> qv(qnz);
+            for (auto &e : qp) in >> e;
+            for (auto &e : qi) in >> e;
+            for (auto &e : qv) e = rd(in);
+            std::vector<sih::model::Triplet> qt;
+            for (long j = 0; j < n; ++j)
+                for (long k = qp[j]; k < qp[j + 1]; ++k) qt.emplace_back(qi[k], j, qv[k]);
+            p.set_quadratic_objective(sih::model::SparseMatrix::from_triplets(n, n, qt));
+        }
+        std::ifstream sin(argv[3]);
+        sih::model::Solution sol;
+        int st;
+        sin >> tag >> st >> tag;
+        sol.status = static_cast<sih::model::SolutionStatus>(st);
+        sol.primal_objective = rd(sin);
+        auto vec = [&](std::vector<double> &x) {
+            long k;
+            sin >> tag >> k;
+            x.resize(static_cast<size_t>(k));
+            for (auto &e : x) e = rd(sin);
+        };
+        vec(sol.x);
+        vec(sol.row_duals);
+        vec(sol.reduced_costs);
+        vec(sol.ray);
+        double tp, td, ti;
+        sin >> tag;
+        tp = rd(sin);
+        td = rd(sin);
+        ti = rd(sin);
+        const sih::checker::CheckResult r = sih::checker::check_solution(p, sol, tp, td, ti);
+        std::printf("is_primal_feasible %d\nmax_row_violation %a\nmax_col_bound_violation %a\n"
+                    "max_primal_residual %a\nis_dual_feasible %d\nmax_dual_residual %a\n"
+                    "is_complementary %d\nmax_complementarity_slack %a\nis_integrality_satisfied %d\n"
+                    "max_integrality_violation %a\nevaluated_objective %a\nobjective_discrepancy %a\n"
+                    "is_certificate_valid %d\ncertificate_violation %a\nall_checks_passed %d\nsummary %s\n",
+                    r.is_primal_feasible, r.max_row_violation, r.max_col_bound_violation, r.max_primal_residual,
+                    r.is_dual_feasible, r.max_dual_residual, r.is_complementary, r.max_complementarity_slack,
+                    r.is_integrality_satisfied, r.max_integrality_violation, r.evaluated_objective,
+                    r.objective_discrepancy, r.is_certificate_valid, r.certificate_violation,
+                    r.all_checks_passed, r.summary.c_str());
+        return 0;
+    }
     if (from_basis) {
         std::ifstream bin(argv[3]);
         std::vector<sih::model::BasisStatus> cb(n), rb(m);

@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -66,6 +67,7 @@ int check_config(const mpdhg_config *cfg) {
         return mpdhg::fail(MPDHG_E_ARG, "cpu_fallback must be 0 or 1");
     if (cfg->spmv_bands < 0 || cfg->spmv_bands > mpdhg::kMaxBands)
         return mpdhg::fail(MPDHG_E_ARG, "spmv_bands must be in [0, 16]");
+    if (cfg->vertex_check < 0) return mpdhg::fail(MPDHG_E_ARG, "vertex_check must be >= 0");
     return MPDHG_OK;
 }
 
@@ -150,6 +152,65 @@ void clear_result(mpdhg_result *r) {
     r->row_activity = ra;
     r->status = MPDHG_STATUS_UNKNOWN;
     r->csr_bands = 1;
+    r->vertex_status = MPDHG_STATUS_UNKNOWN;
+    r->vertex_objective = std::numeric_limits<double>::quiet_NaN();
+    r->vertex_rel_error = std::numeric_limits<double>::quiet_NaN();
+}
+
+// The vertex check needs each LP's point even where the caller skips x:
+// such results get a temporary x for the solve.
+struct XHold {
+    std::vector<std::vector<double>> tmp;
+    std::vector<const double *> x;
+    XHold(const mpdhg_lp *lps, int count, mpdhg_result *res) {
+        tmp.resize(static_cast<size_t>(count));
+        for (int q = 0; q < count; ++q) {
+            if (!res[q].x && lps[q].n > 0) {
+                tmp[static_cast<size_t>(q)].resize(static_cast<size_t>(lps[q].n));
+                res[q].x = tmp[static_cast<size_t>(q)].data();
+            }
+            x.push_back(res[q].x);
+        }
+    }
+    void restore(mpdhg_result *res) {
+        for (size_t q = 0; q < tmp.size(); ++q)
+            if (!tmp[q].empty()) res[q].x = nullptr;
+    }
+};
+
+// mpdhg_solve_batch / mpdhg_solve_warm: the device solve, then the optional
+// vertex check.
+int run_solve(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdhg_config *cfg,
+              const mpdhg_warm_start *warm, mpdhg_result *res) {
+    if (count < 0 || (count > 0 && (!lps || !res))) return mpdhg::fail(MPDHG_E_ARG, "bad batch");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    for (int q = 0; q < count; ++q) clear_result(&res[q]);
+    if (!ctx && cfg->cpu_fallback && visible_devices() <= 0) return host_fallback(lps, count, cfg, res);
+    mpdhg_ctx *tmp = nullptr;
+    if (!ctx) {
+        rc = make_ctx(0, &tmp);
+        if (rc) return rc;
+        ctx = tmp;
+    }
+    rc = guarded([&] {
+        MPDHG_HIP(hipSetDevice(ctx->c.device));
+        XHold hold(lps, count, res);
+        try {
+            if (cfg->mode == MPDHG_MODE_REFERENCE)
+                mpdhg::solve_reference(ctx->c, lps, count, *cfg, warm, res);
+            else
+                mpdhg::solve_enhanced(ctx->c, lps, count, *cfg, warm, res);
+            if (cfg->vertex_check > 0) mpdhg::vertex_check(lps, count, hold.x, cfg->vertex_check, res);
+        } catch (...) {
+            hold.restore(res);
+            throw;
+        }
+        hold.restore(res);
+        return MPDHG_OK;
+    });
+    delete tmp;
+    return rc;
 }
 
 }  // namespace
@@ -221,27 +282,12 @@ void mpdhg_destroy(mpdhg_ctx *ctx) { delete ctx; }
 
 int mpdhg_solve_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdhg_config *cfg,
                       mpdhg_result *res) {
-    if (count < 0 || (count > 0 && (!lps || !res))) return mpdhg::fail(MPDHG_E_ARG, "bad batch");
-    int rc = check_config(cfg);
-    if (rc) return rc;
-    for (int q = 0; q < count; ++q) clear_result(&res[q]);
-    if (!ctx && cfg->cpu_fallback && visible_devices() <= 0) return host_fallback(lps, count, cfg, res);
-    mpdhg_ctx *tmp = nullptr;
-    if (!ctx) {
-        rc = make_ctx(0, &tmp);
-        if (rc) return rc;
-        ctx = tmp;
-    }
-    rc = guarded([&] {
-        MPDHG_HIP(hipSetDevice(ctx->c.device));
-        if (cfg->mode == MPDHG_MODE_REFERENCE)
-            mpdhg::solve_reference(ctx->c, lps, count, *cfg, res);
-        else
-            mpdhg::solve_enhanced(ctx->c, lps, count, *cfg, res);
-        return MPDHG_OK;
-    });
-    delete tmp;
-    return rc;
+    return run_solve(ctx, lps, count, cfg, nullptr, res);
+}
+
+int mpdhg_solve_warm(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, const mpdhg_config *cfg,
+                     const mpdhg_warm_start *warm, mpdhg_result *res) {
+    return run_solve(ctx, lps, count, cfg, warm, res);
 }
 
 int mpdhg_adjoint_batch(mpdhg_ctx *ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
@@ -285,6 +331,7 @@ int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *
     c.launch_graph = 0;
     c.spmv_tune = 0;
     c.spmv_bands = 1;
+    c.vertex_check = 0;
     return guarded([&] {
         MPDHG_HIP(hipSetDevice(ctx->c.device));
         mpdhg::Arena arena;
@@ -292,9 +339,9 @@ int mpdhg_solve_noalloc(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *
         arena.cap = MPDHG_NOALLOC_SCRATCH_BYTES;
         mpdhg::ArenaScope scope(&arena);
         if (c.mode == MPDHG_MODE_REFERENCE)
-            mpdhg::solve_reference(ctx->c, lp, 1, c, res);
+            mpdhg::solve_reference(ctx->c, lp, 1, c, nullptr, res);
         else
-            mpdhg::solve_enhanced(ctx->c, lp, 1, c, res);
+            mpdhg::solve_enhanced(ctx->c, lp, 1, c, nullptr, res);
         return MPDHG_OK;
     });
 }

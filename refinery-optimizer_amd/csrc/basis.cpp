@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -108,3 +110,42 @@ extern "C" int mpdhg_solve_hybrid(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdh
     out->time_sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return MPDHG_OK;
 }
+
+namespace mpdhg {
+
+// The error of each PDHG result against an FP64 optimal vertex: the
+// crossover of solve_hybrid (basis identification, then the polish from that
+// basis and the PDHG point, then from the slack basis if that does not end
+// optimal, gpu_solver.cpp:397-416), on host threads, for the LPs with at most
+// max_rows rows.
+void vertex_check(const mpdhg_lp *lps, int count, const std::vector<const double *> &x, int max_rows,
+                  mpdhg_result *res) {
+    parallel_for_each(count, [&](int q) {
+        mpdhg_result &r = res[q];
+        r.vertex_status = MPDHG_STATUS_UNKNOWN;
+        r.vertex_objective = std::numeric_limits<double>::quiet_NaN();
+        r.vertex_rel_error = std::numeric_limits<double>::quiet_NaN();
+        const mpdhg_lp &lp = lps[q];
+        const double *xq = x[static_cast<size_t>(q)];
+        if (!xq || lp.m <= 0 || lp.n <= 0 || lp.m > max_rows) return;
+        std::vector<int8_t> cb(static_cast<size_t>(lp.n)), rb(static_cast<size_t>(lp.m));
+        if (mpdhg_identify_basis(&lp, xq, cb.data(), rb.data()) != MPDHG_OK) return;
+        mpdhg_polish_opts o;
+        mpdhg_polish_opts_default(&o);
+        o.time_limit_sec = 60.0;
+        mpdhg_polish_result pr;
+        std::memset(&pr, 0, sizeof(pr));
+        if (mpdhg_polish(&lp, cb.data(), rb.data(), xq, &o, &pr) != MPDHG_OK) return;
+        if (pr.status != MPDHG_STATUS_OPTIMAL && pr.status != MPDHG_STATUS_TIME_LIMIT) {
+            std::memset(&pr, 0, sizeof(pr));
+            if (mpdhg_polish(&lp, nullptr, nullptr, nullptr, &o, &pr) != MPDHG_OK) return;
+        }
+        r.vertex_status = pr.status;
+        r.vertex_objective = pr.objective;
+        if (pr.status == MPDHG_STATUS_OPTIMAL)
+            r.vertex_rel_error = std::fabs(r.primal_objective - pr.objective) / std::max(1.0, std::fabs(pr.objective));
+    });
+}
+
+}  // namespace mpdhg
+

@@ -239,7 +239,7 @@ namespace {
 
 struct RefHost {
     std::vector<int> cptr, cidx, rptr, ridx;
-    std::vector<float> cval, rval, c, l, u, tau, rl, ru, sigma, x0;
+    std::vector<float> cval, rval, c, l, u, tau, rl, ru, sigma, x0, y0;
     std::vector<RefInst> inst;
 };
 
@@ -256,7 +256,7 @@ void event_pool(Context &ctx, size_t n) {
 // Reference-mode batch solve.  Per instance this is gpu_solver.cpp:60-307
 // with a correct A^T y; all running instances advance in the same launches.
 void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
-                     mpdhg_result *res) {
+                     const mpdhg_warm_start *warm, mpdhg_result *res) {
     const double t_call = now_sec();
     if (cfg.check_frequency <= 0) throw Error(MPDHG_E_ARG, "check_frequency must be positive");
     std::vector<HostLp> H(static_cast<size_t>(count));
@@ -291,7 +291,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     hh.cval.resize(Z);
     hh.rval.resize(Z);
     hh.c.resize(C); hh.l.resize(C); hh.u.resize(C); hh.tau.resize(C); hh.x0.resize(C);
-    hh.rl.resize(R); hh.ru.resize(R); hh.sigma.resize(R);
+    hh.rl.resize(R); hh.ru.resize(R); hh.sigma.resize(R); hh.y0.assign(R, 0.0f);
     hh.inst.resize(static_cast<size_t>(na));
     for (int a = 0; a < na; ++a) {
         const int q = act[static_cast<size_t>(a)];
@@ -325,12 +325,19 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             if (hh.ru[g] < 1e15f) mab = std::max(mab, std::abs(hh.ru[g]));
             if (hh.rl[g] > -1e15f) mab = std::max(mab, std::abs(hh.rl[g]));
         }
+        const double *wx = (warm && warm[q].x) ? warm[q].x : nullptr;
         for (int j = 0; j < h.n; ++j) {  // :130-140
             float s = 0.0f;
             for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) s += std::abs(hh.cval[static_cast<size_t>(zo + k)]);
             const size_t g = static_cast<size_t>(co + j);
             hh.tau[g] = (cfg.step_tau > 0.0f) ? cfg.step_tau : 0.95f / std::max(1.0f, s);
-            hh.x0[g] = std::max(hh.l[g], std::min(hh.u[g], 0.0f));  // :194-196
+            // :194-196, or the warm start clamped to the bounds
+            const float x0 = wx ? static_cast<float>(wx[j]) : 0.0f;
+            hh.x0[g] = std::max(hh.l[g], std::min(hh.u[g], x0));
+        }
+        if (warm && warm[q].y) {
+            const double sg = cfg.dual_sign ? -1.0 : 1.0;
+            for (int i = 0; i < h.m; ++i) hh.y0[static_cast<size_t>(ro + i)] = static_cast<float>(sg * warm[q].y[i]);
         }
         for (int i = 0; i < h.m; ++i) {  // :141-151
             float s = 0.0f;
@@ -465,7 +472,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_xbest.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
-    MPDHG_HIP(hipMemsetAsync(d_y.get(), 0, R * sizeof(float), st));
+    MPDHG_HIP(hipMemcpyAsync(d_y.get(), hh.y0.data(), R * sizeof(float), hipMemcpyHostToDevice, st));
     DBuf<float> d_prow(RD.host.size()), d_pcol(CD.host.size());
     const TileSet rts = RD.set(), cts = CD.set();
     MPDHG_HIP(hipStreamSynchronize(st));

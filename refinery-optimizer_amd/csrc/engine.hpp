@@ -184,10 +184,16 @@ struct Context {
     ~Context();
 };
 
+// warm: one start per LP, or null (every LP cold).
 void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
-                     mpdhg_result *res);
+                     const mpdhg_warm_start *warm, mpdhg_result *res);
 void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
-                    mpdhg_result *res);
+                    const mpdhg_warm_start *warm, mpdhg_result *res);
+// mpdhg_config::vertex_check: the FP64 crossover from each result's point
+// (host threads), filling vertex_status / vertex_objective / vertex_rel_error.
+// x[q] is LP q's PDHG point.
+void vertex_check(const mpdhg_lp *lps, int count, const std::vector<const double *> &x, int max_rows,
+                  mpdhg_result *res);
 void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_io *io,
                    double active_tol, double tol, int max_iter);
 

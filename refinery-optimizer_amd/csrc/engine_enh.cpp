@@ -23,7 +23,7 @@ constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
 
 struct EnhHost {
     std::vector<int> cptr, cidx, rptr, ridx;
-    std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
+    std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, y0, powv;
     std::vector<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
     std::vector<EnhInst> inst;
 };
@@ -87,7 +87,7 @@ void parallel_for(int count, F &&f) {
 void parallel_for_each(int count, const std::function<void(int)> &f) { parallel_for(count, f); }
 
 void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_config &cfg,
-                    mpdhg_result *res) {
+                    const mpdhg_warm_start *warm, mpdhg_result *res) {
     const double t_call = now_sec();
     const int F = cfg.check_frequency, K = cfg.max_iterations;
     const double tol = cfg.tol_kkt > 0.0 ? cfg.tol_kkt
@@ -131,6 +131,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     hh.cs.resize(C); hh.ls.resize(C); hh.us.resize(C); hh.x0.resize(C); hh.powv.resize(C);
     hh.c64.resize(C); hh.l64.resize(C); hh.u64.resize(C); hh.dc.resize(C);
     hh.rls.resize(R); hh.rus.resize(R); hh.rl64.resize(R); hh.ru64.resize(R); hh.dr.resize(R);
+    hh.y0.assign(R, 0.0f);
     hh.inst.resize(static_cast<size_t>(na));
     std::vector<double> pow0(static_cast<size_t>(na));
 
@@ -159,6 +160,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                 hh.ridx[gd] = static_cast<int>(co + j);
             }
         double cn2 = 0.0, bn2 = 0.0, cinf = 0.0, binf = 0.0, pn = 0.0;
+        // a warm start in the scaled space: x^ = x / D_c, y^ = y / D_r
+        const mpdhg_warm_start *ws = warm ? &warm[q] : nullptr;
         for (int j = 0; j < h.n; ++j) {
             const size_t g = static_cast<size_t>(co + j);
             hh.c64[g] = lp.c[j];
@@ -167,7 +170,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             hh.cs[g] = static_cast<float>(lp.c[j] * dc[j]);
             hh.ls[g] = static_cast<float>(lp.col_lower[j] / dc[j]);
             hh.us[g] = static_cast<float>(lp.col_upper[j] / dc[j]);
-            float v = 0.0f;
+            float v = (ws && ws->x) ? static_cast<float>(ws->x[j] / dc[j]) : 0.0f;
             if (v < hh.ls[g]) v = hh.ls[g];
             if (v > hh.us[g]) v = hh.us[g];
             hh.x0[g] = v;
@@ -176,6 +179,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             const float pv = 1.0f + 0.5f * static_cast<float>((static_cast<uint32_t>(j) * 2654435761u) % 1000u) / 1000.0f;
             hh.powv[g] = pv;
             pn += static_cast<double>(pv) * pv;
+        }
+        if (ws && ws->y) {
+            const double sg = cfg.dual_sign ? -1.0 : 1.0;
+            for (int i = 0; i < h.m; ++i)
+                hh.y0[static_cast<size_t>(ro + i)] = static_cast<float>(sg * ws->y[i] / dr[i]);
         }
         for (int i = 0; i < h.m; ++i) {
             const size_t g = static_cast<size_t>(ro + i);
@@ -194,7 +202,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.status = kRunning;
         s.restart_at = -1;
         s.max_iter = K;
-        s.omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+        s.omega = (ws && ws->primal_weight > 0.0) ? ws->primal_weight
+                                                   : ((cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0);
+        if (ws && ws->step_size > 0.0) s.eta = ws->step_size;
         s.r_anchor = s.r_prev = -1.0;
         s.bnorm = binf;
         s.cnorm = cinf;
@@ -272,8 +282,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     MPDHG_HIP(hipMemcpyAsync(d_x.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_x0.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipMemcpyAsync(d_xp.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
-    for (float *p : {d_y.get(), d_y0.get(), d_yp.get(), d_ybar.get()})
-        MPDHG_HIP(hipMemsetAsync(p, 0, R * sizeof(float), st));
+    for (float *p : {d_y.get(), d_y0.get(), d_yp.get()})
+        MPDHG_HIP(hipMemcpyAsync(p, hh.y0.data(), R * sizeof(float), hipMemcpyHostToDevice, st));
+    MPDHG_HIP(hipMemsetAsync(d_ybar.get(), 0, R * sizeof(float), st));
     for (double *p : {d_xbase.get(), d_xbest.get(), d_lambest.get()})
         MPDHG_HIP(hipMemsetAsync(p, 0, C * sizeof(double), st));
     for (double *p : {d_ybase.get(), d_ybest.get(), d_axbest.get()})
@@ -301,8 +312,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     MPDHG_HIP(hipStreamSynchronize(st));
     const double t_loop = now_sec();
 
-    // ---- step size: power iteration on (D_r A D_c)^T (D_r A D_c) ----
-    for (int p = 0; p < 64; ++p) {
+    // ---- step size: power iteration on (D_r A D_c)^T (D_r A D_c), unless
+    // every LP brings its step size with its warm start ----
+    bool all_steps = true;
+    for (int a = 0; a < na; ++a) all_steps = all_steps && hh.inst[static_cast<size_t>(a)].eta > 0.0;
+    for (int p = 0; p < (all_steps ? 0 : 64); ++p) {
         launch::pow_scale(st, cts, d_inst.get(), d_xbar.get(), d_xp.get());
         launch::pow_rows(st, rts, d_rptr.get(), RD.idx, RD.v32, d_xp.get(), d_yp.get());
         launch::pow_cols(st, cts, d_cptr.get(), CD.idx, CD.v32, d_yp.get(), d_xbar.get(),
@@ -316,7 +330,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     for (int a = 0; a < na; ++a) {
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
         const double sig = std::sqrt(std::sqrt(hs[static_cast<size_t>(a)].pow_norm2));
-        s.eta = 0.998 / (sig > 1e-12 ? sig : 1.0);
+        if (!(s.eta > 0.0)) s.eta = 0.998 / (sig > 1e-12 ? sig : 1.0);
         s.tau = static_cast<float>(s.eta / s.omega);
         s.sigma = static_cast<float>(s.eta * s.omega);
         s.inv_sigma = static_cast<float>(1.0 / (s.eta * s.omega));
@@ -324,7 +338,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     }
     d_inst.upload(hh.inst, st);
     MPDHG_HIP(hipMemcpyAsync(d_xp.get(), hh.x0.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
-    MPDHG_HIP(hipMemsetAsync(d_yp.get(), 0, R * sizeof(float), st));
+    MPDHG_HIP(hipMemcpyAsync(d_yp.get(), hh.y0.data(), R * sizeof(float), hipMemcpyHostToDevice, st));
 
     // ---- iteration loop ----
     // The first iteration of the current check cycle lives on the device

@@ -166,6 +166,15 @@ OwnedLp *gen_staircase(int32_t T, int32_t R, int32_t C, uint64_t seed) {
     o->m = static_cast<int64_t>(T) * R;
     o->n = static_cast<int64_t>(T) * C;
     o->row_kind.assign(static_cast<size_t>(o->m), 0);
+    o->col_kind.assign(static_cast<size_t>(o->n), 0);
+    o->row_block.resize(static_cast<size_t>(o->m));
+    o->col_block.resize(static_cast<size_t>(o->n));
+    for (int64_t i = 0; i < o->m; ++i) o->row_block[static_cast<size_t>(i)] = static_cast<int32_t>(i / R);
+    for (int64_t j = 0; j < o->n; ++j) {
+        const int64_t k = j % C;
+        o->col_block[static_cast<size_t>(j)] = static_cast<int32_t>(j / C);
+        o->col_kind[static_cast<size_t>(j)] = static_cast<int8_t>(k < NC ? 0 : (k < NC + P ? 1 : 2));
+    }
     o->rl.assign(static_cast<size_t>(o->m), 0.0);
     o->ru.assign(static_cast<size_t>(o->m), 0.0);
     o->c.assign(static_cast<size_t>(o->n), 0.0);
@@ -260,9 +269,108 @@ OwnedLp *gen_scenario(const OwnedLp *base, uint64_t seed, double shock, int32_t 
     return o;
 }
 
+// The perturbations of py/refinery/parameterized_generator.py:28-133 on the
+// staircase model: a crude column is a crude purchase (its cost the crude
+// price, its upper bound the supply), a sale column a product sale (its cost
+// minus the product price, its upper bound the demand cap), a capacity row a
+// unit's capacity.  Each multiplier is 1 + clip(N(0, std), lo, hi) with the
+// reference's clips; the regime factors follow :63-113.  The draws run in a
+// fixed order (costs, then supplies, capacities, demands, then the outage),
+// each over the LP in index order.
+OwnedLp *gen_scenario_ex(const OwnedLp *base, uint64_t seed, const mpdhg_scenario_opts &op) {
+    auto *o = new OwnedLp(*base);
+    Rng g(seed);
+    auto mult = [&](double sd, double lo, double hi) {
+        if (sd <= 0.0) return 1.0;
+        return 1.0 + std::max(lo, std::min(hi, sd * g.normal()));
+    };
+    if (op.shock_sigma > 0.0)
+        for (auto &cj : o->c) cj *= std::exp(op.shock_sigma * g.normal());
+    const bool stair = !o->col_kind.empty();
+    if (stair) {
+        // position of a crude / sale column within its block
+        std::vector<int32_t> pos(static_cast<size_t>(o->n), 0);
+        for (int64_t j = 0, run = 0; j < o->n; ++j) {
+            const bool same = j > 0 && o->col_kind[static_cast<size_t>(j)] == o->col_kind[static_cast<size_t>(j - 1)] &&
+                              o->col_block[static_cast<size_t>(j)] == o->col_block[static_cast<size_t>(j - 1)];
+            run = same ? run + 1 : 0;
+            pos[static_cast<size_t>(j)] = static_cast<int32_t>(run);
+        }
+        const int32_t rg = op.regime;
+        for (int64_t j = 0; j < o->n; ++j) {  // prices
+            const int8_t k = o->col_kind[static_cast<size_t>(j)];
+            const int32_t p = pos[static_cast<size_t>(j)];
+            double f = 1.0;
+            if (k == 0) {
+                f = mult(op.price_std, -0.30, 0.40);
+                if (rg == MPDHG_REGIME_HIGH_CRUDE) f *= 1.30;
+                if (rg == MPDHG_REGIME_SOUR_CRUDE) f *= (p == 0 ? 0.85 : (p == 1 ? 1.25 : 1.10));
+            } else if (k == 1) {
+                f = mult(op.price_std, -0.25, 0.35);
+                if (rg == MPDHG_REGIME_DIESEL_SURGE) f *= (p % 4 == 0) ? 1.30 * 1.25 : 1.30;
+            }
+            o->c[static_cast<size_t>(j)] *= f;
+        }
+        for (int64_t j = 0; j < o->n; ++j) {  // crude supplies
+            if (o->col_kind[static_cast<size_t>(j)] != 0) continue;
+            double f = mult(op.supply_std, -0.40, 0.50);
+            if (rg == MPDHG_REGIME_SWEET_SHORTAGE && pos[static_cast<size_t>(j)] == 1) f *= 0.35;
+            o->u[static_cast<size_t>(j)] *= f;
+        }
+        std::vector<int32_t> cap_pos(static_cast<size_t>(o->m), 0);
+        for (int64_t i = 0, run = 0; i < o->m; ++i) {
+            const bool same = i > 0 && o->row_kind[static_cast<size_t>(i)] == o->row_kind[static_cast<size_t>(i - 1)] &&
+                              o->row_block[static_cast<size_t>(i)] == o->row_block[static_cast<size_t>(i - 1)];
+            run = same ? run + 1 : 0;
+            cap_pos[static_cast<size_t>(i)] = static_cast<int32_t>(run);
+        }
+        for (int64_t i = 0; i < o->m; ++i) {  // unit capacities
+            if (o->row_kind[static_cast<size_t>(i)] != 2) continue;
+            double f = mult(op.capacity_std, -0.25, 0.30);
+            if (rg == MPDHG_REGIME_TURNAROUND && cap_pos[static_cast<size_t>(i)] == 0) f *= 0.65;
+            o->ru[static_cast<size_t>(i)] *= f;
+        }
+        for (int64_t j = 0; j < o->n; ++j) {  // demand caps
+            if (o->col_kind[static_cast<size_t>(j)] != 1) continue;
+            double f = mult(op.demand_std, -0.20, 0.25);
+            if (rg == MPDHG_REGIME_DIESEL_SURGE && pos[static_cast<size_t>(j)] % 4 == 0) f *= 1.60 / 1.25;
+            o->u[static_cast<size_t>(j)] *= f;
+        }
+    }
+    if (op.zero_capacity && !o->row_kind.empty()) {
+        std::vector<int64_t> caps;
+        for (int64_t i = 0; i < o->m; ++i)
+            if (o->row_kind[static_cast<size_t>(i)] == 2) caps.push_back(i);
+        if (!caps.empty()) o->ru[static_cast<size_t>(caps[g.below(caps.size())])] = 0.0;
+    }
+    finish_views(o);
+    return o;
+}
+
 }  // namespace mpdhg
 
 extern "C" {
+
+void mpdhg_scenario_opts_default(mpdhg_scenario_opts *opts) {
+    if (opts) std::memset(opts, 0, sizeof(*opts));
+}
+
+int mpdhg_gen_scenario_ex(const mpdhg_lp_owned *base, uint64_t seed, const mpdhg_scenario_opts *opts,
+                          mpdhg_lp_owned **out) {
+    if (!base || !opts || !out) return mpdhg::fail(MPDHG_E_ARG, "gen_scenario_ex: NULL argument");
+    if (opts->regime < MPDHG_REGIME_NORMAL || opts->regime > MPDHG_REGIME_SWEET_SHORTAGE)
+        return mpdhg::fail(MPDHG_E_ARG, "gen_scenario_ex: unknown regime");
+    if (opts->shock_sigma < 0 || opts->price_std < 0 || opts->supply_std < 0 || opts->capacity_std < 0 ||
+        opts->demand_std < 0)
+        return mpdhg::fail(MPDHG_E_ARG, "gen_scenario_ex: negative standard deviation");
+    try {
+        *out = reinterpret_cast<mpdhg_lp_owned *>(
+            mpdhg::gen_scenario_ex(reinterpret_cast<const mpdhg::OwnedLp *>(base), seed, *opts));
+    } catch (const std::bad_alloc &) {
+        return mpdhg::fail(MPDHG_E_NOMEM, "gen_scenario_ex: out of host memory");
+    }
+    return MPDHG_OK;
+}
 
 int mpdhg_gen_random(int64_t m, int64_t n, int32_t k, uint64_t seed, mpdhg_lp_owned **out) {
     if (!out || m < 1 || n < 1 || k < 1) return mpdhg::fail(MPDHG_E_ARG, "gen_random: bad size");

@@ -14,7 +14,9 @@ struct OwnedLp {
     int64_t m = 0, n = 0;
     std::vector<int64_t> col_ptr, row_ind;
     std::vector<double> vals, c, l, u, rl, ru;
-    std::vector<int8_t> row_kind;  // staircase only
+    std::vector<int8_t> row_kind;  // staircase only: 0 balance, 1 blend, 2 capacity
+    std::vector<int8_t> col_kind;  // staircase only: 0 crude, 1 sale, 2 inventory
+    std::vector<int32_t> row_block, col_block;  // staircase only: time block of each row / column
     mpdhg_lp view{};
     // model::Problem fields beside the LP (MPS files; defaults otherwise)
     std::string name = "problem";
@@ -37,6 +39,7 @@ void write_mps(const std::string &path, const mpdhg_lp &lp, const mpdhg_lp_meta 
 OwnedLp *gen_random(int64_t m, int64_t n, int32_t k, uint64_t seed);
 OwnedLp *gen_staircase(int32_t T, int32_t R, int32_t C, uint64_t seed);
 OwnedLp *gen_scenario(const OwnedLp *base, uint64_t seed, double shock, int32_t zero_cap);
+OwnedLp *gen_scenario_ex(const OwnedLp *base, uint64_t seed, const mpdhg_scenario_opts &o);
 
 // Records a thread-local message and returns `code`.
 int fail(int code, const std::string &msg);

@@ -291,6 +291,9 @@ class Solution:
     csc_chunk: int = 0
     tune_ms: float = 0.0  # time of the schedule search
     csr_bands: int = 1  # column bands of the A xbar product
+    vertex_status: SolutionStatus = SolutionStatus.Unknown  # vertex_check: crossover status
+    vertex_objective: float = math.nan  # vertex_check: objective of the FP64 vertex
+    vertex_rel_error: float = math.nan  # vertex_check: |objective - vertex| / max(1, |vertex|)
 
     def is_optimal(self) -> bool:
         return self.status == SolutionStatus.Optimal
@@ -329,6 +332,7 @@ class GpuPdhgConfig:
     spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
     cpu_fallback: int = 0  # 1: with no HIP device, solve on the host FP64 simplex (gpu_solver.cpp:66-71)
     spmv_bands: int = 0  # reference mode, one LP: column bands of A xbar (0 rule, 1 none, k bands)
+    vertex_check: int = 0  # k > 0: FP64 crossover of every LP with <= k rows after the solve
 
     @staticmethod
     def enhanced(tol: float = 1e-6, max_iterations: int = 100000, check_frequency: int = 64,
@@ -368,9 +372,28 @@ def _result_to_solution(r: MpdhgResult, x, y, rc, ax) -> Solution:
                  "rel_kkt", "ref_primal", "ref_dual", "kernel_ms_spmv_csr",
                  "kernel_ms_spmv_csc", "kernel_launches_csr", "kernel_launches_csc",
                  "bytes_csr_launch", "bytes_csc_launch", "step_size", "primal_weight",
-                 "csr_block", "csr_chunk", "csc_block", "csc_chunk", "tune_ms", "csr_bands"):
+                 "csr_block", "csr_chunk", "csc_block", "csc_chunk", "tune_ms", "csr_bands",
+                 "vertex_objective", "vertex_rel_error"):
         setattr(s, name, getattr(r, name))
+    s.vertex_status = SolutionStatus(r.vertex_status)
     return s
+
+
+@dataclass
+class WarmStart:
+    """Where a solve starts (mpdhg_warm_start): usually the solution of a base
+    LP that a scenario perturbs (py/refinery/reoptimization.py:31-79 re-solves
+    each perturbation from the previous solution).  y is in the config's
+    dual_sign; primal_weight / step_size <= 0 leave those to the solver."""
+
+    x: Optional[np.ndarray] = None
+    y: Optional[np.ndarray] = None
+    primal_weight: float = 0.0
+    step_size: float = 0.0
+
+    @staticmethod
+    def of(sol: "Solution") -> "WarmStart":
+        return WarmStart(sol.x, sol.row_duals, sol.primal_weight, sol.step_size)
 
 
 class _Ctx:
@@ -410,14 +433,42 @@ def _solve_ctx(config: GpuPdhgConfig):
     return _ctx()._p
 
 
-def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None) -> List[Solution]:
-    """Solve many LPs in one device-resident batch (extension of the reference API)."""
+def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None,
+                warm: Optional[Sequence[Optional[WarmStart]]] = None) -> List[Solution]:
+    """Solve many LPs in one device-resident batch (extension of the reference
+    API); warm: one WarmStart (or Solution, or None) per LP."""
     config = config or GpuPdhgConfig()
     cfg = config._c()
     nb = len(problems)
     lps = (MpdhgLp * max(nb, 1))()
     res = (MpdhgResult * max(nb, 1))()
     keep, outs = [], []
+    ws = None
+    if warm is not None:
+        if len(warm) != nb:
+            raise ValueError("one warm start per problem")
+        ws = (_abi.MpdhgWarmStart * max(nb, 1))()
+        dp = C.POINTER(C.c_double)
+        for q, w in enumerate(warm):
+            if w is None:
+                continue
+            if isinstance(w, Solution):
+                w = WarmStart.of(w)
+            m, n = problems[q].num_rows(), problems[q].num_cols()
+            if w.x is not None:
+                xa = np.ascontiguousarray(w.x, np.float64)
+                if xa.shape != (n,):
+                    raise ValueError("warm x has the wrong length")
+                keep.append(xa)
+                ws[q].x = xa.ctypes.data_as(dp)
+            if w.y is not None:
+                ya = np.ascontiguousarray(w.y, np.float64)
+                if ya.shape != (m,):
+                    raise ValueError("warm y has the wrong length")
+                keep.append(ya)
+                ws[q].y = ya.ctypes.data_as(dp)
+            ws[q].primal_weight = float(w.primal_weight)
+            ws[q].step_size = float(w.step_size)
     for q, p in enumerate(problems):
         lp, arrs = p._c_view()
         lps[q] = lp
@@ -429,8 +480,12 @@ def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = N
         res[q].y = o[1].ctypes.data_as(C.POINTER(C.c_double))
         res[q].reduced_costs = o[2].ctypes.data_as(C.POINTER(C.c_double))
         res[q].row_activity = o[3].ctypes.data_as(C.POINTER(C.c_double))
-    _abi.check(_abi.lib().mpdhg_solve_batch(_solve_ctx(config), lps, nb, C.byref(cfg), res),
-               "mpdhg_solve_batch")
+    if ws is None:
+        _abi.check(_abi.lib().mpdhg_solve_batch(_solve_ctx(config), lps, nb, C.byref(cfg), res),
+                   "mpdhg_solve_batch")
+    else:
+        _abi.check(_abi.lib().mpdhg_solve_warm(_solve_ctx(config), lps, nb, C.byref(cfg), ws, res),
+                   "mpdhg_solve_warm")
     return [_result_to_solution(res[q], *outs[q]) for q in range(nb)]
 
 
@@ -492,8 +547,8 @@ class GpuSolver:
         return s
 
     @staticmethod
-    def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None):
-        return solve_batch(problems, config)
+    def solve_batch(problems: Sequence[Problem], config: Optional[GpuPdhgConfig] = None, warm=None):
+        return solve_batch(problems, config, warm)
 
 
 @dataclass
@@ -685,6 +740,39 @@ def gen_scenario(kind: str, size: tuple, base_seed: int, scenario_seed: int,
     return _owned_to_problem(out, f"{kind}_{'x'.join(map(str, size))}_s{base_seed}_{scenario_seed}")
 
 
+REGIMES = {"normal": 0, "high_crude": 1, "diesel_surge": 2, "turnaround": 3, "sour_crude": 4,
+           "sweet_crude_shortage": 5}
+
+
+def gen_scenario_ex(kind: str, size: tuple, base_seed: int, scenario_seed: int, shock_sigma: float = 0.0,
+                    price_std: float = 0.0, supply_std: float = 0.0, capacity_std: float = 0.0,
+                    demand_std: float = 0.0, regime: str = "normal", zero_capacity: bool = False) -> Problem:
+    """A perturbed scenario of a base workload (mpdhg_gen_scenario_ex): the
+    perturbations of py/refinery/parameterized_generator.py:28-133 on the
+    staircase model (crude prices and supplies, unit capacities, product
+    prices and demand caps, market regimes); random-equality LPs take the
+    lognormal price shock only."""
+    L = _abi.lib()
+    base = C.c_void_p()
+    if kind == "random":
+        _abi.check(L.mpdhg_gen_random(*size, base_seed, C.byref(base)), "gen_random")
+    elif kind == "staircase":
+        _abi.check(L.mpdhg_gen_staircase(*size, base_seed, C.byref(base)), "gen_staircase")
+    else:
+        raise ValueError(f"unknown workload kind {kind!r}")
+    o = _abi.MpdhgScenarioOpts()
+    L.mpdhg_scenario_opts_default(C.byref(o))
+    o.shock_sigma, o.price_std, o.supply_std = shock_sigma, price_std, supply_std
+    o.capacity_std, o.demand_std = capacity_std, demand_std
+    o.regime = REGIMES[regime]
+    o.zero_capacity = int(bool(zero_capacity))
+    out = C.c_void_p()
+    rc = L.mpdhg_gen_scenario_ex(base, scenario_seed, C.byref(o), C.byref(out))
+    L.mpdhg_lp_free(base)
+    _abi.check(rc, "gen_scenario_ex")
+    return _owned_to_problem(out, f"{kind}_{'x'.join(map(str, size))}_s{base_seed}_{regime}_{scenario_seed}")
+
+
 # ---- MPS / QPS files (io/mps_io.hpp:10-13) --------------------------------
 def read_mps(filepath: str) -> Problem:
     """io::read_mps (mps_io.cpp:62-409), parsed by libmpdhg: the LP, name,
@@ -724,10 +812,8 @@ def read_mps(filepath: str) -> Problem:
     return p
 
 
-def write_mps(problem: Problem, filepath: str, free_format: bool = True) -> None:
-    """io::write_mps (mps_io.cpp:411-578): the reference writer's text, byte
-    for byte; free_format does not change the output, as there."""
-    lp, keep = problem._c_view()
+def _meta_view(problem: Problem):
+    """mpdhg_lp_meta of a Problem, and the arrays it points into."""
     m, n = problem.num_rows(), problem.num_cols()
     meta = _abi.MpdhgLpMeta()
     meta.name = str(problem.name).encode()
@@ -762,8 +848,79 @@ def write_mps(problem: Problem, filepath: str, free_format: bool = True) -> None
         meta.q_col_ptr = q[0].ctypes.data_as(C.POINTER(C.c_int64))
         meta.q_row_ind = q[1].ctypes.data_as(C.POINTER(C.c_int64))
         meta.q_vals = q[2].ctypes.data_as(C.POINTER(C.c_double))
+    return meta, (names, vt, q)
+
+
+def write_mps(problem: Problem, filepath: str, free_format: bool = True) -> None:
+    """io::write_mps (mps_io.cpp:411-578): the reference writer's text, byte
+    for byte; free_format does not change the output, as there."""
+    lp, keep = problem._c_view()
+    meta, keep_meta = _meta_view(problem)
     _abi.check(_abi.lib().mpdhg_write_mps(str(filepath).encode(), C.byref(lp), C.byref(meta),
                                           int(bool(free_format))), "write_mps")
+
+
+@dataclass
+class CheckResult:
+    """checker::CheckResult (checker.hpp:9-34)."""
+
+    is_primal_feasible: bool = False
+    max_row_violation: float = 0.0
+    max_col_bound_violation: float = 0.0
+    max_primal_residual: float = 0.0
+    is_dual_feasible: bool = False
+    max_dual_residual: float = 0.0
+    is_complementary: bool = False
+    max_complementarity_slack: float = 0.0
+    is_integrality_satisfied: bool = False
+    max_integrality_violation: float = 0.0
+    evaluated_objective: float = 0.0
+    objective_discrepancy: float = 0.0
+    is_certificate_valid: bool = False
+    certificate_violation: float = 0.0
+    all_checks_passed: bool = False
+    summary: str = ""
+
+
+def check_solution(problem: Problem, solution: Solution, tol_primal: float = 1e-6,
+                   tol_dual: float = 1e-6, tol_int: float = 1e-5, ray=None) -> CheckResult:
+    """checker::check_solution (checker.cpp:9-268) in libmpdhg: independent FP64
+    verification of a candidate solution.  The duals are read in the checker's
+    sign (reduced cost = c - A^T y): pass a solution solved with dual_sign = 1,
+    or negate the PDHG-sign row_duals first.  Empty vectors count as absent, as
+    there.  ray: the certificate of an Infeasible / Unbounded status."""
+    lp, keep = problem._c_view()
+    meta, keep_meta = _meta_view(problem)
+    dp = C.POINTER(C.c_double)
+    inp = _abi.MpdhgCheckInput()
+    inp.status = int(solution.status)
+    arrs = []
+    for name, v in (("x", solution.x), ("row_duals", solution.row_duals),
+                    ("reduced_costs", solution.reduced_costs), ("ray", ray)):
+        if v is not None and len(v):
+            a = np.ascontiguousarray(v, np.float64)
+            arrs.append(a)
+            setattr(inp, name, a.ctypes.data_as(dp))
+    want = {"x": problem.num_cols(), "row_duals": problem.num_rows(), "reduced_costs": problem.num_cols()}
+    for name, k in want.items():  # a vector of the wrong length counts as absent, as there
+        v = getattr(solution, name)
+        if v is not None and len(v) and len(v) != k:
+            setattr(inp, name, None)
+    inp.primal_objective = float(solution.primal_objective)
+    inp.ray_len = 0 if ray is None else len(ray)
+    out = _abi.MpdhgCheckResult()
+    _abi.check(_abi.lib().mpdhg_check_solution(C.byref(lp), C.byref(meta), C.byref(inp), tol_primal,
+                                               tol_dual, tol_int, C.byref(out)), "check_solution")
+    r = CheckResult()
+    for name, _ in _abi.MpdhgCheckResult._fields_:
+        v = getattr(out, name)
+        if name == "summary":
+            r.summary = v.decode(errors="replace")
+        elif isinstance(getattr(r, name), bool):
+            setattr(r, name, bool(v))
+        else:
+            setattr(r, name, float(v))
+    return r
 
 
 class MpsReader:
@@ -780,6 +937,7 @@ __all__ = [
     "SparseMatrix", "Triplet", "Problem", "Solution", "SolutionStatus", "BasisStatus",
     "ObjectiveSense", "VariableType", "GpuSolver", "GpuPdhgConfig", "GpuMemoryInfo",
     "solve_gpu_pdhg", "solve_gpu_hybrid", "solve_batch", "identify_basis", "gen_random",
-    "gen_staircase", "read_mps", "write_mps", "MpsReader", "polish", "PolishResult",
+    "gen_staircase", "read_mps", "write_mps", "MpsReader", "polish", "PolishResult", "WarmStart",
+    "check_solution", "CheckResult", "gen_scenario_ex", "REGIMES",
     "MPDHG_MODE_REFERENCE", "MPDHG_MODE_ENHANCED", "SIH_INFINITY",
 ]

@@ -41,6 +41,7 @@ class MpdhgConfig(C.Structure):
         ("spmv_block", C.c_int32),
         ("spmv_tune", C.c_int32),
         ("cpu_fallback", C.c_int32),
+        ("vertex_check", C.c_int32),
         ("spmv_bands", C.c_int32),
     ]
 
@@ -96,6 +97,63 @@ class MpdhgResult(C.Structure):
         ("csc_chunk", C.c_int32),
         ("tune_ms", C.c_double),
         ("csr_bands", C.c_int32),
+        ("vertex_status", C.c_int32),
+        ("vertex_objective", C.c_double),
+        ("vertex_rel_error", C.c_double),
+    ]
+
+
+class MpdhgWarmStart(C.Structure):
+    _fields_ = [
+        ("x", C.POINTER(C.c_double)),
+        ("y", C.POINTER(C.c_double)),
+        ("primal_weight", C.c_double),
+        ("step_size", C.c_double),
+    ]
+
+
+class MpdhgScenarioOpts(C.Structure):
+    _fields_ = [
+        ("shock_sigma", C.c_double),
+        ("price_std", C.c_double),
+        ("supply_std", C.c_double),
+        ("capacity_std", C.c_double),
+        ("demand_std", C.c_double),
+        ("regime", C.c_int32),
+        ("zero_capacity", C.c_int32),
+    ]
+
+
+class MpdhgCheckInput(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("x", C.POINTER(C.c_double)),
+        ("row_duals", C.POINTER(C.c_double)),
+        ("reduced_costs", C.POINTER(C.c_double)),
+        ("primal_objective", C.c_double),
+        ("ray", C.POINTER(C.c_double)),
+        ("ray_len", C.c_int64),
+    ]
+
+
+class MpdhgCheckResult(C.Structure):
+    _fields_ = [
+        ("is_primal_feasible", C.c_int32),
+        ("max_row_violation", C.c_double),
+        ("max_col_bound_violation", C.c_double),
+        ("max_primal_residual", C.c_double),
+        ("is_dual_feasible", C.c_int32),
+        ("max_dual_residual", C.c_double),
+        ("is_complementary", C.c_int32),
+        ("max_complementarity_slack", C.c_double),
+        ("is_integrality_satisfied", C.c_int32),
+        ("max_integrality_violation", C.c_double),
+        ("evaluated_objective", C.c_double),
+        ("objective_discrepancy", C.c_double),
+        ("is_certificate_valid", C.c_int32),
+        ("certificate_violation", C.c_double),
+        ("all_checks_passed", C.c_int32),
+        ("summary", C.c_char * 256),
     ]
 
 
@@ -180,6 +238,8 @@ SIGNATURES = {
                                     C.POINTER(MpdhgResult)]),
     "mpdhg_solve_noalloc": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.POINTER(MpdhgConfig),
                                       C.POINTER(MpdhgResult)]),
+    "mpdhg_solve_warm": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgConfig),
+                                   C.POINTER(MpdhgWarmStart), C.POINTER(MpdhgResult)]),
     "mpdhg_adjoint_batch": (C.c_int, [_VP, C.POINTER(MpdhgLp), C.c_int, C.POINTER(MpdhgAdjointIO),
                                       C.c_double, C.c_double, C.c_int]),
     "mpdhg_identify_basis": (C.c_int, [C.POINTER(MpdhgLp), C.POINTER(C.c_double),
@@ -188,6 +248,8 @@ SIGNATURES = {
     "mpdhg_gen_staircase": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                       C.POINTER(_VP)]),
     "mpdhg_gen_scenario": (C.c_int, [_VP, C.c_uint64, C.c_double, C.c_int32, C.POINTER(_VP)]),
+    "mpdhg_scenario_opts_default": (None, [C.POINTER(MpdhgScenarioOpts)]),
+    "mpdhg_gen_scenario_ex": (C.c_int, [_VP, C.c_uint64, C.POINTER(MpdhgScenarioOpts), C.POINTER(_VP)]),
     "mpdhg_lp_view": (C.POINTER(MpdhgLp), [_VP]),
     "mpdhg_lp_row_kind": (C.POINTER(C.c_int8), [_VP]),
     "mpdhg_lp_free": (None, [_VP]),
@@ -201,6 +263,8 @@ SIGNATURES = {
     "mpdhg_read_mps": (C.c_int, [C.c_char_p, C.POINTER(_VP)]),
     "mpdhg_lp_meta_view": (C.POINTER(MpdhgLpMeta), [_VP]),
     "mpdhg_write_mps": (C.c_int, [C.c_char_p, C.POINTER(MpdhgLp), C.POINTER(MpdhgLpMeta), C.c_int32]),
+    "mpdhg_check_solution": (C.c_int, [C.POINTER(MpdhgLp), C.POINTER(MpdhgLpMeta), C.POINTER(MpdhgCheckInput),
+                                       C.c_double, C.c_double, C.c_double, C.POINTER(MpdhgCheckResult)]),
 }
 
 _lib = None

@@ -84,6 +84,31 @@ def test_first_iterate_closed_form():
     assert r["iterations"] == 2 and r["status"] == 4  # K+1 when not converged
 
 
+def test_first_iterate_closed_form_from_a_start_point():
+    """The same iteration from a given start (x0 clamped to the bounds, y0):
+    x1 = clamp(x0 - tau*(c + A^T y0)), xbar = 2 x1 - x0, w = A xbar + y0/sigma,
+    y1 = sigma * (w - clamp(w, rl, ru)).  A start at (clamp(0), 0) is the cold
+    solve exactly."""
+    p, dense = random_lp(20, 30, 0.3, 7)
+    lp = O.LpArrays.of(p)
+    rng = np.random.default_rng(3)
+    xs, ys = rng.uniform(-2, 12, 30), rng.normal(size=20)
+    r = O.pdhg_ref(lp, max_iterations=1, check_frequency=1, fp64=True, x0=xs, y0=ys)
+    tau = 0.95 / np.maximum(1.0, np.abs(dense).sum(axis=0))
+    sigma = 0.95 / np.maximum(1.0, np.abs(dense).sum(axis=1))
+    x0 = np.clip(xs, lp.l, lp.u)
+    x1 = np.clip(x0 - tau * (lp.c + dense.T @ ys), lp.l, lp.u)
+    xbar = 2 * x1 - x0
+    w = dense @ xbar + ys / sigma
+    y1 = sigma * (w - np.clip(w, lp.rl, lp.ru))
+    np.testing.assert_allclose(r["x_last"], x1, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(r["xbar_last"], xbar, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(r["y"], y1, rtol=1e-12, atol=1e-13)
+    cold = O.pdhg_ref(lp, max_iterations=40, check_frequency=7)
+    same = O.pdhg_ref(lp, max_iterations=40, check_frequency=7, x0=np.zeros(30), y0=np.zeros(20))
+    assert np.array_equal(cold["x"], same["x"]) and np.array_equal(cold["y"], same["y"])
+
+
 def test_toy01_reference_pdhg_reaches_known_optimum():
     lp = O.LpArrays.of(toy01())
     for fp64 in (False, True):
