@@ -84,6 +84,95 @@ def test_first_iterate_closed_form():
     assert r["iterations"] == 2 and r["status"] == 4  # K+1 when not converged
 
 
+def _f32(q):
+    """The float32 nearest the rational q (ties to even): one correct rounding."""
+    from fractions import Fraction
+
+    q = Fraction(q)
+    if q == 0:
+        return np.float32(0.0)
+    sgn, q = (-1 if q < 0 else 1), abs(q)
+    e = q.numerator.bit_length() - q.denominator.bit_length()
+    if Fraction(2) ** e > q:
+        e -= 1
+    quantum = Fraction(2) ** (max(e, -126) - 23)
+    n, r = divmod(q, quantum)
+    n = int(n)
+    if r > quantum / 2 or (r == quantum / 2 and n % 2 == 1):
+        n += 1
+    return np.float32(float(sgn * n * quantum))
+
+
+def _fmaf(a, b, c):
+    """fmaf(a, b, c): a*b + c with one rounding to float32."""
+    from fractions import Fraction
+
+    return _f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def test_fp32_first_iterate_closed_form_with_contraction():
+    """One FP32 reference iteration (cuda_pdhg.cu:14-103) from a random start
+    point, written out with every operation rounded to float32 and nvcc's
+    default contraction (-fmad=true) of `sum += v * x` in both SpMVs and of
+    `x_old - step * grad` into single-rounding fmaf (emulated exactly with
+    rationals); `2 x_new - x_old` is exact with or without contraction, and the
+    dual step has no contractible product.  The step sizes are the host's
+    sequential float sums (gpu_solver.cpp:128-151).  The FP32 oracle must match
+    bit for bit -- a dropped or misplaced fma changes the last bits."""
+    f32 = np.float32
+    p, dense = random_lp(12, 18, 0.35, 17)
+    lp = O.LpArrays.of(p)
+    rng = np.random.default_rng(29)
+    xs, ys = rng.uniform(-1, 11, 18), rng.normal(size=12)
+    r = O.pdhg_ref(lp, max_iterations=1, check_frequency=1, x0=xs, y0=ys)
+    m, n = 12, 18
+    cp, ri = lp.cp, lp.ri
+    v = lp.v.astype(np.float32)
+    rp, ci, _ = O.csc_to_csr(lp)
+    csr_v = np.array([f32(dense[i, ci[k]]) for i in range(m) for k in range(rp[i], rp[i + 1])], np.float32)
+    tau, sigma = np.zeros(n, np.float32), np.zeros(m, np.float32)
+    for j in range(n):
+        s = f32(0)
+        for k in range(cp[j], cp[j + 1]):
+            s = f32(s + abs(v[k]))
+        tau[j] = f32(f32(0.95) / max(f32(1), s))
+    for i in range(m):
+        s = f32(0)
+        for k in range(rp[i], rp[i + 1]):
+            s = f32(s + abs(csr_v[k]))
+        sigma[i] = f32(f32(0.95) / max(f32(1), s))
+    l, u = lp.l.astype(np.float32), lp.u.astype(np.float32)
+    rl, ru, c = lp.rl.astype(np.float32), lp.ru.astype(np.float32), lp.c.astype(np.float32)
+    x0 = np.array([max(l[j], min(u[j], f32(xs[j]))) for j in range(n)], np.float32)
+    y0 = ys.astype(np.float32)
+    x1, xbar = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    for j in range(n):
+        g = f32(0)
+        for k in range(cp[j], cp[j + 1]):
+            g = _fmaf(v[k], y0[ri[k]], g)
+        grad = f32(c[j] + g)
+        xn = _fmaf(-tau[j], grad, x0[j])
+        xn = l[j] if xn < l[j] else xn
+        xn = u[j] if xn > u[j] else xn
+        x1[j] = xn
+        xbar[j] = f32(f32(2) * xn - x0[j])
+    y1 = np.zeros(m, np.float32)
+    for i in range(m):
+        h = f32(0)
+        for k in range(rp[i], rp[i + 1]):
+            h = _fmaf(csr_v[k], xbar[ci[k]], h)
+        w = f32(h + f32(y0[i] / sigma[i]))
+        q = rl[i] if w < rl[i] else w
+        q = ru[i] if q > ru[i] else q
+        y1[i] = f32(sigma[i] * f32(w - q))
+    assert np.array_equal(r["x_last"], x1)
+    assert np.array_equal(r["xbar_last"], xbar)
+    assert np.array_equal(r["y"], y1.astype(np.float64))
+    # the same arithmetic without contraction differs on such data: the test can tell
+    assert any(_fmaf(a, b, cc) != f32(f32(a * b) + cc)
+               for a, b, cc in zip(v, y0[ri], rng.normal(size=v.size).astype(np.float32)))
+
+
 def test_first_iterate_closed_form_from_a_start_point():
     """The same iteration from a given start (x0 clamped to the bounds, y0):
     x1 = clamp(x0 - tau*(c + A^T y0)), xbar = 2 x1 - x0, w = A xbar + y0/sigma,
