@@ -23,7 +23,7 @@ Other workloads:
              objective error against the committed FP64 HiGHS value.
   adjoint64  64 staircase LPs (48 blocks) forward to 1e-8 and backward
              through the torch.autograd layer, with central finite
-             differences on 4 of them.
+             differences on 4 of them at their FP64 crossover vertices.
   hybrid48   solve_hybrid on a 48-block staircase LP: GPU PDHG to 1e-8,
              then the FP64 crossover on the host, against HiGHS's optimum.
   copy       a 256 MiB device-to-device elementwise copy (torch), the
@@ -531,81 +531,88 @@ def bench_adjoint(args):
     (w * x).sum().backward()
     t2 = time.perf_counter()
     log(f"forward {t1 - t0:.2f}s, backward {t2 - t1:.2f}s")
-    # Central differences of L along shifts of b (forward solves to 1e-9).
-    # x*(b) is a differentiable function only where the solution is a unique
-    # vertex: the basic columns B must be determined by the active rows R
-    # (A_RB of full column rank, so A_RB^T mu = w_B is consistent) and every
-    # nonbasic column must have a reduced cost away from zero.  LPs failing
-    # either test (PDHG then returns a point inside an optimal face) are
-    # counted, not differenced.  On the others the direction d = A_RB v keeps
-    # the active set (x_B moves by h v), and h also keeps every basic column
-    # inside its bounds and every inactive row feasible.
+    # Central differences at FP64 vertices of the 48-block LPs.  A PDHG point
+    # lies inside the optimal face, where x*(b) is set-valued; at a vertex
+    # with its optimal basis, x*(b) is the basis's solution B^-1 (b - N x_N)
+    # as long as that basis stays optimal.  solve_hybrid (PDHG to 1e-9, the
+    # basis hand-off, the FP64 crossover) gives the vertex and basis of each
+    # sampled LP; the GPU adjoint is evaluated there, and L = w . x* is
+    # differenced along d = A_RB v (B: basic columns inside their bounds, R:
+    # rows at a bound), a shift that keeps the basis primal feasible and so
+    # optimal: each shifted LP is re-solved by the FP64 simplex from that
+    # basis (no pivots expected).  These vertices are dual degenerate (a few
+    # hundred nonbasic columns with zero reduced cost), so other optimal
+    # vertices exist: re-solving from scratch could land on another one.
     import scipy.sparse as sp
 
+    hcfg = S.GpuPdhgConfig.enhanced(tol=1e-9, max_iterations=2000000)
     co = np.concatenate([[0], np.cumsum([p.num_cols() for p in probs])])
-    ro = np.concatenate([[0], np.cumsum([p.num_rows() for p in probs])])
-    wn, gb, xs = w.numpy(), b.grad.numpy(), x.detach().numpy()
-    fsols = x.grad_fn.sols  # the forward solutions the layer kept
-    fd_probs, meta, screened = [], [], []
-    for k in range(min(16, K)):
-        if len(meta) == 4:
+    wn = w.numpy()
+    errs, steps, screened, checks, t_fd = [], [], [], [], time.perf_counter()
+    for k in range(min(8, K)):
+        if len(errs) == args.adjoint_fd:
             break
-        p, sk = probs[k], fsols[k]
-        xk, yk, lam = xs[co[k]:co[k + 1]], sk.row_duals, sk.reduced_costs
-        wk, gk = wn[co[k]:co[k + 1]], gb[ro[k]:ro[k + 1]]
+        p = probs[k]
+        wk = wn[co[k]:co[k + 1]]
+        v0 = S.solve_gpu_hybrid(p, hcfg)
+        if v0.status != S.SolutionStatus.Optimal:
+            screened.append(dict(lp=k, reason=f"crossover status {v0.status.name}"))
+            continue
+        xk, yk = v0.x, -v0.row_duals  # simplex sign -> PDHG sign
+        r = S.adjoint_batch([p], [xk], [yk], [wk])[0]
         A = sp.csc_matrix((p.A.csc_values(), p.A.csc_row_ind(), p.A.csc_col_ptr()),
                           shape=(p.num_rows(), p.num_cols()))
         ax = A @ xk
-        marg = 1e-7 * (1.0 + np.abs(xk))
-        isb = (xk - p.col_lower > marg) & (p.col_upper - xk > marg)
-        B = np.flatnonzero(isb)
+        isb = (xk - p.col_lower > 1e-7 * (1 + abs(xk))) & (p.col_upper - xk > 1e-7 * (1 + abs(xk)))
         fin = np.concatenate([p.row_lower[np.isfinite(p.row_lower)], p.row_upper[np.isfinite(p.row_upper)]])
-        tl = 1e-7 * (1.0 + np.abs(fin).max(initial=0.0))
-        isr = ((np.abs(yk) > 1e-7 * (1.0 + np.abs(yk).max())) |
-               (np.abs(ax - p.row_lower) <= tl) | (np.abs(ax - p.row_upper) <= tl))
-        act = np.flatnonzero(isr)
-        AB = A[act][:, B]
-        sys_res = np.linalg.norm(AB.T @ gk[act] - wk[B]) / np.linalg.norm(wk[B])
-        # reduced costs are accurate to about the KKT target times (1 + |c|)
-        lam_min = float(np.min(np.abs(lam[~isb]), initial=np.inf)) / (1.0 + np.abs(p.c).max())
-        dual_deg = int(np.sum(np.abs(lam[~isb]) <= 1e-7 * (1.0 + np.abs(p.c).max())))
-        log(f"LP {k}: min |reduced cost| of nonbasic columns / (1 + |c|) = {lam_min:.2e}")
-        if sys_res > 1e-6 or dual_deg > 0:
-            screened.append(dict(lp=k, system_residual=sys_res, zero_reduced_costs=dual_deg))
-            log(f"LP {k}: not a unique vertex (||A_RB^T mu - w_B||/||w_B|| = {sys_res:.1e}, "
-                f"{dual_deg} nonbasic columns with zero reduced cost): no finite difference")
+        tl = 1e-7 * (1 + np.abs(fin).max(initial=0.0))
+        isr = ((np.abs(yk) > 1e-7 * (1 + np.abs(yk).max())) | (np.abs(ax - p.row_lower) <= tl) |
+               (np.abs(ax - p.row_upper) <= tl))
+        B, R = np.flatnonzero(isb), np.flatnonzero(isr)
+        AB = A[R][:, B]
+        sres = np.linalg.norm(AB.T @ r.dl_db[R] - wk[B]) / np.linalg.norm(wk[B])
+        dual_deg = int(np.sum(np.abs(v0.reduced_costs[~isb]) <= 1e-9 * (1 + np.abs(p.c).max())))
+        if sres > 1e-6:
+            screened.append(dict(lp=k, system_residual=sres, zero_reduced_costs=dual_deg))
+            log(f"LP {k}: vertex |B|={B.size} |R|={R.size}: ||A_RB^T mu - w_B||/||w_B|| = {sres:.1e}: "
+                "A_RB short of column rank, no finite difference")
             continue
+        cb = np.array([int(z) for z in v0.col_basis], np.int8)
+        rb = np.array([int(z) for z in v0.row_basis], np.int8)
         v = rng.normal(size=B.size)
         d = np.zeros(p.num_rows())
-        d[act] = AB @ v
+        d[R] = AB @ v
         scale = np.linalg.norm(d)
         d /= scale
         v /= scale  # a unit shift h d moves x_B by h v
         room = np.minimum(xk[B] - p.col_lower[B], p.col_upper[B] - xk[B])
         hs = [1e-4 * max(1.0, np.abs(fin).max(initial=0.0)),
               0.5 * float(np.min(room / np.maximum(np.abs(v), 1e-300)))]
-        dv = A[~isr][:, B] @ v  # activity change of the inactive rows per unit step
-        lo_gap = (ax - p.row_lower)[~isr]
-        hi_gap = (p.row_upper - ax)[~isr]
+        dv = A[~isr][:, B] @ v
+        lo_gap, hi_gap = (ax - p.row_lower)[~isr], (p.row_upper - ax)[~isr]
         with np.errstate(divide="ignore", invalid="ignore"):
             lim = np.where(dv < 0, lo_gap / -dv, np.where(dv > 0, hi_gap / dv, np.inf))
         hs.append(0.5 * float(np.min(lim, initial=np.inf)))
         h = min(hs)
+        xs, pivots = [], []
         for sgn in (1.0, -1.0):
-            fd_probs.append(_shifted(p, p.c, sgn * h * d))
-        meta.append((k, d, h, B.size, act.size, sys_res))
-    sols = S.solve_batch(fd_probs, S.GpuPdhgConfig.enhanced(tol=1e-9, max_iterations=2000000))
-    fd_ok = all(s.status == S.SolutionStatus.Optimal for s in sols)
-    errs, sys_res_max, steps = [], 0.0, []
-    for i, (k, d, h, nb, nr, sres) in enumerate(meta):
-        wk = wn[co[k]:co[k + 1]]
-        fd = (wk @ sols[2 * i].x - wk @ sols[2 * i + 1].x) / (2 * h)
-        an = gb[ro[k]:ro[k + 1]] @ d
+            vs = S.polish(_shifted(p, p.c, sgn * h * d), cb, rb)
+            xs.append(vs.x if vs.status == S.SolutionStatus.Optimal else None)
+            pivots.append(vs.iterations)
+        if xs[0] is None or xs[1] is None:
+            screened.append(dict(lp=k, reason="a shifted LP did not end optimal from the basis"))
+            continue
+        fd = (wk @ xs[0] - wk @ xs[1]) / (2 * h)
+        an = r.dl_db @ d
         errs.append(abs(fd - an) / max(abs(fd), abs(an), 1e-12))
-        sys_res_max = max(sys_res_max, sres)
         steps.append(h)
-        log(f"LP {k}: |B|={nb} |R|={nr} h={h:.3g} FD {fd:.6e} adjoint {an:.6e} "
-            f"rel err {errs[-1]:.2e}; ||A_RB^T mu - w_B||/||w_B|| = {sres:.2e}")
+        checks.append(dict(lp=k, basic_cols=int(B.size), active_rows=int(R.size), h=h, fd=fd, adjoint=an,
+                           rel_error=errs[-1], system_residual=sres, zero_reduced_costs=dual_deg,
+                           shifted_pivots=pivots))
+        log(f"LP {k}: vertex |B|={B.size} |R|={R.size} ({dual_deg} zero reduced costs) h={h:.3g} FD {fd:.9e} "
+            f"adjoint {an:.9e} rel err {errs[-1]:.2e}; ||A_RB^T mu - w_B||/||w_B|| = {sres:.2e}; "
+            f"pivots from the basis {pivots}")
+    t_fd = time.perf_counter() - t_fd
     vert = adjoint_fd_at_vertices(rng)
     return {"metric": "batched forward + adjoint seconds (64 staircase LPs to 1e-8)",
             "value": t2 - t0, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * (t2 - t0),
@@ -615,9 +622,10 @@ def bench_adjoint(args):
             "forward_s": t1 - t0, "backward_s": t2 - t1,
             "dl_dc_all_zero": bool(torch.all(c.grad == 0)),
             "fd_max_rel_error": max(errs) if errs else None, "fd_checks": len(errs),
-            "fd_solves_converged": fd_ok, "fd_tol": 1e-9, "fd_steps": steps,
-            "adjoint_system_rel_residual_max": sys_res_max, "not_unique_vertex": screened,
-            "fd_at_vertices": vert}
+            "fd_at": "FP64 crossover vertices of the 48-block LPs (solve_hybrid, PDHG to 1e-9); shifted "
+                     "LPs re-solved by the FP64 simplex from the vertex's basis",
+            "fd_target": 1e-4, "fd_steps": steps, "fd_seconds": t_fd, "fd_detail": checks,
+            "not_unique_vertex": screened, "fd_at_vertices_4_blocks": vert}
 
 
 def _highs_solve(p, rl, ru):
@@ -746,6 +754,8 @@ def main():
                     help="large-ref: column bands of the A xbar product (mpdhg_config.spmv_bands: "
                          "0 = rule, 1 = none, k = k bands)")
     ap.add_argument("--no-vendor", action="store_true", help="large-ref: skip the rocSPARSE baseline")
+    ap.add_argument("--adjoint-fd", type=int, default=4,
+                    help="adjoint64: LPs differenced at their FP64 vertices")
     ap.add_argument("--blocks", type=int, default=48, choices=(12, 48, 96),
                     help="hybrid48: time blocks of the staircase LP (HiGHS fixtures exist for these)")
     ap.add_argument("--no-graph", action="store_true",

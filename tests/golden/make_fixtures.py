@@ -78,6 +78,70 @@ def write_lp(path: Path, p: S.Problem) -> None:
         fmt(p.row_upper)]) + "\n")
 
 
+def nondegenerate(lp: dict, sol: dict) -> bool:
+    """The screen of the B^-1 adjoint test: a square basis, basic columns
+    strictly inside their bounds, nonzero duals on the rows at a bound, no
+    basic slack at a bound."""
+    import scipy.sparse as sp
+
+    if sol["status"] != 0:
+        return False
+    bc = np.where(sol["col_basis"] == 0)[0]
+    ar = np.where(sol["row_basis"] != 0)[0]
+    if len(bc) != len(ar):
+        return False
+    x = sol["x"]
+    if not ((x[bc] - lp["l"][bc] > 1e-7) & (lp["u"][bc] - x[bc] > 1e-7)).all():
+        return False
+    if (np.abs(sol["row_duals"][ar]) <= 1e-9).any():
+        return False
+    A = sp.csc_matrix((lp["vals"], lp["row_ind"], lp["col_ptr"]), shape=(lp["m"], lp["n"]))
+    ax = A @ x
+    fin = np.concatenate([lp["rl"][np.isfinite(lp["rl"])], lp["ru"][np.isfinite(lp["ru"])]])
+    bn = 1 + np.abs(fin).max(initial=0)
+    slack = np.setdiff1d(np.arange(lp["m"]), ar)
+    return not (np.minimum(np.abs(ax - lp["rl"]), np.abs(ax - lp["ru"]))[slack] <= 1e-7 * bn).any()
+
+
+def perturbed_staircase(T: int, R: int, C: int, seed: int) -> S.Problem:
+    """A staircase LP moved into general position: the price and supply /
+    capacity / demand perturbations of the scenario generator, a small
+    exogenous supply U[0, 2] on every product balance (rl = ru = -supply,
+    instead of 0) and bounds jittered by up to 1%, so that its optimal vertex
+    is unlikely to be degenerate."""
+    p = S.gen_scenario_ex("staircase", (T, R, C), seed, seed + 1, shock_sigma=0.3, price_std=0.1,
+                          supply_std=0.15, capacity_std=0.08, demand_std=0.15)
+    rng = np.random.default_rng(seed)
+    bal = p.row_kind == 0
+    sup = rng.uniform(0.0, 2.0, p.num_rows())
+    p.row_lower = np.where(bal, -sup, p.row_lower)
+    p.row_upper = np.where(bal, -sup, p.row_upper * rng.uniform(0.99, 1.01, p.num_rows()))
+    p.col_upper = p.col_upper * rng.uniform(0.99, 1.01, p.num_cols())
+    return p
+
+
+def nondegenerate_staircases(td: Path, count: int, T: int, R: int, C: int, seeds):
+    out = []
+    for seed in seeds:
+        p = perturbed_staircase(T, R, C, seed)
+        path = td / "nd.txt"
+        write_lp(path, p)
+        res = subprocess.run([str(REFSIMPLEX), str(path)], check=True, capture_output=True, text=True,
+                             timeout=300)
+        sol = parse_solution(res.stdout)
+        A = p.A
+        lp = dict(m=p.num_rows(), n=p.num_cols(), col_ptr=A.csc_col_ptr(), row_ind=A.csc_row_ind(),
+                  vals=A.csc_values(), c=p.c, l=p.col_lower, u=p.col_upper, rl=p.row_lower, ru=p.row_upper)
+        ok = nondegenerate(lp, sol)
+        print(f"staircase T={T} R={R} C={C} seed {seed}: status {sol['status']}, nondegenerate {ok}")
+        if ok:
+            out.append((f"staircase_nd_T{T}_R{R}_C{C}_s{seed}",
+                        "sih_solver staircase generator, perturbed into general position", 1, lp, sol))
+        if len(out) == count:
+            break
+    return out
+
+
 def main():
     arrays, meta = {}, []
     with tempfile.TemporaryDirectory() as td:
@@ -107,6 +171,9 @@ def main():
                       row_ind=A.csc_row_ind(), vals=A.csc_values(), c=p.c, l=p.col_lower,
                       u=p.col_upper, rl=p.row_lower, ru=p.row_upper)
             cases.append((name, "sih_solver generator", 1, lp, parse_solution(res.stdout)))
+        # the refinery staircase family at nondegenerate vertices (B^-1 adjoint pin)
+        cases += nondegenerate_staircases(td, 1, 2, 40, 96, range(300, 340))
+        cases += nondegenerate_staircases(td, 1, 3, 60, 144, range(400, 440))
         for name, source, sense, lp, sol in cases:
             for k in ("col_ptr", "row_ind", "vals", "c", "l", "u", "rl", "ru"):
                 arrays[f"{name}/lp/{k}"] = np.asarray(lp[k])
