@@ -339,7 +339,7 @@ def bench_large(args, mode):
                 f"A x {v.us_spmv:.1f} us); engine {1e6 * sol.time_solver_sec / args.steps:.1f} us/iter")
     else:
         cfg = S.GpuPdhgConfig.enhanced(tol=1e-4, max_iterations=args.steps, profile=True,
-                                       spmv_tune=args.spmv_tune)
+                                       spmv_tune=args.spmv_tune, launch_graph=0 if args.no_graph else 1)
         sol = S.solve_gpu_pdhg(p, cfg)
         # the committed counters cover the sweep only
         rf, spmv = roofline(sol, enh_kernels(p.num_nonzeros()), traffic=False)
@@ -520,7 +520,7 @@ def bench_adjoint(args):
     probs = [S.gen_staircase(T, R, Cb, 100 + k) for k in range(K)]
     nnz = sum(p.num_nonzeros() for p in probs)
     log(f"generated {K} staircase LPs T={T} ({nnz} nonzeros) in {time.perf_counter() - t:.1f}s")
-    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=1000000)
+    cfg = S.GpuPdhgConfig.enhanced(tol=tol, max_iterations=1000000, launch_graph=0 if args.no_graph else 1)
     rng = np.random.default_rng(7)
     c = torch.tensor(np.concatenate([p.c for p in probs]), dtype=torch.float64, requires_grad=True)
     b = torch.zeros(sum(p.num_rows() for p in probs), dtype=torch.float64, requires_grad=True)

@@ -277,6 +277,52 @@ __device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
     return block_max_d<kInstBlock>(acc, red);
 }
 
+// canon_sum / canon_max of NQ quantities at once (group partials src[q][g];
+// is_max[q] picks the max): lane l runs through its groups b + l,
+// b + l + kInstBlock, ... once, carrying every quantity in exactly the order
+// canon_sum / canon_max would, and one block pass combines the wave partials
+// of all quantities with the same shuffle tree and the same sequential sum
+// over the waves.  out[q] equals the separate call's result bit for bit;
+// the quantities' loads interleave and the block synchronises twice instead
+// of twice per quantity.  `red` holds NQ * kInstBlock / 64 doubles.
+template <int NQ>
+__device__ __forceinline__ void canon_multi(int b, int e, const double *const (&src)[NQ],
+                                            const bool (&is_max)[NQ], double (&out)[NQ], double *red) {
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) {
+        double v[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[q] = src[q][g];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = is_max[q] ? fmax(acc[q], v[q]) : acc[q] + v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(acc[q], off, 64);
+            acc[q] = is_max[q] ? fmax(acc[q], o) : acc[q] + o;
+        }
+    }
+    constexpr int W = kInstBlock / 64;
+    const int w = static_cast<int>(threadIdx.x) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) red[q * W + w] = acc[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double r = red[q * W];
+#pragma unroll
+        for (int k = 1; k < W; ++k) r = is_max[q] ? fmax(r, red[q * W + k]) : r + red[q * W + k];
+        out[q] = r;
+    }
+}
+
 }  // namespace dev
 
 // Launch-argument prefix of a tile kernel over a tile set (in launcher

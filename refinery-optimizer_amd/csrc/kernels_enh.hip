@@ -355,7 +355,7 @@ __device__ __forceinline__ void set_steps(EnhInst &s) {
 
 __global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict__ st, int ninst,
                                                              FinalizeArgs A, int off) {
-    __shared__ double red[kInstBlock / 64];
+    __shared__ double red[6 * (kInstBlock / 64)];
     const int q = blockIdx.x;
     if (q >= ninst) return;
     EnhInst &s = st[q];
@@ -370,17 +370,19 @@ __global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict
     const int cb = s.cgrp_begin, ce = s.cgrp_end, rb = s.rgrp_begin, re = s.rgrp_end;
     const double *gc = A.gc, *gr = A.gr;
     const int gcn = A.gcn, grn = A.grn;
-    const double dx2 = canon_sum(cb, ce, [&](int g) { return gc[kGcDx2 * gcn + g]; }, red);
-    const double dxa2 = canon_sum(cb, ce, [&](int g) { return gc[kGcDxa2 * gcn + g]; }, red);
-    const double pobj = canon_sum(cb, ce, [&](int g) { return gc[kGcPobj * gcn + g]; }, red);
-    const double dobjc = canon_sum(cb, ce, [&](int g) { return gc[kGcDobj * gcn + g]; }, red);
-    const double dv = canon_max(cb, ce, [&](int g) { return gc[kGcDualViol * gcn + g]; }, red);
-    const double cv = canon_max(cb, ce, [&](int g) { return gc[kGcColViol * gcn + g]; }, red);
-    const double dy2 = canon_sum(rb, re, [&](int g) { return gr[kGrDy2 * grn + g]; }, red);
-    const double dya2 = canon_sum(rb, re, [&](int g) { return gr[kGrDya2 * grn + g]; }, red);
-    const double dobjr = canon_sum(rb, re, [&](int g) { return gr[kGrDobj * grn + g]; }, red);
-    const double rv = canon_max(rb, re, [&](int g) { return gr[kGrRowViol * grn + g]; }, red);
-    const double sv = canon_max(rb, re, [&](int g) { return gr[kGrSignViol * grn + g]; }, red);
+    // one fused pass per group kind; each quantity in canon_sum / canon_max's order
+    const double *const csrc[6] = {gc + kGcDx2 * gcn, gc + kGcDxa2 * gcn, gc + kGcPobj * gcn,
+                                   gc + kGcDobj * gcn, gc + kGcDualViol * gcn, gc + kGcColViol * gcn};
+    const bool cmax[6] = {false, false, false, false, true, true};
+    double cq[6];
+    canon_multi<6>(cb, ce, csrc, cmax, cq, red);
+    const double *const rsrc[5] = {gr + kGrDy2 * grn, gr + kGrDya2 * grn, gr + kGrDobj * grn,
+                                   gr + kGrRowViol * grn, gr + kGrSignViol * grn};
+    const bool rmax[5] = {false, false, false, true, true};
+    double rq[5];
+    canon_multi<5>(rb, re, rsrc, rmax, rq, red);
+    const double dx2 = cq[0], dxa2 = cq[1], pobj = cq[2], dobjc = cq[3], dv = cq[4], cv = cq[5];
+    const double dy2 = rq[0], dya2 = rq[1], dobjr = rq[2], rv = rq[3], sv = rq[4];
     if (threadIdx.x != 0) return;
 
     const int done = *A.itb + off + 1;
