@@ -114,7 +114,8 @@ typedef struct mpdhg_config {
                                      mpdhg_polish from that basis and point; 60 s per LP at most),
                                      and its result carries the vertex's status and objective and
                                      the PDHG objective's relative error against it. */
-    int32_t spmv_bands;           /* reference mode, one LP: split the A xbar product (CSR) into
+    int32_t spmv_bands;           /* one LP (either mode): split the A xbar product (CSR; in
+                                     enhanced mode the fused dual kernel's) into
                                      this many column bands, one launch each, so that each band's
                                      slice of xbar stays in an XCD's 4 MB L2 (rows carry their
                                      running sums from band to band in storage order: results do

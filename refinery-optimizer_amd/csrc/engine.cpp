@@ -179,6 +179,74 @@ int band_count(int forced, int64_t vec_len) {
     return static_cast<int>(std::min<int64_t>(kMaxBands, (bytes + kBandBytes - 1) / kBandBytes));
 }
 
+double build_bands(int bands, int rows, int64_t cols, const std::vector<int> &rptr,
+                   const std::vector<int> &ridx, const std::vector<float> &rval,
+                   const mpdhg_config &cfg, Layout lay, bool tune, float *vin, float *vout,
+                   hipStream_t st, ColumnBands &out) {
+    const size_t R = static_cast<size_t>(rows);
+    std::vector<int64_t> cb(static_cast<size_t>(bands) + 1);
+    for (int b = 0; b <= bands; ++b) cb[static_cast<size_t>(b)] = cols * b / bands;
+    out.bd = std::vector<ColumnBands::Band>(static_cast<size_t>(bands));
+    for (ColumnBands::Band &bd : out.bd) bd.rptr.assign(R + 1, 0);
+    // pass 1: nonzeros per row and band; pass 2: scatter in storage order
+    for (size_t i = 0; i < R; ++i) {
+        int b = 0;
+        for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
+            while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
+            out.bd[static_cast<size_t>(b)].rptr[i + 1]++;
+        }
+    }
+    for (ColumnBands::Band &bd : out.bd) {
+        for (size_t i = 0; i < R; ++i) bd.rptr[i + 1] += bd.rptr[i];
+        bd.ridx.resize(static_cast<size_t>(bd.rptr[R]));
+        bd.rval.resize(static_cast<size_t>(bd.rptr[R]));
+    }
+    std::vector<int> fill(static_cast<size_t>(bands));
+    for (size_t i = 0; i < R; ++i) {
+        for (int b = 0; b < bands; ++b) fill[static_cast<size_t>(b)] = out.bd[static_cast<size_t>(b)].rptr[i];
+        int b = 0;
+        for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
+            while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
+            ColumnBands::Band &bd = out.bd[static_cast<size_t>(b)];
+            const int d = fill[static_cast<size_t>(b)]++;
+            bd.ridx[static_cast<size_t>(d)] = ridx[static_cast<size_t>(k)];
+            bd.rval[static_cast<size_t>(d)] = rval[static_cast<size_t>(k)];
+        }
+    }
+    double tune_ms = 0.0;
+    Schedule bsc{0, 0};
+    for (size_t b = 0; b < out.bd.size(); ++b) {
+        ColumnBands::Band &bd = out.bd[b];
+        bd.d_rptr.upload(bd.rptr, st);
+        bd.d_ridx.upload(bd.ridx, st);
+        bd.d_rval.upload(bd.rval, st);
+        DirPlan P;
+        P.ptr = {&bd.rptr};
+        P.count = {rows};
+        P.row_off = {0};
+        P.nnz_off = {0};
+        P.grp_off = {0};
+        P.vec_off = {cb[b]};
+        P.vec_len = {cb[b + 1] - cb[b]};
+        P.rows = rows;
+        P.nnz = bd.rptr[R];
+        const DirSource BS{bd.d_ridx.get(), bd.d_rval.get(), nullptr};
+        if (b == 0 && tune) {
+            std::vector<TuneTrial> trials;
+            tune_ms += tune_dir(P, true, lay, BS, bd.d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true,
+                                false, vin, vout, nullptr, st, bd.dir, trials);
+            bsc = bd.dir.sc;
+        } else {
+            if (b == 0) bsc = choose_schedule(P.nnz, rows, cfg.spmv_block, cfg.spmv_chunk);
+            build_dir(P, bsc, lay, BS, true, false, st, bd.dir);
+        }
+    }
+    out.hpart.resize(R);
+    out.sets.clear();
+    for (const ColumnBands::Band &bd : out.bd) out.sets.push_back(bd.dir.set());
+    return tune_ms;
+}
+
 static inline bool finite_bound(double v) { return std::isfinite(v); }
 
 void host_kkt(const mpdhg_lp &lp, const HostLp &h, const std::vector<double> &x,
@@ -385,82 +453,13 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         build_dir(CP, choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk), lay, CS, true, false, st, CD);
     }
     // ---- column bands of the A xbar product (one LP whose xbar exceeds an
-    // XCD's L2): band b holds each row's nonzeros in columns [cb_b, cb_{b+1}),
-    // a contiguous piece of the row in storage order, as a CSR of its own ----
-    const int bands = (na == 1) ? band_count(cfg.spmv_bands, L.cols) : 1;
-    struct Band {
-        std::vector<int> rptr, ridx;
-        std::vector<float> rval;
-        DBuf<int> d_rptr, d_ridx;
-        DBuf<float> d_rval;
-        DevDir dir;
-    };
-    std::vector<Band> BD(bands > 1 ? static_cast<size_t>(bands) : 0);
-    DBuf<float> d_hpart;
-    if (bands > 1) {
-        std::vector<int64_t> cb(static_cast<size_t>(bands) + 1);
-        for (int b = 0; b <= bands; ++b) cb[static_cast<size_t>(b)] = L.cols * b / bands;
-        for (Band &bd : BD) bd.rptr.assign(R + 1, 0);
-        for (size_t i = 0; i < R; ++i) {
-            int b = 0;
-            for (int k = hh.rptr[i]; k < hh.rptr[i + 1]; ++k) {
-                while (hh.ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
-                BD[static_cast<size_t>(b)].rptr[i + 1]++;
-            }
-        }
-        for (Band &bd : BD) {
-            for (size_t i = 0; i < R; ++i) bd.rptr[i + 1] += bd.rptr[i];
-            bd.ridx.resize(static_cast<size_t>(bd.rptr[R]));
-            bd.rval.resize(static_cast<size_t>(bd.rptr[R]));
-        }
-        std::vector<int> fill(static_cast<size_t>(bands));
-        for (size_t i = 0; i < R; ++i) {
-            for (int b = 0; b < bands; ++b) fill[static_cast<size_t>(b)] = BD[static_cast<size_t>(b)].rptr[i];
-            int b = 0;
-            for (int k = hh.rptr[i]; k < hh.rptr[i + 1]; ++k) {
-                while (hh.ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
-                Band &bd = BD[static_cast<size_t>(b)];
-                const int d = fill[static_cast<size_t>(b)]++;
-                bd.ridx[static_cast<size_t>(d)] = hh.ridx[static_cast<size_t>(k)];
-                bd.rval[static_cast<size_t>(d)] = hh.rval[static_cast<size_t>(k)];
-            }
-        }
-        // Each band gathers from its own slice of xbar: the slice start is the
-        // tiles' gather base, so the packed index word of the gather-order
-        // layout only has to span the band (larger chunks qualify).  Every
-        // band uses the schedule searched (or chosen by rule) on the first.
-        const int hm = H[static_cast<size_t>(act[0])].m;
-        Schedule bsc{0, 0};
-        for (size_t b = 0; b < BD.size(); ++b) {
-            Band &bd = BD[b];
-            bd.d_rptr.upload(bd.rptr, st);
-            bd.d_ridx.upload(bd.ridx, st);
-            bd.d_rval.upload(bd.rval, st);
-            DirPlan P;
-            P.ptr = {&bd.rptr};
-            P.count = {hm};
-            P.row_off = {0};
-            P.nnz_off = {0};
-            P.grp_off = {0};
-            P.vec_off = {cb[b]};
-            P.vec_len = {cb[b + 1] - cb[b]};
-            P.rows = L.rows;
-            P.nnz = bd.rptr[R];
-            const DirSource BS{bd.d_ridx.get(), bd.d_rval.get(), nullptr};
-            if (b == 0 && tune_wanted(cfg, P.nnz) && !noalloc) {
-                std::vector<TuneTrial> trials;
-                tune_ms += tune_dir(P, true, lay, BS, bd.d_rptr.get(), cfg.spmv_block, cfg.spmv_chunk, true,
-                                    false, d_x.get(), d_y.get(), nullptr, st, bd.dir, trials);
-                bsc = bd.dir.sc;
-            } else {
-                if (b == 0) bsc = choose_schedule(P.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk);
-                build_dir(P, bsc, lay, BS, true, false, st, bd.dir);
-            }
-        }
-        d_hpart.resize(R);
-    }
-    std::vector<TileSet> bts;
-    for (const Band &bd : BD) bts.push_back(bd.dir.set());
+    // XCD's L2; the no-allocation solve keeps the unbanded product, whose
+    // results are the same bit for bit) ----
+    const int bands = (na == 1 && !noalloc) ? band_count(cfg.spmv_bands, L.cols) : 1;
+    ColumnBands BD;
+    if (bands > 1)
+        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr, hh.ridx, hh.rval, cfg, lay,
+                               tune_wanted(cfg, L.nnz) && !noalloc, d_x.get(), d_y.get(), st, BD);
     for (int a = 0; a < na; ++a) {
         RefInst &s = hh.inst[static_cast<size_t>(a)];
         s.row_tile_begin = RD.first[static_cast<size_t>(a)];
@@ -512,9 +511,9 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
                                e ? e[1] : nullptr);
             if (bands > 1) {
                 for (int b = 0; b < bands; ++b) {
-                    const Band &bd = BD[static_cast<size_t>(b)];
-                    launch::ref_dual_band(st, bts[static_cast<size_t>(b)], d_inst.get(), bd.d_rptr.get(),
-                                          bd.dir.idx, bd.dir.v32, d_xbar.get(), d_hpart.get(), b == 0,
+                    const ColumnBands::Band &bd = BD.bd[static_cast<size_t>(b)];
+                    launch::ref_dual_band(st, BD.sets[static_cast<size_t>(b)], d_inst.get(), bd.d_rptr.get(),
+                                          bd.dir.idx, bd.dir.v32, d_xbar.get(), BD.hpart.get(), b == 0,
                                           b == bands - 1, d_rl.get(), d_ru.get(), d_sigma.get(), d_y.get(),
                                           (e && b == 0) ? e[2] : nullptr,
                                           (e && b == bands - 1) ? e[3] : nullptr);
@@ -536,7 +535,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
         launch::ref_copy_best(st, cts, d_inst.get(), d_x.get(), d_xbest.get());
         launch::tile_run(st, rts, d_inst.get());
         launch::tile_run(st, cts, d_inst.get());
-        for (const TileSet &bt : bts) launch::tile_run(st, bt, d_inst.get());
+        for (const TileSet &bt : BD.sets) launch::tile_run(st, bt, d_inst.get());
         launch::enh_advance(st, d_itb.get(), len);
     };
     struct CycleGraph {
@@ -649,8 +648,8 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     }
     for (int a = 0; a < na; ++a) {
         mpdhg_result &r = res[act[static_cast<size_t>(a)]];
-        r.csr_block = bands > 1 ? BD[0].dir.sc.block : RD.sc.block;  // the schedule of A xbar
-        r.csr_chunk = bands > 1 ? BD[0].dir.sc.tn : RD.sc.tn;
+        r.csr_block = bands > 1 ? BD.bd[0].dir.sc.block : RD.sc.block;  // the schedule of A xbar
+        r.csr_chunk = bands > 1 ? BD.bd[0].dir.sc.tn : RD.sc.tn;
         r.csc_block = CD.sc.block;
         r.csc_chunk = CD.sc.tn;
         r.tune_ms = tune_ms;

@@ -277,6 +277,35 @@ double tune_dir(const DirPlan &P, bool csr, Layout layout, const DirSource &src,
                 int forced_block, int forced_tn, bool want32, bool want64, float *vin, float *vout,
                 double *gpart, hipStream_t st, DevDir &out, std::vector<TuneTrial> &trials);
 
+// Column bands of one LP's CSR product (for an LP whose gathered vector
+// exceeds an XCD's L2): band b holds each row's nonzeros in columns
+// [cb_b, cb_{b+1}), cb_b = cols * b / bands, a contiguous piece of the row in
+// storage order, as a CSR of its own; its tiles gather from that slice of the
+// vector (the slice start is the tiles' gather base, so the packed index word
+// of the gather-order layout only has to span the band).
+struct ColumnBands {
+    struct Band {
+        std::vector<int> rptr, ridx;
+        std::vector<float> rval;
+        DBuf<int> d_rptr, d_ridx;
+        DBuf<float> d_rval;
+        DevDir dir;
+    };
+    std::vector<Band> bd;
+    std::vector<TileSet> sets;  // one tile set per band
+    DBuf<float> hpart;          // running row sums between band passes
+    int count() const { return static_cast<int>(bd.size()); }
+};
+// Splits the rows x cols CSR (rptr / ridx / rval, column indices ascending
+// within each row) into `bands` > 1 bands and uploads them.  The first band's
+// schedule is searched when `tune` (vin / vout: scratch vectors of cols and
+// rows floats) or else chosen by rule, and every band uses it.  Returns the
+// search time in ms.
+double build_bands(int bands, int rows, int64_t cols, const std::vector<int> &rptr,
+                   const std::vector<int> &ridx, const std::vector<float> &rval,
+                   const mpdhg_config &cfg, Layout lay, bool tune, float *vin, float *vout,
+                   hipStream_t st, ColumnBands &out);
+
 // FP64 post-processing on the host for reference-mode results: A x,
 // c + A^T y and the relative KKT of (x, y).
 void host_kkt(const mpdhg_lp &lp, const HostLp &h, const std::vector<double> &x,

@@ -270,6 +270,14 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         build_dir(RP, choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk), lay, RS, true, true, st, RD);
         build_dir(CP, choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk), lay, CS, true, true, st, CD);
     }
+    // ---- column bands of the A xbar product of the fused dual kernel (one
+    // LP whose xbar exceeds an XCD's L2; bit-identical to the unbanded
+    // product, which the no-allocation solve keeps) ----
+    const int bands = (na == 1 && !noalloc) ? band_count(cfg.spmv_bands, L.cols) : 1;
+    ColumnBands BD;
+    if (bands > 1)
+        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr, hh.ridx, hh.rval, cfg, lay,
+                               tune_wanted(cfg, L.nnz) && !noalloc, d_x.get(), d_y.get(), st, BD);
     for (int a = 0; a < na; ++a) {
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
         s.row_tile_begin = RD.first[static_cast<size_t>(a)];
@@ -307,6 +315,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         return b;
     };
     rts.stream_nt = cts.stream_nt = stream_bytes([](int) { return true; }) > kNtStreamBytes;
+    for (TileSet &bt : BD.sets) bt.stream_nt = rts.stream_nt;
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
@@ -389,14 +398,32 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     std::vector<std::vector<Tile>> host_keep_rt, host_keep_ct;  // sources of their async uploads
     // One check cycle of len iterations in stream order: F fused launch pairs,
     // the FP64 check and decisions, the iteration advance.
+    // The fused dual step: one launch, or one per column band (events, when
+    // given, at the first band's start and the last band's end).
+    auto dual = [&](int i, int check, hipEvent_t e0, hipEvent_t e1) {
+        if (bands == 1) {
+            launch::enh_dual(st, rts, d_inst.get(), D, i, check, e0, e1);
+            return;
+        }
+        for (int b = 0; b < bands; ++b) {
+            const ColumnBands::Band &bd = BD.bd[static_cast<size_t>(b)];
+            launch::enh_dual_band(st, BD.sets[static_cast<size_t>(b)], d_inst.get(), D, bd.d_rptr.get(),
+                                  bd.dir.idx, bd.dir.v32, BD.hpart.get(), b == 0, b == bands - 1, i, check,
+                                  b == 0 ? e0 : nullptr, b == bands - 1 ? e1 : nullptr);
+        }
+    };
+    auto check_step = [&](int len) {
+        launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
+        for (const TileSet &bt : BD.sets) launch::tile_run(st, bt, d_inst.get());
+        launch::enh_advance(st, d_itb.get(), len);
+    };
     auto issue_cycle = [&](int len) {
         for (int i = 0; i < len; ++i) {
             const int check = (i + 1 == len);
             launch::enh_primal(st, cts, d_inst.get(), P, i, check);
-            launch::enh_dual(st, rts, d_inst.get(), D, i, check);
+            dual(i, check, nullptr, nullptr);
         }
-        launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
-        launch::enh_advance(st, d_itb.get(), len);
+        check_step(len);
     };
     // Full cycles replay a HIP graph recorded from issue_cycle(F); it is
     // recorded again after each tile-list compaction (new lists and grids).
@@ -432,11 +459,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     const bool samp = used < nsamp && !check;
                     hipEvent_t *e = samp ? ctx.events.data() + 4 * used : nullptr;
                     launch::enh_primal(st, cts, d_inst.get(), P, i, check, e ? e[0] : nullptr, e ? e[1] : nullptr);
-                    launch::enh_dual(st, rts, d_inst.get(), D, i, check, e ? e[2] : nullptr, e ? e[3] : nullptr);
+                    dual(i, check, e ? e[2] : nullptr, e ? e[3] : nullptr);
                     if (samp) samp_it[static_cast<size_t>(used++)] = it + i;
                 }
-                launch::enh_check(st, rts, cts, d_inst.get(), na, CK, len - 1);
-                launch::enh_advance(st, d_itb.get(), len);
+                check_step(len);
             } else if (cfg.launch_graph && len == F && !noalloc) {
                 if (graph_lists != lists) {
                     graph.retire();
@@ -552,8 +578,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         r.rel_kkt = s.best_kkt;
         r.step_size = s.eta;
         r.primal_weight = s.omega;
-        r.csr_block = RD.sc.block;
-        r.csr_chunk = RD.sc.tn;
+        r.csr_block = bands > 1 ? BD.bd[0].dir.sc.block : RD.sc.block;  // the schedule of A xbar
+        r.csr_chunk = bands > 1 ? BD.bd[0].dir.sc.tn : RD.sc.tn;
+        r.csr_bands = bands;
         r.csc_block = CD.sc.block;
         r.csc_chunk = CD.sc.tn;
         r.tune_ms = tune_ms;

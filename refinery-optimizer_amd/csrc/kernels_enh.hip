@@ -177,10 +177,21 @@ struct DualArgs {
     float rho;
 };
 
-template <int B, int TN, bool G, bool NT>
+// One pass of a column-banded A xbar: band b holds each row's nonzeros in
+// columns [cb_b, cb_{b+1}) as a CSR of its own (A.rptr / A.ridx / A.rval of
+// the pass).  The first pass starts each row's sum at zero, later passes
+// continue the FMA chain from the running sum in hpart, and every pass but
+// the last stores its running sums and ends there, so the last pass sees the
+// unbanded product bit for bit.
+struct BandPass {
+    float *hpart;
+    int first, last;
+};
+
+template <int B, int TN, bool G, bool NT, bool BAND>
 __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, int ntiles, int chunk,
                                               const EnhInst *__restrict__ st, const DualArgs &A,
-                                              int off, int check) {
+                                              int off, int check, BandPass bp) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
@@ -189,18 +200,24 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
     const TileRows tr = tile_rows(tl, A.rptr);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     const bool mine = i < tl.end;
-    float yo = 0.0f, y0i = 0.0f, yb = 0.0f, lo = 0.0f, hi = 0.0f;
-    if (mine) {
+    const bool epilogue = !BAND || bp.last;
+    float yo = 0.0f, y0i = 0.0f, yb = 0.0f, lo = 0.0f, hi = 0.0f, h0 = 0.0f;
+    if (mine && epilogue) {
         yo = A.y[i];
         y0i = A.y0[i];
         yb = A.ybar[i];
         lo = A.rl[i];
         hi = A.ru[i];
     }
+    if (BAND && mine && !bp.first) h0 = bp.hpart[i];
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.ridx, A.rval, A.xbar, lds);
+    const float h = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.ridx, A.rval, A.xbar, lds, h0);
+    if (BAND && !bp.last) {
+        if (mine) bp.hpart[i] = h;
+        return;
+    }
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -246,9 +263,19 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
     }
 MPDHG_ENH_KERNEL(k_enh_primal, enh_primal_body, PrimalArgs, false)
 MPDHG_ENH_KERNEL(k_enh_primal_nt, enh_primal_body, PrimalArgs, true)
-MPDHG_ENH_KERNEL(k_enh_dual, enh_dual_body, DualArgs, false)
-MPDHG_ENH_KERNEL(k_enh_dual_nt, enh_dual_body, DualArgs, true)
 #undef MPDHG_ENH_KERNEL
+#define MPDHG_ENH_DUAL(NAME, NT, BAND)                                                           \
+    template <int B, int TN, bool G>                                                             \
+    __global__ __launch_bounds__(B) void NAME(const Tile *__restrict__ tiles, int ntiles,       \
+                                              int chunk, const EnhInst *__restrict__ st,         \
+                                              DualArgs A, int off, int check, BandPass bp) {     \
+        enh_dual_body<B, TN, G, NT, BAND>(tiles, ntiles, chunk, st, A, off, check, bp);          \
+    }
+MPDHG_ENH_DUAL(k_enh_dual, false, false)
+MPDHG_ENH_DUAL(k_enh_dual_nt, true, false)
+MPDHG_ENH_DUAL(k_enh_dual_band, false, true)
+MPDHG_ENH_DUAL(k_enh_dual_band_nt, true, true)
+#undef MPDHG_ENH_DUAL
 
 // ---- FP64 checks on the original (unscaled) LP ----------------------------
 __device__ __forceinline__ bool fin(double v) { return isfinite(v); }
@@ -620,15 +647,32 @@ void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDua
               int check, hipEvent_t e0, hipEvent_t e1) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
                     a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
+    const dev::BandPass bp{nullptr, 1, 1};
     if (!rts.count) return;
     if (e0 && rts.stream_nt)
-        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual_nt, st, A, off, check);
+        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual_nt, st, A, off, check, bp);
     else if (e0)
-        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual, st, A, off, check);
+        MPDHG_LAUNCH_TN_EV(rts, e0, e1, dev::k_enh_dual, st, A, off, check, bp);
     else if (rts.stream_nt)
-        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual_nt, st, A, off, check);
+        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual_nt, st, A, off, check, bp);
     else
-        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check);
+        MPDHG_LAUNCH_TN(rts, dev::k_enh_dual, st, A, off, check, bp);
+}
+void enh_dual_band(hipStream_t s, const TileSet &bts, const EnhInst *st, const EnhDual &a,
+                   const int *rptr, const int *ridx, const float *rval, float *hpart, int first,
+                   int last, int off, int check, hipEvent_t e0, hipEvent_t e1) {
+    dev::DualArgs A{rptr, ridx, rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
+                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
+    const dev::BandPass bp{hpart, first, last};
+    if (!bts.count) return;
+    if ((e0 || e1) && bts.stream_nt)
+        MPDHG_LAUNCH_TN_EV(bts, e0, e1, dev::k_enh_dual_band_nt, st, A, off, check, bp);
+    else if (e0 || e1)
+        MPDHG_LAUNCH_TN_EV(bts, e0, e1, dev::k_enh_dual_band, st, A, off, check, bp);
+    else if (bts.stream_nt)
+        MPDHG_LAUNCH_TN(bts, dev::k_enh_dual_band_nt, st, A, off, check, bp);
+    else
+        MPDHG_LAUNCH_TN(bts, dev::k_enh_dual_band, st, A, off, check, bp);
 }
 void enh_advance(hipStream_t s, int *itb, int len) {
     hipLaunchKernelGGL(dev::k_enh_advance, dim3(1), dim3(64), 0, s, itb, len);

@@ -115,6 +115,11 @@ void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhP
                 int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
               int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// One pass of a column-banded A xbar (k_enh_dual_band): the band's own CSR
+// replaces a.rptr / a.ridx / a.rval; the last pass runs the dual update.
+void enh_dual_band(hipStream_t s, const TileSet &bts, const EnhInst *st, const EnhDual &a,
+                   const int *rptr, const int *ridx, const float *rval, float *hpart, int first,
+                   int last, int off, int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // *itb += len (end of a check cycle of len iterations).
 void enh_advance(hipStream_t s, int *itb, int len);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)

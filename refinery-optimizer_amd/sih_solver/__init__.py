@@ -331,7 +331,7 @@ class GpuPdhgConfig:
     spmv_block: int = 0  # threads per SpMV workgroup: 0 = auto, or 128 / 256 / 512
     spmv_tune: int = 1  # 1: time the schedules on the batch when it is large, 2: always, 0: rule
     cpu_fallback: int = 0  # 1: with no HIP device, solve on the host FP64 simplex (gpu_solver.cpp:66-71)
-    spmv_bands: int = 0  # reference mode, one LP: column bands of A xbar (0 rule, 1 none, k bands)
+    spmv_bands: int = 0  # one LP: column bands of A xbar (0 rule, 1 none, k bands)
     vertex_check: int = 0  # k > 0: FP64 crossover of every LP with <= k rows after the solve
 
     @staticmethod
