@@ -28,6 +28,8 @@ Other workloads:
              then the FP64 crossover on the host, against HiGHS's optimum.
   copy       a 256 MiB device-to-device elementwise copy (torch), the
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
+  stream     the device's streaming ceilings: float4 kernels reading 1 GiB
+             (cached and non-temporal) and copying it, best of five grids.
 
 The roofline object reports the dominant fused SpMV kernel: algorithmic
 bytes per launch / its average launch duration from HIP events recorded on
@@ -732,6 +734,23 @@ def bench_copy(args):
                          "frac": gbs / HBM_PEAK_GBS, "traffic": 2.0 * 4 * n}}
 
 
+def bench_stream(args):
+    """Streaming ceilings of the device (bench-only vendor library): best GB/s
+    of float4 kernels reading 1 GiB through the caches, reading it
+    non-temporally, and copying it -- the practical HBM bandwidth against
+    which the fused kernels' roofline fractions are read."""
+    from sih_solver import vendor
+
+    r = vendor.stream(1 << 30, max(args.steps, 1))
+    return {"metric": "device streaming GB/s (HBM ceilings)", "value": r["read_nt_gbs"], "unit": "GB/s",
+            "ms_per_step": 1e3 * (1 << 30) / (r["read_nt_gbs"] * 1e9), "dtype": "f32",
+            "config": {"workload": "stream: 1 GiB float4 read (cached, non-temporal) and copy, "
+                                   "grids of 1024-16384 workgroups of 256, best grid"},
+            "read_gbs": r["read_gbs"], "read_nt_gbs": r["read_nt_gbs"], "copy_gbs": r["copy_gbs"],
+            "roofline": {"bound": "hbm", "achieved": r["read_nt_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": r["read_nt_gbs"] / HBM_PEAK_GBS, "traffic": float(1 << 30)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -739,7 +758,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
                     choices=["sweep512", "cfg0", "large-ref", "large-enh", "staircase96", "adjoint64",
-                             "hybrid48", "copy"])
+                             "hybrid48", "copy", "stream"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
@@ -776,6 +795,8 @@ def main():
         out = bench_sweep(args)
     elif args.workload == "copy":
         out = bench_copy(args)
+    elif args.workload == "stream":
+        out = bench_stream(args)
     elif args.workload == "cfg0":
         out = bench_cfg0(args)
     elif args.workload == "staircase96":

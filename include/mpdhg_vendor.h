@@ -37,6 +37,19 @@ int mpdhg_vendor_pdhg(const mpdhg_lp *lp, int iterations, int transpose_mode, do
                       mpdhg_vendor_result *out);
 const char *mpdhg_vendor_last_error(void);
 
+/* Streaming ceilings of device 0 (no counterpart in the reference: the
+ * practical HBM bandwidth the fused kernels' roofline fractions are read
+ * against).  Over a buffer of `bytes` (a multiple of 16), float4 kernels with
+ * grid-stride loops at 1024 ... 16384 workgroups of 256 threads; each figure
+ * is the best grid's bytes moved / time, over `reps` timed launches after one
+ * untimed one (HIP events). */
+typedef struct mpdhg_vendor_stream_result {
+    double read_gbs;     /* reads through the caches (a sum kept in registers) */
+    double read_nt_gbs;  /* non-temporal reads */
+    double copy_gbs;     /* copy: bytes read + bytes written */
+} mpdhg_vendor_stream_result;
+int mpdhg_vendor_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out);
+
 #ifdef __cplusplus
 }
 #endif

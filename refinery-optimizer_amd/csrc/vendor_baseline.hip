@@ -225,9 +225,81 @@ int run_baseline(const mpdhg_lp *lp, int iters, int mode, double *xout, mpdhg_ve
     return MPDHG_OK;
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Reads a[0, n) with a grid-stride loop; the sum is stored only when it
+// takes a value it never takes (the buffer is zero), so the loads stay.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_read(const f4 *__restrict__ a, size_t n, float *__restrict__ out) {
+    float s = 0.0f;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256ull) {
+        f4 v;
+        if constexpr (NT) v = __builtin_nontemporal_load(a + i);
+        else v = a[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    if (s == 1.0f) out[0] = s;
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256ull)
+        b[i] = a[i];
+}
+
+int run_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out) {
+    if (!out || bytes < 16 || bytes % 16 != 0 || reps < 1) throw Fail{MPDHG_E_ARG, "bytes must be a positive multiple of 16, reps >= 1"};
+    hip_ok(hipSetDevice(0), "hipSetDevice");
+    const size_t n = bytes / 16;
+    Dev<f4> a(n), b(n);
+    Dev<float> o(1);
+    hip_ok(hipMemset(a.p, 0, bytes), "hipMemset");
+    hip_ok(hipMemset(b.p, 0, bytes), "hipMemset");
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (auto &v : ev.e) hip_ok(hipEventCreate(&v), "hipEventCreate");
+    double best[3] = {0.0, 0.0, 0.0};
+    for (int grid : {1024, 2048, 4096, 8192, 16384}) {
+        for (int kind = 0; kind < 3; ++kind) {
+            auto launch = [&] {
+                if (kind == 0) hipLaunchKernelGGL(k_stream_read<false>, dim3(grid), dim3(256), 0, 0, a.p, n, o.p);
+                else if (kind == 1) hipLaunchKernelGGL(k_stream_read<true>, dim3(grid), dim3(256), 0, 0, a.p, n, o.p);
+                else hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, a.p, b.p, n);
+            };
+            launch();
+            hip_ok(hipGetLastError(), "stream launch");
+            hip_ok(hipEventRecord(ev.e[0], 0), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) launch();
+            hip_ok(hipEventRecord(ev.e[1], 0), "hipEventRecord");
+            hip_ok(hipEventSynchronize(ev.e[1]), "hipEventSynchronize");
+            float ms = 0.0f;
+            hip_ok(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "hipEventElapsedTime");
+            const double moved = (kind == 2 ? 2.0 : 1.0) * static_cast<double>(bytes) * reps;
+            best[kind] = std::max(best[kind], moved / (1e-3 * ms) / 1e9);
+        }
+    }
+    out->read_gbs = best[0];
+    out->read_nt_gbs = best[1];
+    out->copy_gbs = best[2];
+    return MPDHG_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpdhg_vendor_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out) {
+    try {
+        return run_stream(bytes, reps, out);
+    } catch (const Fail &f) {
+        g_err = f.msg;
+        return f.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return MPDHG_E_STATE;
+    }
+}
 
 int mpdhg_vendor_pdhg(const mpdhg_lp *lp, int iterations, int transpose_mode, double *x,
                       mpdhg_vendor_result *out) {

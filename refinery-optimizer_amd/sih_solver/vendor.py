@@ -2,7 +2,9 @@
 
 bench/cusparse_pdhg_baseline.cu:60-255 counterpart: a fixed number of
 reference PDHG iterations with rocSPARSE products, for comparison with the
-engine's fused kernels.  Not used by any solve path.
+engine's fused kernels, and ``stream()``, the device's streaming ceilings
+(the practical HBM bandwidth of plain float4 read and copy kernels).  Not
+used by any solve path.
 """
 from __future__ import annotations
 
@@ -27,7 +29,12 @@ class MpdhgVendorResult(C.Structure):
                 ("objective", C.c_double)]
 
 
+class MpdhgVendorStreamResult(C.Structure):
+    _fields_ = [("read_gbs", C.c_double), ("read_nt_gbs", C.c_double), ("copy_gbs", C.c_double)]
+
+
 SIGNATURES = {
+    "mpdhg_vendor_stream": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(MpdhgVendorStreamResult)]),
     "mpdhg_vendor_pdhg": (C.c_int, [C.POINTER(MpdhgLp), C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(MpdhgVendorResult)]),
     "mpdhg_vendor_last_error": (C.c_char_p, []),
@@ -79,3 +86,14 @@ def run(problem: Problem, iterations: int, mode: int = TRANSPOSE_OP) -> VendorRu
                            f"{lib().mpdhg_vendor_last_error().decode(errors='replace')}")
     del keep
     return VendorRun(r.us_per_iter, r.us_spmv_t, r.us_spmv, r.objective, x)
+
+
+def stream(bytes_: int = 1 << 30, reps: int = 10) -> dict:
+    """Streaming ceilings of device 0 (mpdhg_vendor_stream): best GB/s of a
+    cached float4 read, a non-temporal read and a copy (read + write bytes)."""
+    r = MpdhgVendorStreamResult()
+    rc = lib().mpdhg_vendor_stream(bytes_, reps, C.byref(r))
+    if rc != 0:
+        raise RuntimeError(f"mpdhg_vendor_stream failed (code {rc}): "
+                           f"{lib().mpdhg_vendor_last_error().decode(errors='replace')}")
+    return {"read_gbs": r.read_gbs, "read_nt_gbs": r.read_nt_gbs, "copy_gbs": r.copy_gbs}
