@@ -71,15 +71,15 @@ def log(msg: str) -> None:
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_sweep512.md"
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed PMC summary, or None.
+def measured_traffic(kernel: str, summary: Path = TRAFFIC_FILE):
+    """HBM bytes per launch of `kernel` from a committed PMC summary of its
+    workload (the sweep's by default), or None.
 
     The summary holds one line `traffic_bytes <kernel> <bytes>` per kernel
     (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor, in bytes)."""
-    if not TRAFFIC_FILE.exists():
+    if summary is None or not summary.exists():
         return None
-    m = re.search(rf"^traffic_bytes {re.escape(kernel)} (\d+)\s*$", TRAFFIC_FILE.read_text(),
-                  re.MULTILINE)
+    m = re.search(rf"^traffic_bytes {re.escape(kernel)} (\d+)\s*$", summary.read_text(), re.MULTILINE)
     return float(m.group(1)) if m else None
 
 
@@ -94,7 +94,7 @@ def enh_kernels(nnz: int):
     return ("k_enh_primal" + sfx, "k_enh_dual" + sfx)
 
 
-def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=True):
+def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=TRAFFIC_FILE):
     n_l = max(sol.kernel_launches_csc, 1)
     csc_us = 1e3 * sol.kernel_ms_spmv_csc / n_l
     csr_us = 1e3 * sol.kernel_ms_spmv_csr / n_l
@@ -105,7 +105,7 @@ def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=True):
     else:
         name, us, b, g = kernels[1], csr_us, sol.bytes_csr_launch, gbs_csr
     return ({"bound": "hbm", "kernel": name, "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": g / HBM_PEAK_GBS, "traffic": measured_traffic(name) if traffic else None,
+             "frac": g / HBM_PEAK_GBS, "traffic": measured_traffic(name, traffic) if traffic else None,
              "bytes_per_launch": b, "us_per_launch": us},
             {"csc_fused_gbs": gbs_csc, "csr_fused_gbs": gbs_csr, "csc_fused_us": csc_us,
              "csr_fused_us": csr_us,
@@ -321,7 +321,9 @@ def bench_large(args, mode):
         sol = S.solve_gpu_pdhg(p, cfg)
         # kernel timings from a separate run whose first cycles launch eagerly with events
         cfg.profile, cfg.max_iterations = True, min(args.steps, 100)
-        rf, spmv = roofline(S.solve_gpu_pdhg(p, cfg), ("k_ref_primal", "k_ref_dual"))
+        kd = "k_ref_dual" if sol.csr_bands == 1 else f"k_ref_dual_band x{sol.csr_bands}"
+        rf, spmv = roofline(S.solve_gpu_pdhg(p, cfg), ("k_ref_primal", kd),
+                            traffic=ROOT / "profiles" / "pmc_traffic_large_ref.md")
         out = {"metric": "PDHG iterations/s (reference mode); SpMV HBM GB/s vs peak",
                "value": args.steps / sol.time_solver_sec, "unit": "iterations/s",
                "ms_per_step": 1e3 * sol.time_solver_sec / args.steps, "dtype": "fp32",
@@ -347,8 +349,7 @@ def bench_large(args, mode):
         kp, kd = enh_kernels(p.num_nonzeros())
         if sol.csr_bands > 1:  # the dual product's time spans its band launches
             kd = kd.replace("k_enh_dual", "k_enh_dual_band") + f" x{sol.csr_bands}"
-        # the committed counters cover the sweep only
-        rf, spmv = roofline(sol, (kp, kd), traffic=False)
+        rf, spmv = roofline(sol, (kp, kd), traffic=ROOT / "profiles" / "pmc_traffic_large_enh.md")
         out = {"metric": "time to 1e-4 rel KKT (enhanced); SpMV HBM GB/s vs peak",
                "value": sol.time_solver_sec, "unit": "s", "higher_is_better": False,
                "ms_per_step": 1e3 * sol.time_solver_sec / max(sol.simplex_iterations, 1),

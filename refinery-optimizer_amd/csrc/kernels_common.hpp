@@ -264,16 +264,38 @@ __device__ __forceinline__ void put_group(double *part, const Tile &tl, double v
 // of a per-instance kernel over an instance's groups; with per-group
 // partials formed by one wave each, the result does not depend on how the
 // rows were tiled.  `red` holds kInstBlock / 64 doubles.
+// A lane's walk issues the loads of kCanonUnroll steps before it adds them,
+// in the same order, so that one instance's thousands of groups do not cost
+// one memory latency per step.
+constexpr int kCanonUnroll = 8;
 template <typename F>
 __device__ __forceinline__ double canon_sum(int b, int e, F f, double *red) {
+    constexpr int U = kCanonUnroll;
     double acc = 0.0;
-    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) acc += f(g);
+    int g = b + static_cast<int>(threadIdx.x);
+    for (; g + (U - 1) * kInstBlock < e; g += U * kInstBlock) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = f(g + u * kInstBlock);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u];
+    }
+    for (; g < e; g += kInstBlock) acc += f(g);
     return block_sum_d<kInstBlock>(acc, red);
 }
 template <typename F>
 __device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
+    constexpr int U = kCanonUnroll;
     double acc = 0.0;
-    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) acc = fmax(acc, f(g));
+    int g = b + static_cast<int>(threadIdx.x);
+    for (; g + (U - 1) * kInstBlock < e; g += U * kInstBlock) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = f(g + u * kInstBlock);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = fmax(acc, v[u]);
+    }
+    for (; g < e; g += kInstBlock) acc = fmax(acc, f(g));
     return block_max_d<kInstBlock>(acc, red);
 }
 
@@ -288,10 +310,23 @@ __device__ __forceinline__ double canon_max(int b, int e, F f, double *red) {
 template <int NQ>
 __device__ __forceinline__ void canon_multi(int b, int e, const double *const (&src)[NQ],
                                             const bool (&is_max)[NQ], double (&out)[NQ], double *red) {
+    constexpr int U = kCanonUnroll;  // steps whose loads are issued together (NQ x U in flight)
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int g = b + static_cast<int>(threadIdx.x); g < e; g += kInstBlock) {
+    int g = b + static_cast<int>(threadIdx.x);
+    for (; g + (U - 1) * kInstBlock < e; g += U * kInstBlock) {
+        double v[U][NQ];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v[u][q] = src[q][g + u * kInstBlock];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] = is_max[q] ? fmax(acc[q], v[u][q]) : acc[q] + v[u][q];
+    }
+    for (; g < e; g += kInstBlock) {
         double v[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) v[q] = src[q][g];
