@@ -2,6 +2,8 @@
 // reference-mode driver loop (gpu_solver.cpp:60-307) over a batch.
 #include "engine.hpp"
 
+#include <exception>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -104,6 +106,20 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int6
     }
 }
 
+void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H) {
+    H.resize(static_cast<size_t>(count));
+    std::vector<std::exception_ptr> err(static_cast<size_t>(count));
+    parallel_for_each(count, [&](int q) {
+        try {
+            prepare_structure(lps[q], H[static_cast<size_t>(q)]);
+        } catch (...) {
+            err[static_cast<size_t>(q)] = std::current_exception();
+        }
+    });
+    for (const std::exception_ptr &e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &act) {
     const size_t na = act.size();
     BatchLayout L;
@@ -179,8 +195,7 @@ int band_count(int forced, int64_t vec_len) {
     return static_cast<int>(std::min<int64_t>(kMaxBands, (bytes + kBandBytes - 1) / kBandBytes));
 }
 
-double build_bands(int bands, int rows, int64_t cols, const std::vector<int> &rptr,
-                   const std::vector<int> &ridx, const std::vector<float> &rval,
+double build_bands(int bands, int rows, int64_t cols, const int *rptr, const int *ridx, const float *rval,
                    const mpdhg_config &cfg, Layout lay, bool tune, float *vin, float *vout,
                    hipStream_t st, ColumnBands &out) {
     const size_t R = static_cast<size_t>(rows);
@@ -328,9 +343,9 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     const double t_call = now_sec();
     if (cfg.check_frequency <= 0) throw Error(MPDHG_E_ARG, "check_frequency must be positive");
     std::vector<HostLp> H(static_cast<size_t>(count));
+    prepare_all(lps, count, H);
     std::vector<int> act;
     for (int q = 0; q < count; ++q) {
-        prepare_structure(lps[q], H[static_cast<size_t>(q)]);
         mpdhg_result &r = res[q];
         r.status = MPDHG_STATUS_UNKNOWN;
         r.iterations = 0;
@@ -458,7 +473,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
     const int bands = (na == 1 && !noalloc) ? band_count(cfg.spmv_bands, L.cols) : 1;
     ColumnBands BD;
     if (bands > 1)
-        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr, hh.ridx, hh.rval, cfg, lay,
+        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr.data(), hh.ridx.data(), hh.rval.data(), cfg, lay,
                                tune_wanted(cfg, L.nnz) && !noalloc, d_x.get(), d_y.get(), st, BD);
     for (int a = 0; a < na; ++a) {
         RefInst &s = hh.inst[static_cast<size_t>(a)];

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -89,11 +90,11 @@ class DBuf {
         }
         n_ = n;
     }
-    void upload(const std::vector<T> &h, hipStream_t s) {
-        if (h.size() > n_) resize(h.size());
-        if (!h.empty())
-            MPDHG_HIP(hipMemcpyAsync(p_, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    void upload(const T *h, size_t n, hipStream_t s) {
+        if (n > n_) resize(n);
+        if (n) MPDHG_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
     }
+    void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
     void download(std::vector<T> &h, size_t n, hipStream_t s) const {
         h.resize(n);
         if (n) MPDHG_HIP(hipMemcpyAsync(h.data(), p_, n * sizeof(T), hipMemcpyDeviceToHost, s));
@@ -140,6 +141,30 @@ struct HostLp {
     std::vector<int> cptr, cidx, rptr, ridx, csc2csr;  // csc2csr[k]: CSR slot of CSC entry k
 };
 void prepare_structure(const mpdhg_lp &lp, HostLp &h);
+// prepare_structure of every LP of a batch, on the host threads; an invalid
+// LP raises the error the lowest-numbered invalid LP gives.
+void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H);
+
+// Host staging array whose elements start uninitialised (no zero fill), for
+// the batch arrays that the per-instance preparation writes in full: their
+// pages are first touched by the threads that fill them.
+template <typename T>
+class HostArray {
+  public:
+    void resize(size_t n) {
+        p_.reset(n ? new T[n] : nullptr);
+        n_ = n;
+    }
+    T *data() { return p_.get(); }
+    const T *data() const { return p_.get(); }
+    size_t size() const { return n_; }
+    T &operator[](size_t i) { return p_[i]; }
+    const T &operator[](size_t i) const { return p_[i]; }
+
+  private:
+    std::unique_ptr<T[]> p_;
+    size_t n_ = 0;
+};
 
 // Concatenation offsets of a batch: rows, columns, nonzeros, and the
 // kGroup-row groups of the rows and of the columns.
@@ -301,8 +326,7 @@ struct ColumnBands {
 // schedule is searched when `tune` (vin / vout: scratch vectors of cols and
 // rows floats) or else chosen by rule, and every band uses it.  Returns the
 // search time in ms.
-double build_bands(int bands, int rows, int64_t cols, const std::vector<int> &rptr,
-                   const std::vector<int> &ridx, const std::vector<float> &rval,
+double build_bands(int bands, int rows, int64_t cols, const int *rptr, const int *ridx, const float *rval,
                    const mpdhg_config &cfg, Layout lay, bool tune, float *vin, float *vout,
                    hipStream_t st, ColumnBands &out);
 

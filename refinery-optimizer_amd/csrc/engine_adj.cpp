@@ -15,9 +15,9 @@ void solve_adjoint(Context &ctx, const mpdhg_lp *lps, int count, mpdhg_adjoint_i
     if (!(active_tol > 0.0) || !(tol > 0.0) || max_iter < 1)
         throw Error(MPDHG_E_ARG, "adjoint: active_tol and tol must be > 0, max_iter >= 1");
     std::vector<HostLp> H(static_cast<size_t>(count));
+    prepare_all(lps, count, H);
     std::vector<int> act;
     for (int q = 0; q < count; ++q) {
-        prepare_structure(lps[q], H[static_cast<size_t>(q)]);
         const HostLp &h = H[static_cast<size_t>(q)];
         mpdhg_adjoint_io &o = io[q];
         if ((h.n > 0 && (!o.x || !o.w)) || (h.m > 0 && !o.y))

@@ -8,7 +8,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <limits>
+#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -21,10 +23,13 @@ namespace {
 // stream non-temporally: the size of MI355X's last-level (Infinity) cache.
 constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
 
+// The batch's host arrays, written in full by the per-instance preparation
+// (hence uninitialised: see HostArray), except the optional warm-start duals.
 struct EnhHost {
-    std::vector<int> cptr, cidx, rptr, ridx;
-    std::vector<float> cval, rval, cs, ls, us, rls, rus, x0, y0, powv;
-    std::vector<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
+    HostArray<int> cptr, cidx, rptr, ridx;
+    HostArray<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
+    HostArray<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
+    std::vector<float> y0;
     std::vector<EnhInst> inst;
 };
 
@@ -66,6 +71,9 @@ void scale_instance(const mpdhg_lp &lp, const HostLp &h, int ruiz, double *dr, d
 
 namespace {
 
+// f(i) for i in [0, count) on up to hardware_concurrency threads.  An
+// exception thrown by f stops the remaining work and is rethrown here, after
+// every thread has joined.
 template <typename F>
 void parallel_for(int count, F &&f) {
     const int nt = std::max(1, std::min<int>(count, static_cast<int>(std::thread::hardware_concurrency())));
@@ -75,11 +83,20 @@ void parallel_for(int count, F &&f) {
     }
     std::vector<std::thread> pool;
     std::atomic<int> next{0};
+    std::mutex mu;
+    std::exception_ptr first;
     for (int t = 0; t < nt; ++t)
         pool.emplace_back([&] {
-            for (int i = next++; i < count; i = next++) f(i);
+            try {
+                for (int i = next++; i < count; i = next++) f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mu);
+                if (!first) first = std::current_exception();
+                next = count;
+            }
         });
     for (auto &th : pool) th.join();
+    if (first) std::rethrow_exception(first);
 }
 
 }  // namespace
@@ -95,10 +112,10 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     if (cfg.reflection < 0.0f || cfg.reflection > 1.0f) throw Error(MPDHG_E_ARG, "reflection must be in [0, 1]");
     if (cfg.ruiz_iterations < 0) throw Error(MPDHG_E_ARG, "ruiz_iterations must be >= 0");
     std::vector<HostLp> H(static_cast<size_t>(count));
+    prepare_all(lps, count, H);
     std::vector<int> act;
     for (int q = 0; q < count; ++q) {
         HostLp &h = H[static_cast<size_t>(q)];
-        prepare_structure(lps[q], h);
         mpdhg_result &r = res[q];
         if (h.m == 0 || h.n == 0) {
             r.status = MPDHG_STATUS_OPTIMAL;
@@ -232,14 +249,14 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s;
     DBuf<float> d_cval_s, d_rval_s, d_cs, d_ls, d_us, d_rls, d_rus;
     DBuf<double> d_cval64_s, d_rval64_s, d_c64, d_l64, d_u64, d_dc, d_rl64, d_ru64, d_dr;
-    d_cptr.upload(hh.cptr, st); d_cidx_s.upload(hh.cidx, st); d_rptr.upload(hh.rptr, st);
-    d_ridx_s.upload(hh.ridx, st); d_cval_s.upload(hh.cval, st); d_rval_s.upload(hh.rval, st);
-    d_cval64_s.upload(hh.cval64, st); d_rval64_s.upload(hh.rval64, st);
-    d_cs.upload(hh.cs, st); d_ls.upload(hh.ls, st); d_us.upload(hh.us, st);
-    d_rls.upload(hh.rls, st); d_rus.upload(hh.rus, st);
-    d_c64.upload(hh.c64, st); d_l64.upload(hh.l64, st); d_u64.upload(hh.u64, st);
-    d_dc.upload(hh.dc, st); d_rl64.upload(hh.rl64, st); d_ru64.upload(hh.ru64, st);
-    d_dr.upload(hh.dr, st);
+    d_cptr.upload(hh.cptr.data(), hh.cptr.size(), st); d_cidx_s.upload(hh.cidx.data(), hh.cidx.size(), st); d_rptr.upload(hh.rptr.data(), hh.rptr.size(), st);
+    d_ridx_s.upload(hh.ridx.data(), hh.ridx.size(), st); d_cval_s.upload(hh.cval.data(), hh.cval.size(), st); d_rval_s.upload(hh.rval.data(), hh.rval.size(), st);
+    d_cval64_s.upload(hh.cval64.data(), hh.cval64.size(), st); d_rval64_s.upload(hh.rval64.data(), hh.rval64.size(), st);
+    d_cs.upload(hh.cs.data(), hh.cs.size(), st); d_ls.upload(hh.ls.data(), hh.ls.size(), st); d_us.upload(hh.us.data(), hh.us.size(), st);
+    d_rls.upload(hh.rls.data(), hh.rls.size(), st); d_rus.upload(hh.rus.data(), hh.rus.size(), st);
+    d_c64.upload(hh.c64.data(), hh.c64.size(), st); d_l64.upload(hh.l64.data(), hh.l64.size(), st); d_u64.upload(hh.u64.data(), hh.u64.size(), st);
+    d_dc.upload(hh.dc.data(), hh.dc.size(), st); d_rl64.upload(hh.rl64.data(), hh.rl64.size(), st); d_ru64.upload(hh.ru64.data(), hh.ru64.size(), st);
+    d_dr.upload(hh.dr.data(), hh.dr.size(), st);
     DBuf<float> d_x(C), d_x0(C), d_xp(C), d_xbar(C), d_y(R), d_y0(R), d_yp(R), d_ybar(R);
     DBuf<double> d_xbase(C), d_xchk(C), d_lam(C), d_xbest(C), d_lambest(C);
     DBuf<double> d_ybase(R), d_ychk(R), d_ax(R), d_ybest(R), d_axbest(R);
@@ -276,7 +293,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const int bands = (na == 1 && !noalloc) ? band_count(cfg.spmv_bands, L.cols) : 1;
     ColumnBands BD;
     if (bands > 1)
-        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr, hh.ridx, hh.rval, cfg, lay,
+        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr.data(), hh.ridx.data(), hh.rval.data(), cfg, lay,
                                tune_wanted(cfg, L.nnz) && !noalloc, d_x.get(), d_y.get(), st, BD);
     for (int a = 0; a < na; ++a) {
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
