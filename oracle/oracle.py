@@ -185,6 +185,18 @@ def spmv_seq_f32(rows, ptr, idx, v, x):
     return out
 
 
+def spmv_enh_f32(rows, ptr, idx, v, x):
+    """The enhanced mode's FP32 row sums (long rows in 64-nonzero segments)."""
+    out = np.zeros(rows, np.float32)
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    v = np.ascontiguousarray(v, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    lib().orc_spmv_enh_f32(rows, _p(ptr, C.c_int), _p(idx, C.c_int), _p(v, C.c_float),
+                           _p(x, C.c_float), _p(out, C.c_float))
+    return out
+
+
 def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
              reflection=1.0, ruiz_iters=10, threads=1, step=0.0, max_refinements=8,
              refine_ratio=1e-2):

@@ -70,14 +70,34 @@ static void scale_lp(enh_lp *P, int ruiz) {
     free(cs);
 }
 
-/* sequential FP32 row sums, as the tile kernels compute them */
+/* FP32 row sums as the enhanced-mode tile kernels define them: a row with at
+ * most ORC_SPLIT_ROW nonzeros is one sequential FMA chain from zero; a longer
+ * row is summed in segments of ORC_SPLIT_SEG nonzeros counted from its first
+ * nonzero, each a sequential FMA chain from zero, and the segment sums are
+ * added in order (kSplitRow / kSplitSeg of the engine). */
+#define ORC_SPLIT_ROW 8192
+#define ORC_SPLIT_SEG 64
 static void spmv32(int rows, const int *ptr, const int *idx, const float *v, const float *x,
                    float *out) {
     for (int r = 0; r < rows; ++r) {
         float s = 0.0f;
-        for (int k = ptr[r]; k < ptr[r + 1]; ++k) s = fmaf(v[k], x[idx[k]], s);
+        if (ptr[r + 1] - ptr[r] <= ORC_SPLIT_ROW) {
+            for (int k = ptr[r]; k < ptr[r + 1]; ++k) s = fmaf(v[k], x[idx[k]], s);
+        } else {
+            for (int k0 = ptr[r]; k0 < ptr[r + 1]; k0 += ORC_SPLIT_SEG) {
+                const int k1 = (k0 + ORC_SPLIT_SEG < ptr[r + 1]) ? k0 + ORC_SPLIT_SEG : ptr[r + 1];
+                float g = 0.0f;
+                for (int k = k0; k < k1; ++k) g = fmaf(v[k], x[idx[k]], g);
+                s = s + g;
+            }
+        }
         out[r] = s;
     }
+}
+
+void orc_spmv_enh_f32(int rows, const int *ptr, const int *idx, const float *vals,
+                      const float *x, float *out) {
+    spmv32(rows, ptr, idx, vals, x, out);
 }
 
 /* Power iteration on A^T A for sigma_max of the scaled matrix. */

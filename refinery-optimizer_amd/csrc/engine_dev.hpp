@@ -61,6 +61,16 @@ constexpr int64_t gather_local_limit(int tn) { return int64_t(1) << (32 - ilog2(
 constexpr int kGroup = 16;
 static_assert(kGroup <= 64 && (kGroup & (kGroup - 1)) == 0, "groups within a wave");
 
+// Enhanced mode's FP32 products sum a row (or column) with more than
+// kSplitRow nonzeros in segments of kSplitSeg nonzeros counted from the row's
+// first nonzero: each segment is a sequential FMA chain from zero, and the
+// row's sum adds the segment sums in order.  Every other row is one
+// sequential FMA chain.  kSplitRow is at least every chunk size, so a split
+// row always lies in a tile of one dense row group (more nonzeros than one
+// chunk), and the definition depends on the row alone, not on the tiling.
+constexpr int kSplitRow = 8192;
+constexpr int kSplitSeg = 64;
+
 // A tile: rows [begin, end) of one instance's compressed structure
 // (CSR rows for A x, CSC columns for A^T y).  end - begin <= the tile set's
 // block, and

@@ -290,7 +290,13 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     // ---- column bands of the A xbar product of the fused dual kernel (one
     // LP whose xbar exceeds an XCD's L2; bit-identical to the unbanded
     // product, which the no-allocation solve keeps) ----
-    const int bands = (na == 1 && !noalloc) ? band_count(cfg.spmv_bands, L.cols) : 1;
+    // (Not for an LP with split rows, kSplitRow: bands would cut their segments.)
+    bool split_rows = false;
+    if (na == 1) {
+        const std::vector<int> &rp = H[static_cast<size_t>(act[0])].rptr;
+        for (size_t i = 0; i + 1 < rp.size() && !split_rows; ++i) split_rows = rp[i + 1] - rp[i] > kSplitRow;
+    }
+    const int bands = (na == 1 && !noalloc && !split_rows) ? band_count(cfg.spmv_bands, L.cols) : 1;
     ColumnBands BD;
     if (bands > 1)
         tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr.data(), hh.ridx.data(), hh.rval.data(), cfg, lay,

@@ -49,7 +49,7 @@ constexpr int stage_bytes(int tn) { return tn * 8; }
 // its other independent loads (epilogue operands, instance state) in flight
 // before it consumes them.
 struct TileRows {
-    int k0, k1, lo, hi, base;
+    int k0, k1, lo, hi, base, rows;
 };
 
 __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restrict__ ptr) {
@@ -64,7 +64,15 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
     t.lo = tl.lo;
     t.hi = tl.hi;
     t.base = tl.base;
+    t.rows = tl.end - tl.begin;
     return t;
+}
+
+// A tile of one dense row group: at most kGroup rows holding more nonzeros
+// than one chunk (the tile planner gives such a group a tile of its own).
+template <int TN>
+__device__ __forceinline__ bool dense_group(int rows, int lo, int hi) {
+    return rows <= kGroup && hi - lo > TN;
 }
 
 // Refreshes the run hints of a tile list from the instance statuses (after
@@ -96,11 +104,105 @@ __global__ __launch_bounds__(256) void k_tile_run(Tile *__restrict__ tiles, int 
 // (products over one band of columns per launch) continues its sequential
 // FMA chain from the sum the previous band stored, so the summation order is
 // still the row's storage order.
-template <int B, int TN, bool G, typename V, typename X, typename A, bool NT = false>
+// Row sums of a dense row-group tile (dense_group) with the split rule of
+// kSplitRow: the rows are taken one at a time, each streamed in chunks that
+// start at its own first nonzero (so segments never straddle a chunk), with
+// every lane loading and gathering.  A split row's segments are summed one
+// per lane, then its owner lane adds the segment sums in order; any other
+// row is walked by its owner lane as in tile_dot_rows.  The tile's nonzeros
+// are in storage order (the gather-order layout leaves dense tiles
+// unsorted), so a pair's LDS slot is its offset in the chunk.
+template <int B, int TN, bool G, typename V, typename X, typename A, bool NT>
+__device__ A dense_rows(const TileRows &tr, const int *__restrict__ idx, const V *__restrict__ val,
+                        const X *__restrict__ xs, void *lds_raw, A init) {
+    static_assert(sizeof(Pair<V, X>) * TN == stage_bytes(TN), "one staging pass per chunk");
+    static_assert(TN / kSplitSeg <= B, "one segment per lane");
+    constexpr int kBits = ilog2(TN);
+    constexpr int PT = TN / B;
+    __shared__ int rb[kGroup + 1];
+    Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
+    A *seg = static_cast<A *>(lds_raw);  // reused once a chunk's pairs are consumed
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < tr.rows) {
+        rb[tid] = tr.k0;
+        if (tid == tr.rows - 1) rb[tr.rows] = tr.k1;
+    }
+    __syncthreads();
+    A sum = init;
+    for (int r = 0; r < tr.rows; ++r) {
+        const int k0 = rb[r], k1 = rb[r + 1];
+        const bool split = k1 - k0 > kSplitRow;
+        for (int c0 = k0; c0 < k1; c0 += TN) {
+            const int c1 = min(k1, c0 + TN), len = c1 - c0;
+            const uint32_t last = static_cast<uint32_t>(len - 1);
+            const int *__restrict__ ib = idx + c0;
+            const V *__restrict__ vb = val + c0;
+            int id[PT];
+            V v[PT];
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
+                if constexpr (NT) {
+                    id[q] = __builtin_nontemporal_load(ib + o);
+                    v[q] = __builtin_nontemporal_load(vb + o);
+                } else {
+                    id[q] = ib[o];
+                    v[q] = vb[o];
+                }
+            }
+            X g[PT];
+            if (G) {
+                const X *__restrict__ xb = xs + tr.base;
+#pragma unroll
+                for (int q = 0; q < PT; ++q) g[q] = xb[static_cast<uint32_t>(id[q]) >> kBits];
+            } else {
+#pragma unroll
+                for (int q = 0; q < PT; ++q) g[q] = xs[static_cast<uint32_t>(id[q])];
+            }
+#pragma unroll
+            for (int q = 0; q < PT; ++q)
+                lds[min(static_cast<uint32_t>(tid + q * B), last)] = Pair<V, X>{v[q], g[q]};
+            __syncthreads();
+            if (split) {
+                const int nseg = (len + kSplitSeg - 1) / kSplitSeg;
+                A s = A(0);
+                if (tid < nseg) {
+                    const int a = tid * kSplitSeg, b = min(len, a + kSplitSeg);
+                    for (int k = a; k < b; ++k) s = fma_acc<A, V>(lds[k].v, lds[k].x, s);
+                }
+                __syncthreads();
+                if (tid < nseg) seg[tid] = s;
+                __syncthreads();
+                if (tid == r)
+                    for (int t = 0; t < nseg; ++t) sum = sum + seg[t];
+            } else if (tid == r) {
+                int k = 0;
+                for (; k + 4 <= len; k += 4) {
+                    const Pair<V, X> p0 = lds[k], p1 = lds[k + 1], p2 = lds[k + 2], p3 = lds[k + 3];
+                    sum = fma_acc<A, V>(p0.v, p0.x, sum);
+                    sum = fma_acc<A, V>(p1.v, p1.x, sum);
+                    sum = fma_acc<A, V>(p2.v, p2.x, sum);
+                    sum = fma_acc<A, V>(p3.v, p3.x, sum);
+                }
+                for (; k < len; ++k) sum = fma_acc<A, V>(lds[k].v, lds[k].x, sum);
+            }
+            __syncthreads();
+        }
+    }
+    return sum;
+}
+
+// SPLIT: apply the split rule of kSplitRow (enhanced mode's FP32 products);
+// dense row-group tiles then take dense_rows.  Without it every row is one
+// sequential chain (reference mode, FP64 checks).
+template <int B, int TN, bool G, typename V, typename X, typename A, bool NT = false, bool SPLIT = false>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
                                            void *lds_raw, A init = A(0)) {
     static_assert(valid_schedule(B, TN), "tile schedule");
+    if constexpr (SPLIT) {
+        if (dense_group<TN>(tr.rows, tr.lo, tr.hi)) return dense_rows<B, TN, G, V, X, A, NT>(tr, idx, val, xs, lds_raw, init);
+    }
     constexpr int kBits = ilog2(TN);
     constexpr uint32_t kMask = static_cast<uint32_t>(TN) - 1u;
     constexpr int S = stage_bytes(TN) / static_cast<int>(sizeof(Pair<V, X>));  // pairs per pass

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(B) void k_pow_rows(const Tile *__restrict__ tiles, 
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<B, TN, G, float>(tl, rptr, ridx, rval, v, lds);
+    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, rptr), ridx, rval, v, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) w[i] = s;
 }
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(B) void k_pow_cols(const Tile *__restrict__ tiles, 
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot<B, TN, G, float>(tl, cptr, cidx, cval, w, lds);
+    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, cptr), cidx, cval, w, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q = 0.0;
     if (j < tl.end) {
@@ -127,7 +127,7 @@ __device__ __forceinline__ void enh_primal_body(const Tile *__restrict__ tiles, 
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float g = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.cidx, A.cval, restart ? A.yp : A.y,
+    const float g = tile_dot_rows<B, TN, G, float, float, float, NT, true>(tr, A.cidx, A.cval, restart ? A.yp : A.y,
                                                                      lds);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<B, TN, G, float, float, float, NT>(tr, A.ridx, A.rval, A.xbar, lds, h0);
+    const float h = tile_dot_rows<B, TN, G, float, float, float, NT, true>(tr, A.ridx, A.rval, A.xbar, lds, h0);
     if (BAND && !bp.last) {
         if (mine) bp.hpart[i] = h;
         return;

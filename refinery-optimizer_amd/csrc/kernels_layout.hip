@@ -31,6 +31,19 @@ __global__ __launch_bounds__(kSortBlock) void k_gather_order(const Tile *__restr
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     const int tid = static_cast<int>(threadIdx.x);
+    if (dense_group<TN>(tl.end - tl.begin, tl.lo, tl.hi)) {
+        // A dense row-group tile stays in storage order (the split rule's
+        // kernels stream its rows from their own first nonzero): the packed
+        // words keep each entry's own position, as an identity sort would.
+        for (int k = tl.lo + tid; k < tl.hi; k += kSortBlock) {
+            const int p = (k - tl.lo) & static_cast<int>(kMask);
+            dst_idx[k] = static_cast<int>((static_cast<uint32_t>(src_idx[k] - tl.base) << kBits) |
+                                          static_cast<uint32_t>(p));
+            if (d32) d32[k] = s32[k];
+            if (d64) d64[k] = s64[k];
+        }
+        return;
+    }
     for (int c0 = tl.lo; c0 < tl.hi; c0 += TN) {
         const int n = min(tl.hi - c0, TN);
         int np2 = 1;

@@ -12,6 +12,7 @@ import math
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import oracle as O
 import sih_solver as S
@@ -282,3 +283,57 @@ def test_column_bands_by_rule_on_a_large_lp(need_gpu):
     assert a.simplex_iterations == b.simplex_iterations == 192
     assert np.array_equal(a.x, b.x) and np.array_equal(a.row_duals, b.row_duals)
     assert a.rel_kkt == b.rel_kkt
+
+
+def _long_row_lp(seed=81):
+    """An LP with split rows (kSplitRow = 8192): a random equality LP of
+    m = 12,000, n = 30,000 with 3 nonzeros per column, plus rows 0, 1 and 2
+    over 20,000, 9,000 and 8,193 distinct columns and column 0 over 12,000
+    distinct rows (values U[-1, 1]), b = A x0 for x0 ~ U[0, 10], 0 <= x <= 10,
+    c = A^T y0 + s with s ~ U[0, 1]."""
+    rng = np.random.default_rng(seed)
+    m, n = 12000, 30000
+    rows = [rng.choice(m, 3, replace=False) for _ in range(n)]
+    for r, cnt in ((0, 20000), (1, 9000), (2, 8193)):
+        for j in rng.choice(n, cnt, replace=False):
+            rows[j] = np.union1d(rows[j], [r])
+    rows[0] = np.union1d(rows[0], rng.choice(m, 12000, replace=False))
+    ptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    ri = np.concatenate(rows).astype(np.int64)
+    vals = rng.uniform(-1, 1, len(ri))
+    p = S.Problem("long_rows")
+    p.resize(m, n)
+    p.A = S.SparseMatrix.from_csc(m, n, ptr, ri, vals)
+    x0 = rng.uniform(0, 10, n)
+    y0 = rng.normal(size=m)
+    A = sp.csc_matrix((vals, ri, ptr), shape=(m, n))
+    b = A @ x0
+    p.c = A.T @ y0 + rng.uniform(0, 1, n)
+    p.row_lower, p.row_upper = b.copy(), b.copy()
+    p.col_lower, p.col_upper = np.zeros(n), np.full(n, 10.0)
+    return p
+
+
+def test_split_rows_first_cycle_bitexact_vs_cpu_port(need_gpu):
+    # Rows and columns of more than 8192 nonzeros are summed in 64-nonzero
+    # segments by the enhanced kernels and by the CPU port alike.
+    p = _long_row_lp()
+    sol = S.solve_gpu_pdhg(p, enh(tol=0.0, max_iterations=64, max_refinements=0))
+    ref = port(p, sol, tol=0.0, max_iterations=64, check_frequency=64, max_refinements=0)
+    assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sol.row_duals, ref["y"])
+    assert sol.csr_bands == 1  # no column bands for an LP with split rows
+
+
+@pytest.mark.parametrize("layout,block,chunk", [(0, 128, 1024), (0, 256, 2048), (1, 256, 4096), (0, 512, 8192)])
+def test_split_rows_every_schedule_and_batch_bit_identical(need_gpu, layout, block, chunk):
+    p = _long_row_lp()
+    other = S.gen_random(2000, 4800, 8, 82)
+    a = S.solve_batch([p, other], enh(tol=1e-6))
+    cfg = enh(tol=1e-6)
+    cfg.spmv_layout, cfg.spmv_block, cfg.spmv_chunk = layout, block, chunk
+    b = S.solve_gpu_pdhg(p, cfg)
+    assert b.status == S.SolutionStatus.Optimal
+    assert b.simplex_iterations == a[0].simplex_iterations and b.restarts == a[0].restarts
+    assert np.array_equal(a[0].x, b.x) and np.array_equal(a[0].row_duals, b.row_duals)
+    assert a[0].rel_kkt == b.rel_kkt
+    assert O.kkt(O.LpArrays.of(p), b.x, b.row_duals)["kkt"] <= 1e-6 * 1.0001

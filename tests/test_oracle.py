@@ -293,3 +293,41 @@ def test_identify_basis_ties_keep_index_order():
     cb, rb = O.identify_basis(O.LpArrays.of(p), [1.0, 1.0, 1.0])
     # row activity 3 with infinite bounds -> distance 1e30 -> basic; all columns tie at 1
     assert rb.tolist() == [0] and cb.tolist() == [1, 1, 1]
+
+
+def test_enhanced_row_sums_are_sequential_up_to_8192_nonzeros():
+    # Rows of at most 8192 nonzeros: one FMA chain, the same bits as the
+    # sequential restatement of the reference's row sums.
+    rng = np.random.default_rng(5)
+    lens = np.array([0, 1, 63, 64, 65, 1000, 8191, 8192])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = rng.integers(0, 500, ptr[-1]).astype(np.int32)
+    v = rng.normal(size=ptr[-1]).astype(np.float32)
+    x = rng.normal(size=500).astype(np.float32)
+    a = O.spmv_enh_f32(len(lens), ptr, idx, v, x)
+    b = O.spmv_seq_f32(len(lens), ptr, idx, v, x)
+    assert np.array_equal(a, b)
+
+
+def test_enhanced_long_rows_sum_in_64_nonzero_segments():
+    # One row of 12,800 nonzeros: 2^24 then 12,799 ones (x = 1).  One FMA
+    # chain loses every +1 (2^24 + 1 rounds to 2^24, ties to even): 2^24.
+    # Segments of 64 counted from the row's first nonzero: the first segment
+    # is 2^24 (its 63 ones lost), each of the other 199 is exactly 64, and
+    # 2^24 + 64 k is exact: 2^24 + 199 * 64 = 16,789,952.  A row of 8,193
+    # ones in segments is 8,193 exactly either way.
+    n1 = 12800
+    v1 = np.ones(n1, np.float32)
+    v1[0] = 2.0 ** 24
+    v = np.concatenate([v1, np.ones(8193, np.float32)])
+    ptr = np.array([0, n1, n1 + 8193], np.int32)
+    idx = np.zeros(len(v), np.int32)
+    x = np.ones(1, np.float32)
+    out = O.spmv_enh_f32(2, ptr, idx, v, x)
+    assert out[0] == np.float32(16789952.0)
+    assert O.spmv_seq_f32(2, ptr, idx, v, x)[0] == np.float32(2.0 ** 24)
+    assert out[1] == np.float32(8193.0)
+    # a dropped, repeated or shifted segment moves the sum of distinct values
+    w = (np.arange(len(v1)) % 7 + 1).astype(np.float32)
+    s = O.spmv_enh_f32(1, np.array([0, n1], np.int32), np.zeros(n1, np.int32), w, x)[0]
+    assert s == np.float32(w.astype(np.int64).sum())
