@@ -186,7 +186,7 @@ def spmv_seq_f32(rows, ptr, idx, v, x):
 
 
 def spmv_enh_f32(rows, ptr, idx, v, x):
-    """The enhanced mode's FP32 row sums (long rows in 64-nonzero segments)."""
+    """The enhanced mode's FP32 row sums (long rows in segments and blocks)."""
     out = np.zeros(rows, np.float32)
     ptr = np.ascontiguousarray(ptr, np.int32)
     idx = np.ascontiguousarray(idx, np.int32)

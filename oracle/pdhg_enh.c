@@ -72,23 +72,32 @@ static void scale_lp(enh_lp *P, int ruiz) {
 
 /* FP32 row sums as the enhanced-mode tile kernels define them: a row with at
  * most ORC_SPLIT_ROW nonzeros is one sequential FMA chain from zero; a longer
- * row is summed in segments of ORC_SPLIT_SEG nonzeros counted from its first
- * nonzero, each a sequential FMA chain from zero, and the segment sums are
- * added in order (kSplitRow / kSplitSeg of the engine). */
+ * row is summed in two levels counted from its first nonzero: segments of
+ * ORC_SPLIT_SEG nonzeros, each a sequential FMA chain from zero; blocks of
+ * ORC_SPLIT_BLOCK nonzeros, each the in-order sum of its segment sums; and
+ * the row sum, the in-order sum of the block sums (kSplitRow, kSplitSeg and
+ * kSplitBlockNnz of the engine). */
 #define ORC_SPLIT_ROW 8192
 #define ORC_SPLIT_SEG 64
+#define ORC_SPLIT_BLOCK 4096
 static void spmv32(int rows, const int *ptr, const int *idx, const float *v, const float *x,
                    float *out) {
     for (int r = 0; r < rows; ++r) {
+        const int k0 = ptr[r], k1 = ptr[r + 1];
         float s = 0.0f;
-        if (ptr[r + 1] - ptr[r] <= ORC_SPLIT_ROW) {
-            for (int k = ptr[r]; k < ptr[r + 1]; ++k) s = fmaf(v[k], x[idx[k]], s);
+        if (k1 - k0 <= ORC_SPLIT_ROW) {
+            for (int k = k0; k < k1; ++k) s = fmaf(v[k], x[idx[k]], s);
         } else {
-            for (int k0 = ptr[r]; k0 < ptr[r + 1]; k0 += ORC_SPLIT_SEG) {
-                const int k1 = (k0 + ORC_SPLIT_SEG < ptr[r + 1]) ? k0 + ORC_SPLIT_SEG : ptr[r + 1];
-                float g = 0.0f;
-                for (int k = k0; k < k1; ++k) g = fmaf(v[k], x[idx[k]], g);
-                s = s + g;
+            for (int b0 = k0; b0 < k1; b0 += ORC_SPLIT_BLOCK) {
+                const int b1 = (b0 + ORC_SPLIT_BLOCK < k1) ? b0 + ORC_SPLIT_BLOCK : k1;
+                float blk = 0.0f;
+                for (int g0 = b0; g0 < b1; g0 += ORC_SPLIT_SEG) {
+                    const int g1 = (g0 + ORC_SPLIT_SEG < b1) ? g0 + ORC_SPLIT_SEG : b1;
+                    float g = 0.0f;
+                    for (int k = g0; k < g1; ++k) g = fmaf(v[k], x[idx[k]], g);
+                    blk = blk + g;
+                }
+                s = s + blk;
             }
         }
         out[r] = s;

@@ -65,7 +65,8 @@ void orc_spmv_csc_f64(int n, const int *col_ptr, const int *row_ind, const doubl
 void orc_spmv_seq_f32(int rows, const int *ptr, const int *idx, const float *vals,
                       const float *x, float *out);
 /* The enhanced mode's FP32 row sums (pdhg_enh.c): sequential chains, with
- * rows of more than 8192 nonzeros summed in 64-nonzero segments. */
+ * rows of more than 8192 nonzeros summed in 64-nonzero segments within
+ * 4096-nonzero blocks. */
 void orc_spmv_enh_f32(int rows, const int *ptr, const int *idx, const float *vals,
                       const float *x, float *out);
 

@@ -61,15 +61,26 @@ constexpr int64_t gather_local_limit(int tn) { return int64_t(1) << (32 - ilog2(
 constexpr int kGroup = 16;
 static_assert(kGroup <= 64 && (kGroup & (kGroup - 1)) == 0, "groups within a wave");
 
-// Enhanced mode's FP32 products sum a row (or column) with more than
-// kSplitRow nonzeros in segments of kSplitSeg nonzeros counted from the row's
-// first nonzero: each segment is a sequential FMA chain from zero, and the
-// row's sum adds the segment sums in order.  Every other row is one
-// sequential FMA chain.  kSplitRow is at least every chunk size, so a split
-// row always lies in a tile of one dense row group (more nonzeros than one
-// chunk), and the definition depends on the row alone, not on the tiling.
+// Enhanced mode's products (FP32 iterations, FP64 checks and adjoint) sum a
+// row (or column) with more than kSplitRow nonzeros in two levels, counted
+// from the row's first nonzero: segments of kSplitSeg nonzeros, each a
+// sequential FMA chain from zero; blocks of kSplitBlockNnz nonzeros, each the
+// in-order sum of its segments' sums; and the row sum, the in-order sum of
+// its blocks' sums.  Every other row is one sequential FMA chain.  kSplitRow
+// is at least every chunk size, so a split row always lies in a tile of one
+// dense row group (more nonzeros than one chunk), and the definition depends
+// on the row alone, not on the tiling.
 constexpr int kSplitRow = 8192;
 constexpr int kSplitSeg = 64;
+constexpr int kSplitBlockNnz = 4096;
+
+// One block of a split row for the pre-pass: nonzeros [c0, c1) of the
+// direction's storage-order stream (c0 - the row's first nonzero is a
+// multiple of kSplitBlockNnz), whose block sum goes to blk[slot].
+struct SplitChunk {
+    int32_t c0, c1, slot, inst;
+};
+constexpr int kSplitBlock = 256;  // threads per pre-pass workgroup
 
 // A tile: rows [begin, end) of one instance's compressed structure
 // (CSR rows for A x, CSC columns for A^T y).  end - begin <= the tile set's

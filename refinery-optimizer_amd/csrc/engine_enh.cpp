@@ -344,15 +344,65 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     MPDHG_HIP(hipStreamSynchronize(st));
     const double t_loop = now_sec();
 
+    // ---- split rows (more than kSplitRow nonzeros, either direction): a
+    // pre-pass spreads their blocks over many workgroups and leaves the block
+    // sums for the fused kernels' owner lanes to add in order (the
+    // no-allocation solve forms them inside their tiles instead: same bits) ----
+    struct SplitDir {
+        std::vector<SplitChunk> host;
+        std::vector<int> rowseg;
+        DBuf<SplitChunk> d_items;
+        DBuf<int> d_rowseg;
+        DBuf<float> d_blk;  // block sums
+        int n = 0;
+    } SR, SC;
+    auto plan_split = [&](bool csr, SplitDir &D) {
+        int64_t blocks = 0;
+        for (int a = 0; a < na; ++a) {
+            const HostLp &h = H[static_cast<size_t>(act[static_cast<size_t>(a)])];
+            const std::vector<int> &ptr = csr ? h.rptr : h.cptr;
+            const int cnt = csr ? h.m : h.n;
+            const int64_t roff = csr ? L.row_off[static_cast<size_t>(a)] : L.col_off[static_cast<size_t>(a)];
+            const int64_t zoff = L.nnz_off[static_cast<size_t>(a)];
+            for (int i = 0; i < cnt; ++i) {
+                const int k0 = ptr[static_cast<size_t>(i)], len = ptr[static_cast<size_t>(i) + 1] - k0;
+                if (len <= kSplitRow) continue;
+                if (D.rowseg.empty()) D.rowseg.assign(csr ? R : C, -1);
+                D.rowseg[static_cast<size_t>(roff + i)] = static_cast<int>(blocks);
+                for (int c0 = 0; c0 < len; c0 += kSplitBlockNnz)
+                    D.host.push_back(SplitChunk{static_cast<int32_t>(zoff + k0 + c0),
+                                                static_cast<int32_t>(zoff + k0 + std::min(len, c0 + kSplitBlockNnz)),
+                                                static_cast<int32_t>(blocks + c0 / kSplitBlockNnz), a});
+                blocks += (len + kSplitBlockNnz - 1) / kSplitBlockNnz;
+            }
+        }
+        D.n = static_cast<int>(D.host.size());
+        if (!D.n) return;
+        D.d_items.upload(D.host, st);
+        D.d_rowseg.upload(D.rowseg, st);
+        D.d_blk.resize(static_cast<size_t>(blocks));
+    };
+    if (!noalloc) {
+        plan_split(true, SR);
+        plan_split(false, SC);
+    }
     // ---- step size: power iteration on (D_r A D_c)^T (D_r A D_c), unless
     // every LP brings its step size with its warm start ----
     bool all_steps = true;
     for (int a = 0; a < na; ++a) all_steps = all_steps && hh.inst[static_cast<size_t>(a)].eta > 0.0;
     for (int p = 0; p < (all_steps ? 0 : 64); ++p) {
         launch::pow_scale(st, cts, d_inst.get(), d_xbar.get(), d_xp.get());
-        launch::pow_rows(st, rts, d_rptr.get(), RD.idx, RD.v32, d_xp.get(), d_yp.get());
+        if (SR.n)
+            launch::split_segments(st, SR.d_items.get(), SR.n, d_ridx_s.get(), d_rval_s.get(), d_xp.get(), nullptr,
+                                   nullptr, nullptr, 0, SR.d_blk.get());
+        launch::pow_rows(st, rts, d_rptr.get(), RD.idx, RD.v32, d_xp.get(), d_yp.get(), SR.n ? SR.d_blk.get() : nullptr,
+                         SR.n ? SR.d_rowseg.get() : nullptr);
+        if (SC.n)
+            launch::split_segments(st, SC.d_items.get(), SC.n, d_cidx_s.get(), d_cval_s.get(), d_yp.get(), nullptr,
+                                   nullptr, nullptr, 0, SC.d_blk.get());
         launch::pow_cols(st, cts, d_cptr.get(), CD.idx, CD.v32, d_yp.get(), d_xbar.get(),
-                         d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
+                         d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups, SC.n ? SC.d_blk.get() : nullptr,
+                         SC.n ? SC.d_rowseg.get() : nullptr);
         launch::pow_fin(st, d_inst.get(), na, d_gc.get() + static_cast<size_t>(kGcPow) * L.cgroups);
     }
     MPDHG_HIP(hipGetLastError());
@@ -385,6 +435,25 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     launch::EnhDual D{d_rptr.get(), RD.idx, RD.v32, d_xbar.get(), d_rls.get(), d_rus.get(),
                       d_ybar.get(), d_y.get(), d_y0.get(), d_yp.get(), d_ybase.get(), d_dr.get(),
                       d_ychk.get(), d_gr.get(), grn, d_itb.get(), rho};
+    if (SC.n) {
+        P.seg = SC.d_blk.get();
+        P.rowseg = SC.d_rowseg.get();
+    }
+    if (SR.n) {
+        D.seg = SR.d_blk.get();
+        D.rowseg = SR.d_rowseg.get();
+    }
+    // the split rows' block pre-passes of iteration offset i of a cycle
+    auto pre_primal = [&](int i) {
+        if (SC.n)
+            launch::split_segments(st, SC.d_items.get(), SC.n, d_cidx_s.get(), d_cval_s.get(), d_y.get(),
+                                   d_yp.get(), d_inst.get(), d_itb.get(), i, SC.d_blk.get());
+    };
+    auto pre_dual = [&](int i) {
+        if (SR.n)
+            launch::split_segments(st, SR.d_items.get(), SR.n, d_ridx_s.get(), d_rval_s.get(), d_xbar.get(),
+                                   nullptr, d_inst.get(), d_itb.get(), i, SR.d_blk.get());
+    };
     launch::EnhCheck CK{d_rptr.get(), RD.idx, d_cptr.get(), CD.idx, RD.v64,
                         CD.v64, d_xchk.get(), d_ychk.get(), d_ax.get(), d_lam.get(),
                         d_rl64.get(), d_ru64.get(), d_c64.get(), d_l64.get(), d_u64.get(), d_dc.get(),
@@ -443,7 +512,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     auto issue_cycle = [&](int len) {
         for (int i = 0; i < len; ++i) {
             const int check = (i + 1 == len);
+            pre_primal(i);
             launch::enh_primal(st, cts, d_inst.get(), P, i, check);
+            pre_dual(i);
             dual(i, check, nullptr, nullptr);
         }
         check_step(len);
@@ -481,7 +552,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     const int check = (i + 1 == len);
                     const bool samp = used < nsamp && !check;
                     hipEvent_t *e = samp ? ctx.events.data() + 4 * used : nullptr;
+                    pre_primal(i);
                     launch::enh_primal(st, cts, d_inst.get(), P, i, check, e ? e[0] : nullptr, e ? e[1] : nullptr);
+                    pre_dual(i);
                     dual(i, check, e ? e[2] : nullptr, e ? e[3] : nullptr);
                     if (samp) samp_it[static_cast<size_t>(used++)] = it + i;
                 }

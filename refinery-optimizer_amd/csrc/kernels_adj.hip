@@ -34,7 +34,7 @@ __global__ __launch_bounds__(B) void k_adj_cols(const Tile *__restrict__ tiles, 
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (st[tl.inst].done) return;
-    const double s = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_rows<B, TN, G, double, double, double, false, true>(tile_rows(tl, A.cptr), A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q2 = 0.0;
     if (j < tl.end) {
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(B) void k_adj_rows(const Tile *__restrict__ tiles, 
     const Tile tl = tiles[t];
     const AdjInst &s = st[tl.inst];
     if (s.done) return;
-    const double v = tile_dot_t<B, TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.rt, lds);
+    const double v = tile_dot_rows<B, TN, G, double, double, double, false, true>(tile_rows(tl, A.rptr), A.ridx, A.rval, A.rt, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double s2 = 0.0;
     if (i < tl.end) {
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(B) void k_adj_atmu(const Tile *__restrict__ tiles, 
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const double s = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.pt, lds);
+    const double s = tile_dot_rows<B, TN, G, double, double, double, false, true>(tile_rows(tl, A.cptr), A.cidx, A.cval, A.pt, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     if (j < tl.end) A.q[j] = s;
 }

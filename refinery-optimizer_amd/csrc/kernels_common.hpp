@@ -49,7 +49,7 @@ constexpr int stage_bytes(int tn) { return tn * 8; }
 // its other independent loads (epilogue operands, instance state) in flight
 // before it consumes them.
 struct TileRows {
-    int k0, k1, lo, hi, base, rows;
+    int k0, k1, lo, hi, base, rows, first;
 };
 
 __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restrict__ ptr) {
@@ -65,6 +65,7 @@ __device__ __forceinline__ TileRows tile_rows(const Tile &tl, const int *__restr
     t.hi = tl.hi;
     t.base = tl.base;
     t.rows = tl.end - tl.begin;
+    t.first = tl.begin;
     return t;
 }
 
@@ -105,20 +106,28 @@ __global__ __launch_bounds__(256) void k_tile_run(Tile *__restrict__ tiles, int 
 // FMA chain from the sum the previous band stored, so the summation order is
 // still the row's storage order.
 // Row sums of a dense row-group tile (dense_group) with the split rule of
-// kSplitRow: the rows are taken one at a time, each streamed in chunks that
-// start at its own first nonzero (so segments never straddle a chunk), with
-// every lane loading and gathering.  A split row's segments are summed one
-// per lane, then its owner lane adds the segment sums in order; any other
-// row is walked by its owner lane as in tile_dot_rows.  The tile's nonzeros
-// are in storage order (the gather-order layout leaves dense tiles
-// unsorted), so a pair's LDS slot is its offset in the chunk.
+// kSplitRow.  The tile's nonzeros are in storage order (the gather-order
+// layout leaves dense tiles unsorted), so a pair's LDS slot is its offset in
+// the chunk; chunks hold CS pairs, one staging pass.
+// - A split row is streamed on its own, in chunks that start at its first
+//   nonzero (CS divides kSplitBlockNnz or is a multiple of it, so neither a
+//   segment nor a block straddles two chunks): its segments are summed one
+//   per lane, and its owner lane folds the segment sums into block sums and
+//   the block sums into the row sum, in order.  With `pre` (and rowseg) the
+//   block sums come from the pre-pass (k_split_seg, the same blocks) and the
+//   owner lane only adds them in order.
+// - A run of consecutive other rows is streamed together, each lane walking
+//   its own row as in tile_dot_rows.
 template <int B, int TN, bool G, typename V, typename X, typename A, bool NT>
 __device__ A dense_rows(const TileRows &tr, const int *__restrict__ idx, const V *__restrict__ val,
-                        const X *__restrict__ xs, void *lds_raw, A init) {
-    static_assert(sizeof(Pair<V, X>) * TN == stage_bytes(TN), "one staging pass per chunk");
-    static_assert(TN / kSplitSeg <= B, "one segment per lane");
+                        const X *__restrict__ xs, void *lds_raw, A init, const A *__restrict__ pre,
+                        const int *__restrict__ rowseg) {
+    constexpr int CS = stage_bytes(TN) / static_cast<int>(sizeof(Pair<V, X>));
+    static_assert(CS % kSplitSeg == 0 && (kSplitBlockNnz % CS == 0 || CS % kSplitBlockNnz == 0), "aligned chunks");
+    static_assert(CS / kSplitSeg <= B && CS % B == 0, "one segment per lane");
     constexpr int kBits = ilog2(TN);
-    constexpr int PT = TN / B;
+    constexpr int PT = CS / B;
+    constexpr int kSegsPerBlock = kSplitBlockNnz / kSplitSeg;
     __shared__ int rb[kGroup + 1];
     Pair<V, X> *lds = static_cast<Pair<V, X> *>(lds_raw);
     A *seg = static_cast<A *>(lds_raw);  // reused once a chunk's pairs are consumed
@@ -128,66 +137,109 @@ __device__ A dense_rows(const TileRows &tr, const int *__restrict__ idx, const V
         if (tid == tr.rows - 1) rb[tr.rows] = tr.k1;
     }
     __syncthreads();
-    A sum = init;
-    for (int r = 0; r < tr.rows; ++r) {
-        const int k0 = rb[r], k1 = rb[r + 1];
-        const bool split = k1 - k0 > kSplitRow;
-        for (int c0 = k0; c0 < k1; c0 += TN) {
-            const int c1 = min(k1, c0 + TN), len = c1 - c0;
-            const uint32_t last = static_cast<uint32_t>(len - 1);
-            const int *__restrict__ ib = idx + c0;
-            const V *__restrict__ vb = val + c0;
-            int id[PT];
-            V v[PT];
+    // stages chunk [c0, c1) of the stream as (value, x) pairs at offsets k - c0
+    auto stage = [&](int c0, int c1) {
+        const uint32_t last = static_cast<uint32_t>(c1 - 1 - c0);
+        const int *__restrict__ ib = idx + c0;
+        const V *__restrict__ vb = val + c0;
+        int id[PT];
+        V v[PT];
 #pragma unroll
-            for (int q = 0; q < PT; ++q) {
-                const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
-                if constexpr (NT) {
-                    id[q] = __builtin_nontemporal_load(ib + o);
-                    v[q] = __builtin_nontemporal_load(vb + o);
-                } else {
-                    id[q] = ib[o];
-                    v[q] = vb[o];
-                }
-            }
-            X g[PT];
-            if (G) {
-                const X *__restrict__ xb = xs + tr.base;
-#pragma unroll
-                for (int q = 0; q < PT; ++q) g[q] = xb[static_cast<uint32_t>(id[q]) >> kBits];
+        for (int q = 0; q < PT; ++q) {
+            const uint32_t o = min(static_cast<uint32_t>(tid + q * B), last);
+            if constexpr (NT) {
+                id[q] = __builtin_nontemporal_load(ib + o);
+                v[q] = __builtin_nontemporal_load(vb + o);
             } else {
-#pragma unroll
-                for (int q = 0; q < PT; ++q) g[q] = xs[static_cast<uint32_t>(id[q])];
+                id[q] = ib[o];
+                v[q] = vb[o];
             }
+        }
+        X g[PT];
+        if (G) {
+            const X *__restrict__ xb = xs + tr.base;
 #pragma unroll
-            for (int q = 0; q < PT; ++q)
-                lds[min(static_cast<uint32_t>(tid + q * B), last)] = Pair<V, X>{v[q], g[q]};
-            __syncthreads();
-            if (split) {
-                const int nseg = (len + kSplitSeg - 1) / kSplitSeg;
-                A s = A(0);
-                if (tid < nseg) {
-                    const int a = tid * kSplitSeg, b = min(len, a + kSplitSeg);
-                    for (int k = a; k < b; ++k) s = fma_acc<A, V>(lds[k].v, lds[k].x, s);
+            for (int q = 0; q < PT; ++q) g[q] = xb[static_cast<uint32_t>(id[q]) >> kBits];
+        } else {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) g[q] = xs[static_cast<uint32_t>(id[q])];
+        }
+#pragma unroll
+        for (int q = 0; q < PT; ++q) lds[min(static_cast<uint32_t>(tid + q * B), last)] = Pair<V, X>{v[q], g[q]};
+        __syncthreads();
+    };
+    A sum = init;
+    for (int r = 0; r < tr.rows;) {
+        const int k0 = rb[r], k1 = rb[r + 1];
+        if (k1 - k0 > kSplitRow) {
+            if (pre) {
+                // all lanes stage the block sums in LDS with coalesced loads; the
+                // owner lane adds them in order from there
+                const A *__restrict__ sp = pre + rowseg[tr.first + r];
+                const int nblk = (k1 - k0 + kSplitBlockNnz - 1) / kSplitBlockNnz;
+                constexpr int cap = stage_bytes(TN) / static_cast<int>(sizeof(A));
+                for (int t0 = 0; t0 < nblk; t0 += cap) {
+                    const int cnt = min(cap, nblk - t0);
+                    for (int t = tid; t < cnt; t += B) seg[t] = sp[t0 + t];
+                    __syncthreads();
+                    if (tid == r)
+                        for (int t = 0; t < cnt; ++t) sum = sum + seg[t];
+                    __syncthreads();
                 }
-                __syncthreads();
-                if (tid < nseg) seg[tid] = s;
-                __syncthreads();
-                if (tid == r)
-                    for (int t = 0; t < nseg; ++t) sum = sum + seg[t];
-            } else if (tid == r) {
-                int k = 0;
-                for (; k + 4 <= len; k += 4) {
+            } else {
+                A blk = A(0);
+                int done = 0;  // segments of the row summed so far
+                for (int c0 = k0; c0 < k1; c0 += CS) {
+                    const int c1 = min(k1, c0 + CS), len = c1 - c0;
+                    stage(c0, c1);
+                    const int nseg = (len + kSplitSeg - 1) / kSplitSeg;
+                    A sg = A(0);
+                    if (tid < nseg) {
+                        const int a = tid * kSplitSeg, e = min(len, a + kSplitSeg);
+                        for (int k = a; k < e; ++k) sg = fma_acc<A, V>(lds[k].v, lds[k].x, sg);
+                    }
+                    __syncthreads();
+                    if (tid < nseg) seg[tid] = sg;
+                    __syncthreads();
+                    if (tid == r) {
+                        for (int t = 0; t < nseg; ++t) {
+                            blk = blk + seg[t];
+                            if (++done % kSegsPerBlock == 0) {
+                                sum = sum + blk;
+                                blk = A(0);
+                            }
+                        }
+                        if (c1 == k1 && done % kSegsPerBlock != 0) sum = sum + blk;
+                    }
+                    __syncthreads();
+                }
+            }
+            ++r;
+            continue;
+        }
+        // a run of consecutive rows of at most kSplitRow nonzeros
+        int r2 = r + 1;
+        while (r2 < tr.rows && rb[r2 + 1] - rb[r2] <= kSplitRow) ++r2;
+        const int lo = rb[r], hi = rb[r2];
+        const bool mine = tid >= r && tid < r2;
+        for (int c0 = lo; c0 < hi; c0 += CS) {
+            const int c1 = min(hi, c0 + CS);
+            stage(c0, c1);
+            if (mine) {
+                const int a = max(tr.k0, c0) - c0, e = min(tr.k1, c1) - c0;
+                int k = a;
+                for (; k + 4 <= e; k += 4) {
                     const Pair<V, X> p0 = lds[k], p1 = lds[k + 1], p2 = lds[k + 2], p3 = lds[k + 3];
                     sum = fma_acc<A, V>(p0.v, p0.x, sum);
                     sum = fma_acc<A, V>(p1.v, p1.x, sum);
                     sum = fma_acc<A, V>(p2.v, p2.x, sum);
                     sum = fma_acc<A, V>(p3.v, p3.x, sum);
                 }
-                for (; k < len; ++k) sum = fma_acc<A, V>(lds[k].v, lds[k].x, sum);
+                for (; k < e; ++k) sum = fma_acc<A, V>(lds[k].v, lds[k].x, sum);
             }
             __syncthreads();
         }
+        r = r2;
     }
     return sum;
 }
@@ -198,10 +250,12 @@ __device__ A dense_rows(const TileRows &tr, const int *__restrict__ idx, const V
 template <int B, int TN, bool G, typename V, typename X, typename A, bool NT = false, bool SPLIT = false>
 __device__ __forceinline__ A tile_dot_rows(const TileRows &tr, const int *__restrict__ idx,
                                            const V *__restrict__ val, const X *__restrict__ xs,
-                                           void *lds_raw, A init = A(0)) {
+                                           void *lds_raw, A init = A(0), const A *pre = nullptr,
+                                           const int *rowseg = nullptr) {
     static_assert(valid_schedule(B, TN), "tile schedule");
     if constexpr (SPLIT) {
-        if (dense_group<TN>(tr.rows, tr.lo, tr.hi)) return dense_rows<B, TN, G, V, X, A, NT>(tr, idx, val, xs, lds_raw, init);
+        if (dense_group<TN>(tr.rows, tr.lo, tr.hi))
+            return dense_rows<B, TN, G, V, X, A, NT>(tr, idx, val, xs, lds_raw, init, pre, rowseg);
     }
     constexpr int kBits = ilog2(TN);
     constexpr uint32_t kMask = static_cast<uint32_t>(TN) - 1u;

@@ -42,12 +42,14 @@ __global__ __launch_bounds__(B) void k_pow_rows(const Tile *__restrict__ tiles, 
                                                      const int *__restrict__ ridx,
                                                      const float *__restrict__ rval,
                                                      const float *__restrict__ v,
-                                                     float *__restrict__ w) {
+                                                     float *__restrict__ w, const float *__restrict__ pre,
+                                                     const int *__restrict__ rowseg) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, rptr), ridx, rval, v, lds);
+    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, rptr), ridx, rval, v, lds,
+                                                                              0.0f, pre, rowseg);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     if (i < tl.end) w[i] = s;
 }
@@ -59,12 +61,14 @@ __global__ __launch_bounds__(B) void k_pow_cols(const Tile *__restrict__ tiles, 
                                                      const float *__restrict__ cval,
                                                      const float *__restrict__ w,
                                                      float *__restrict__ z,
-                                                     double *__restrict__ gpow) {
+                                                     double *__restrict__ gpow, const float *__restrict__ pre,
+                                                     const int *__restrict__ rowseg) {
     __shared__ __align__(16) unsigned char lds[stage_bytes(TN)];
     const int t = xcd_tile(blockIdx.x, chunk);
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
-    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, cptr), cidx, cval, w, lds);
+    const float s = tile_dot_rows<B, TN, G, float, float, float, false, true>(tile_rows(tl, cptr), cidx, cval, w, lds,
+                                                                              0.0f, pre, rowseg);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double q = 0.0;
     if (j < tl.end) {
@@ -97,6 +101,8 @@ struct PrimalArgs {
     int gcn;
     const int *itb;  // first iteration of the current check cycle (device)
     float rho;
+    const float *seg;   // split columns' segment sums, or null
+    const int *rowseg;  // per column: offset of its first segment sum
 };
 
 // Iteration it = *A.itb + off: the iteration number lives on the device, so
@@ -128,7 +134,7 @@ __device__ __forceinline__ void enh_primal_body(const Tile *__restrict__ tiles, 
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
     const float g = tile_dot_rows<B, TN, G, float, float, float, NT, true>(tr, A.cidx, A.cval, restart ? A.yp : A.y,
-                                                                     lds);
+                                                                           lds, 0.0f, A.seg, A.rowseg);
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
@@ -175,6 +181,8 @@ struct DualArgs {
     int grn;
     const int *itb;
     float rho;
+    const float *seg;   // split rows' segment sums, or null
+    const int *rowseg;  // per row: offset of its first segment sum
 };
 
 // One pass of a column-banded A xbar: band b holds each row's nonzeros in
@@ -213,7 +221,8 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
     const EnhInst &s = st[tl.inst];
     const int it = *A.itb + off;
     const bool restart = (s.restart_at == it);
-    const float h = tile_dot_rows<B, TN, G, float, float, float, NT, true>(tr, A.ridx, A.rval, A.xbar, lds, h0);
+    const float h = tile_dot_rows<B, TN, G, float, float, float, NT, true>(tr, A.ridx, A.rval, A.xbar, lds, h0,
+                                                                           A.seg, A.rowseg);
     if (BAND && !bp.last) {
         if (mine) bp.hpart[i] = h;
         return;
@@ -277,6 +286,64 @@ MPDHG_ENH_DUAL(k_enh_dual_band, false, true)
 MPDHG_ENH_DUAL(k_enh_dual_band_nt, true, true)
 #undef MPDHG_ENH_DUAL
 
+// ---- split rows: block sums, over many workgroups --------------------------
+// One workgroup per SplitChunk (one block of a split row): the block's
+// (value, x) pairs are staged in LDS in storage order, lane t sums segment t
+// (kSplitSeg nonzeros, a sequential FMA chain from zero), and lane 0 adds the
+// segment sums in order: exactly the block sums dense_rows forms.
+__global__ __launch_bounds__(kSplitBlock) void k_split_seg(const SplitChunk *__restrict__ items, int n,
+                                                           const int *__restrict__ idx,
+                                                           const float *__restrict__ val,
+                                                           const float *__restrict__ x,
+                                                           const float *__restrict__ xalt,
+                                                           const EnhInst *__restrict__ st,
+                                                           const int *__restrict__ itb, int off,
+                                                           float *__restrict__ blk) {
+    constexpr int PT = kSplitBlockNnz / kSplitBlock;
+    constexpr int NS = kSplitBlockNnz / kSplitSeg;
+    static_assert(NS <= kSplitBlock, "one segment per lane");
+    __shared__ Pair<float, float> lds[kSplitBlockNnz];
+    __shared__ float sg[NS];
+    const int i = static_cast<int>(blockIdx.x);
+    if (i >= n) return;
+    const SplitChunk c = items[i];
+    if (st) {
+        const EnhInst &s = st[c.inst];
+        if (s.status != kRunning) return;
+        if (xalt && s.restart_at == *itb + off) x = xalt;
+    }
+    const int tid = static_cast<int>(threadIdx.x), len = c.c1 - c.c0;
+    const uint32_t last = static_cast<uint32_t>(len - 1);
+    int id[PT];
+    float v[PT];
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+        const uint32_t o = min(static_cast<uint32_t>(tid + q * kSplitBlock), last);
+        id[q] = idx[c.c0 + static_cast<int>(o)];
+        v[q] = val[c.c0 + static_cast<int>(o)];
+    }
+    float g[PT];
+#pragma unroll
+    for (int q = 0; q < PT; ++q) g[q] = x[id[q]];
+#pragma unroll
+    for (int q = 0; q < PT; ++q)
+        lds[min(static_cast<uint32_t>(tid + q * kSplitBlock), last)] = Pair<float, float>{v[q], g[q]};
+    __syncthreads();
+    const int nseg = (len + kSplitSeg - 1) / kSplitSeg;
+    if (tid < nseg) {
+        const int a = tid * kSplitSeg, b = min(len, a + kSplitSeg);
+        float acc = 0.0f;
+        for (int k = a; k < b; ++k) acc = __builtin_fmaf(lds[k].v, lds[k].x, acc);
+        sg[tid] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float bs = 0.0f;
+        for (int t = 0; t < nseg; ++t) bs = bs + sg[t];
+        blk[c.slot] = bs;
+    }
+}
+
 // ---- FP64 checks on the original (unscaled) LP ----------------------------
 __device__ __forceinline__ bool fin(double v) { return isfinite(v); }
 
@@ -299,7 +366,8 @@ __global__ __launch_bounds__(B) void k_enh_chk_rows(const Tile *__restrict__ til
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const double axi = tile_dot_t<B, TN, G, double, double, double>(tl, A.rptr, A.ridx, A.rval, A.xchk, lds);
+    const double axi = tile_dot_rows<B, TN, G, double, double, double, false, true>(tile_rows(tl, A.rptr), A.ridx,
+                                                                                    A.rval, A.xchk, lds);
     const int i = tl.begin + static_cast<int>(threadIdx.x);
     double viol = 0.0, sviol = 0.0, dobj = 0.0;
     if (i < tl.end) {
@@ -339,7 +407,8 @@ __global__ __launch_bounds__(B) void k_enh_chk_cols(const Tile *__restrict__ til
     if (t >= ntiles) return;
     const Tile tl = tiles[t];
     if (!tl.run || st[tl.inst].status != kRunning) return;
-    const double aty = tile_dot_t<B, TN, G, double, double, double>(tl, A.cptr, A.cidx, A.cval, A.ychk, lds);
+    const double aty = tile_dot_rows<B, TN, G, double, double, double, false, true>(tile_rows(tl, A.cptr), A.cidx,
+                                                                                    A.cval, A.ychk, lds);
     const int j = tl.begin + static_cast<int>(threadIdx.x);
     double dv = 0.0, cv = 0.0, pobj = 0.0, dobj = 0.0;
     if (j < tl.end) {
@@ -619,12 +688,13 @@ void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float
     if (cts.count) hipLaunchKernelGGL(dev::k_pow_scale, MPDHG_TILE_GRID(cts), st, z, v);
 }
 void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
-              const float *rval, const float *v, float *w) {
-    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_pow_rows, rptr, ridx, rval, v, w);
+              const float *rval, const float *v, float *w, const float *pre, const int *rowseg) {
+    if (rts.count) MPDHG_LAUNCH_TN(rts, dev::k_pow_rows, rptr, ridx, rval, v, w, pre, rowseg);
 }
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
-              const float *cval, const float *w, float *z, double *gpow) {
-    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_pow_cols, cptr, cidx, cval, w, z, gpow);
+              const float *cval, const float *w, float *z, double *gpow, const float *pre,
+              const int *rowseg) {
+    if (cts.count) MPDHG_LAUNCH_TN(cts, dev::k_pow_cols, cptr, cidx, cval, w, z, gpow, pre, rowseg);
 }
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow) {
     hipLaunchKernelGGL(dev::k_pow_fin, dim3(ninst), dim3(kInstBlock), 0, s, st, ninst, gpow);
@@ -632,7 +702,7 @@ void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow) {
 void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhPrimal &a, int off,
                 int check, hipEvent_t e0, hipEvent_t e1) {
     dev::PrimalArgs A{a.cptr, a.cidx, a.cval, a.y, a.yp, a.c, a.l, a.u, a.x, a.x0, a.xp,
-                      a.xbar, a.xbase, a.dc, a.xchk, a.gc, a.gcn, a.itb, a.rho};
+                      a.xbar, a.xbase, a.dc, a.xchk, a.gc, a.gcn, a.itb, a.rho, a.seg, a.rowseg};
     if (!cts.count) return;
     if (e0 && cts.stream_nt)
         MPDHG_LAUNCH_TN_EV(cts, e0, e1, dev::k_enh_primal_nt, st, A, off, check);
@@ -646,7 +716,7 @@ void enh_primal(hipStream_t s, const TileSet &cts, const EnhInst *st, const EnhP
 void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDual &a, int off,
               int check, hipEvent_t e0, hipEvent_t e1) {
     dev::DualArgs A{a.rptr, a.ridx, a.rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
-                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
+                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho, a.seg, a.rowseg};
     const dev::BandPass bp{nullptr, 1, 1};
     if (!rts.count) return;
     if (e0 && rts.stream_nt)
@@ -662,7 +732,7 @@ void enh_dual_band(hipStream_t s, const TileSet &bts, const EnhInst *st, const E
                    const int *rptr, const int *ridx, const float *rval, float *hpart, int first,
                    int last, int off, int check, hipEvent_t e0, hipEvent_t e1) {
     dev::DualArgs A{rptr, ridx, rval, a.xbar, a.rl, a.ru, a.ybar, a.y, a.y0, a.yp,
-                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho};
+                    a.ybase, a.dr, a.ychk, a.gr, a.grn, a.itb, a.rho, nullptr, nullptr};
     const dev::BandPass bp{hpart, first, last};
     if (!bts.count) return;
     if ((e0 || e1) && bts.stream_nt)
@@ -673,6 +743,13 @@ void enh_dual_band(hipStream_t s, const TileSet &bts, const EnhInst *st, const E
         MPDHG_LAUNCH_TN(bts, dev::k_enh_dual_band_nt, st, A, off, check, bp);
     else
         MPDHG_LAUNCH_TN(bts, dev::k_enh_dual_band, st, A, off, check, bp);
+}
+void split_segments(hipStream_t s, const SplitChunk *items, int n, const int *idx, const float *val,
+                    const float *x, const float *xalt, const EnhInst *st, const int *itb, int off,
+                    float *blk) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(dev::k_split_seg, dim3(static_cast<unsigned>(n)), dim3(kSplitBlock), 0, s, items, n, idx,
+                       val, x, xalt, st, itb, off, blk);
 }
 void enh_advance(hipStream_t s, int *itb, int len) {
     hipLaunchKernelGGL(dev::k_enh_advance, dim3(1), dim3(64), 0, s, itb, len);

@@ -70,6 +70,8 @@ struct EnhPrimal {
     int gcn;
     const int *itb;  // device: first iteration of the current check cycle
     float rho;
+    const float *seg = nullptr;    // split columns' block sums (k_split_seg), or null
+    const int *rowseg = nullptr;   // per column: offset of its first block sum
 };
 struct EnhDual {
     const int *rptr, *ridx;
@@ -83,6 +85,8 @@ struct EnhDual {
     int grn;
     const int *itb;
     float rho;
+    const float *seg = nullptr;    // split rows' block sums (k_split_seg), or null
+    const int *rowseg = nullptr;   // per row: offset of its first block sum
 };
 struct EnhCheck {
     const int *rptr, *ridx, *cptr, *cidx;
@@ -102,11 +106,15 @@ struct EnhCheck {
 };
 
 void pow_scale(hipStream_t s, const TileSet &cts, const EnhInst *st, const float *z, float *v);
+// pre / rowseg (optional): the split rows' block sums formed from v by
+// split_segments (else the tiles form them).
 void pow_rows(hipStream_t s, const TileSet &rts, const int *rptr, const int *ridx,
-              const float *rval, const float *v, float *w);
+              const float *rval, const float *v, float *w, const float *pre = nullptr,
+              const int *rowseg = nullptr);
 // z = A^T w, with the column-group partials of |z|^2 in gpow.
 void pow_cols(hipStream_t s, const TileSet &cts, const int *cptr, const int *cidx,
-              const float *cval, const float *w, float *z, double *gpow);
+              const float *cval, const float *w, float *z, double *gpow, const float *pre = nullptr,
+              const int *rowseg = nullptr);
 // pow_norm2 = |z|^2 per instance, summed over its column groups in a fixed order.
 void pow_fin(hipStream_t s, EnhInst *st, int ninst, const double *gpow);
 // Iteration *a.itb + off of the current check cycle.  e0 / e1 (optional):
@@ -120,6 +128,13 @@ void enh_dual(hipStream_t s, const TileSet &rts, const EnhInst *st, const EnhDua
 void enh_dual_band(hipStream_t s, const TileSet &bts, const EnhInst *st, const EnhDual &a,
                    const int *rptr, const int *ridx, const float *rval, float *hpart, int first,
                    int last, int off, int check, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Block sums of the split rows (k_split_seg) into blk: each block gathers
+// from x, or from xalt where its instance restarts at iteration *itb + off
+// (the primal's A^T y reads y+ then); blocks of stopped instances are
+// skipped.  With st null every block is formed from x (power iteration).
+void split_segments(hipStream_t s, const SplitChunk *items, int n, const int *idx, const float *val,
+                    const float *x, const float *xalt, const EnhInst *st, const int *itb, int off,
+                    float *blk);
 // *itb += len (end of a check cycle of len iterations).
 void enh_advance(hipStream_t s, int *itb, int len);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)

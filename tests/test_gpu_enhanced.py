@@ -316,7 +316,8 @@ def _long_row_lp(seed=81):
 
 def test_split_rows_first_cycle_bitexact_vs_cpu_port(need_gpu):
     # Rows and columns of more than 8192 nonzeros are summed in 64-nonzero
-    # segments by the enhanced kernels and by the CPU port alike.
+    # segments within 4096-nonzero blocks, by the enhanced kernels (block sums
+    # from the pre-pass over many workgroups) and by the CPU port alike.
     p = _long_row_lp()
     sol = S.solve_gpu_pdhg(p, enh(tol=0.0, max_iterations=64, max_refinements=0))
     ref = port(p, sol, tol=0.0, max_iterations=64, check_frequency=64, max_refinements=0)

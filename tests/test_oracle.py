@@ -309,25 +309,33 @@ def test_enhanced_row_sums_are_sequential_up_to_8192_nonzeros():
     assert np.array_equal(a, b)
 
 
-def test_enhanced_long_rows_sum_in_64_nonzero_segments():
-    # One row of 12,800 nonzeros: 2^24 then 12,799 ones (x = 1).  One FMA
-    # chain loses every +1 (2^24 + 1 rounds to 2^24, ties to even): 2^24.
-    # Segments of 64 counted from the row's first nonzero: the first segment
-    # is 2^24 (its 63 ones lost), each of the other 199 is exactly 64, and
-    # 2^24 + 64 k is exact: 2^24 + 199 * 64 = 16,789,952.  A row of 8,193
-    # ones in segments is 8,193 exactly either way.
-    n1 = 12800
-    v1 = np.ones(n1, np.float32)
-    v1[0] = 2.0 ** 24
-    v = np.concatenate([v1, np.ones(8193, np.float32)])
-    ptr = np.array([0, n1, n1 + 8193], np.int32)
-    idx = np.zeros(len(v), np.int32)
+def test_enhanced_long_rows_sum_in_segments_and_blocks():
+    # Rows of more than 8192 nonzeros: segments of 64 nonzeros (FMA chains
+    # from zero) added in order within blocks of 4096, blocks added in order.
     x = np.ones(1, np.float32)
-    out = O.spmv_enh_f32(2, ptr, idx, v, x)
-    assert out[0] == np.float32(16789952.0)
-    assert O.spmv_seq_f32(2, ptr, idx, v, x)[0] == np.float32(2.0 ** 24)
-    assert out[1] == np.float32(8193.0)
-    # a dropped, repeated or shifted segment moves the sum of distinct values
-    w = (np.arange(len(v1)) % 7 + 1).astype(np.float32)
-    s = O.spmv_enh_f32(1, np.array([0, n1], np.int32), np.zeros(n1, np.int32), w, x)[0]
-    assert s == np.float32(w.astype(np.int64).sum())
+
+    def row(v):
+        return O.spmv_enh_f32(1, np.array([0, len(v)], np.int32), np.zeros(len(v), np.int32), v, x)[0]
+
+    # 2^24 then 12,799 ones: one chain loses every +1 (2^24 + 1 rounds to
+    # 2^24): 2^24.  Segment 0 is 2^24 (its 63 ones lost), the other 63 of
+    # block 0 are 64 each and 2^24 + 64 k is exact: block 0 = 2^24 + 4032;
+    # blocks 1 and 2 hold 4096 ones, block 3 512: 2^24 + 12,736.
+    v = np.ones(12800, np.float32)
+    v[0] = 2.0 ** 24
+    assert row(v) == np.float32(16789952.0)
+    ptr = np.array([0, 12800], np.int32)
+    assert O.spmv_seq_f32(1, ptr, np.zeros(12800, np.int32), v, x)[0] == np.float32(2.0 ** 24)
+    # 2^24, then a single 1 at the start of every other segment: inside block
+    # 0 each +1 is lost against 2^24, so block 0 is 2^24; blocks 1 and 2 sum
+    # 64 ones, block 3 eight, and those add exactly: 2^24 + 136.  One level
+    # of segments added in order would keep 2^24.
+    w = np.zeros(12800, np.float32)
+    w[0] = 2.0 ** 24
+    w[64::64] = 1.0
+    assert row(w) == np.float32(16777216.0 + 136.0)
+    # 8,193 ones: exact in any order
+    assert row(np.ones(8193, np.float32)) == np.float32(8193.0)
+    # distinct small integers sum exactly: no nonzero dropped or repeated
+    u = (np.arange(12800) % 7 + 1).astype(np.float32)
+    assert row(u) == np.float32(u.astype(np.int64).sum())
