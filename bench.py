@@ -30,6 +30,8 @@ Other workloads:
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
   stream     the device's streaming ceilings: float4 kernels reading 1 GiB
              (cached and non-temporal) and copying it, best of five grids.
+  longrow    enhanced-mode iterations on a random LP with one 500k-nonzero
+             row and one 100k-nonzero column (the row-split rule).
 
 The roofline object reports the dominant fused SpMV kernel: algorithmic
 bytes per launch / its average launch duration from HIP events recorded on
@@ -735,6 +737,60 @@ def bench_copy(args):
                          "frac": gbs / HBM_PEAK_GBS, "traffic": 2.0 * 4 * n}}
 
 
+def longrow_lp(seed: int = 3):
+    """A random equality LP (m = 100,000, n = 500,000, 4 nonzeros per column at
+    random rows, values U[-1, 1]) plus one dense row (row 0 over every column)
+    and one dense column (column 0 over every row): 2,599,950 nonzeros.
+    b = A x0 (x0 ~ U[0, 10]), 0 <= x <= 10, c = A^T y0 + s."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    import sih_solver as S
+
+    rng = np.random.default_rng(seed)
+    m, n = 100000, 500000
+    rows = np.sort(rng.integers(0, m, size=(n, 4)), axis=1).ravel()
+    A = sp.csc_matrix((rng.uniform(-1, 1, 4 * n), rows, np.arange(0, 4 * n + 1, 4)), shape=(m, n)).tolil()
+    A[0, :] = rng.uniform(-1, 1, n)
+    A[:, 0] = rng.uniform(-1, 1, (m, 1))
+    A = A.tocsc()
+    A.sum_duplicates()
+    A.sort_indices()
+    p = S.Problem("longrow")
+    p.resize(m, n)
+    p.A = S.SparseMatrix.from_csc(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data)
+    x0 = rng.uniform(0, 10, n)
+    b = A @ x0
+    p.c = A.T @ rng.normal(size=m) + rng.uniform(0, 1, n)
+    p.row_lower, p.row_upper = b.copy(), b.copy()
+    p.col_lower, p.col_upper = np.zeros(n), np.full(n, 10.0)
+    return p
+
+
+def bench_longrow(args):
+    """Enhanced-mode iterations on an LP with a dense row and a dense column
+    (the row-split rule: block sums over many workgroups, in-order sums)."""
+    import sih_solver as S
+
+    t = time.perf_counter()
+    p = longrow_lp()
+    log(f"generated m={p.num_rows()} n={p.num_cols()} nnz={p.num_nonzeros()} in {time.perf_counter() - t:.1f}s")
+    K = max(args.steps, 64)
+    cfg = S.GpuPdhgConfig.enhanced(tol=0.0, max_iterations=K, max_refinements=0,
+                                   launch_graph=0 if args.no_graph else 1)
+    for _ in range(args.warmup):
+        S.solve_gpu_pdhg(p, cfg)
+    sol = S.solve_gpu_pdhg(p, cfg)
+    us = 1e6 * sol.time_solver_sec / K
+    return {"metric": "enhanced PDHG iterations/s, LP with a 500k-nonzero row and a 100k-nonzero column",
+            "value": K / sol.time_solver_sec, "unit": "iterations/s", "ms_per_step": us / 1e3,
+            "dtype": "fp32 iterates, fp64 KKT",
+            "config": {"workload": f"longrow: random equality LP m=100000 n=500000 4 nnz/col plus a dense row "
+                                   f"and a dense column ({p.num_nonzeros()} nnz), enhanced mode, {K} iterations; "
+                                   "step = one iteration"},
+            "us_per_iteration": us, "rel_kkt": sol.rel_kkt}
+
+
 def bench_stream(args):
     """Streaming ceilings of the device (bench-only vendor library): best GB/s
     of float4 kernels reading 1 GiB through the caches, reading it
@@ -759,7 +815,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
                     choices=["sweep512", "cfg0", "large-ref", "large-enh", "staircase96", "adjoint64",
-                             "hybrid48", "copy", "stream"])
+                             "hybrid48", "copy", "stream", "longrow"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
@@ -798,6 +854,8 @@ def main():
         out = bench_copy(args)
     elif args.workload == "stream":
         out = bench_stream(args)
+    elif args.workload == "longrow":
+        out = bench_longrow(args)
     elif args.workload == "cfg0":
         out = bench_cfg0(args)
     elif args.workload == "staircase96":
