@@ -134,6 +134,22 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             act.push_back(q);
         }
     }
+    // Layout order, stratified by size: the running LPs ranked by nonzeros are
+    // dealt round-robin into 8 bins, laid out bin after bin.  Each XCD walks a
+    // contiguous eighth of the tile list, which then holds the same mix of LP
+    // sizes; in the caller's order, LPs that stop together (a batch sorted by
+    // size, say) would leave whole XCDs idle until the next compaction.  No
+    // result depends on the order.
+    {
+        std::vector<int> rank(act);
+        std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
+            return H[static_cast<size_t>(a)].nnz > H[static_cast<size_t>(b)].nnz;
+        });
+        constexpr int kXcds = 8;
+        act.clear();
+        for (int bin = 0; bin < kXcds; ++bin)
+            for (size_t k = static_cast<size_t>(bin); k < rank.size(); k += kXcds) act.push_back(rank[k]);
+    }
     const int na = static_cast<int>(act.size());
     if (na == 0) {
         for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;

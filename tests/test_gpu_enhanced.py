@@ -338,3 +338,22 @@ def test_split_rows_every_schedule_and_batch_bit_identical(need_gpu, layout, blo
     assert np.array_equal(a[0].x, b.x) and np.array_equal(a[0].row_duals, b.row_duals)
     assert a[0].rel_kkt == b.rel_kkt
     assert O.kkt(O.LpArrays.of(p), b.x, b.row_duals)["kkt"] <= 1e-6 * 1.0001
+
+
+def test_batch_order_changes_speed_only(need_gpu):
+    # The running LPs are laid out stratified by size (each XCD's eighth of
+    # the tile list holds the same mix), whatever the caller's order: the
+    # same LPs in reversed and in size-sorted order give every result bit for bit.
+    probs = [S.gen_random(300 + 40 * k, 720 + 96 * k, 8, 90 + k) for k in range(6)]
+    probs += [S.gen_staircase(2, 40 + 10 * k, 90 + 20 * k, 96 + k) for k in range(4)] + [toy01()]
+    cfg = enh(tol=1e-6)
+    a = S.solve_batch(probs, cfg)
+    rev = S.solve_batch(probs[::-1], cfg)[::-1]
+    order = sorted(range(len(probs)), key=lambda q: probs[q].num_nonzeros())
+    srt = S.solve_batch([probs[q] for q in order], cfg)
+    by_q = {q: s for q, s in zip(order, srt)}
+    for q, p in enumerate(probs):
+        for t in (rev[q], by_q[q]):
+            assert t.simplex_iterations == a[q].simplex_iterations and t.restarts == a[q].restarts, p.name
+            assert np.array_equal(t.x, a[q].x) and np.array_equal(t.row_duals, a[q].row_duals), p.name
+            assert t.rel_kkt == a[q].rel_kkt, p.name
