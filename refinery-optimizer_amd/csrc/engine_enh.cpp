@@ -19,6 +19,9 @@ namespace mpdhg {
 
 namespace {
 
+// XCDs of an MI355X: the tile lists are laid out in one bin per XCD.
+constexpr size_t kXcds = 8;
+
 // Stream bytes per iteration above which the fused kernels load the matrix
 // stream non-temporally: the size of MI355X's last-level (Infinity) cache.
 constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
@@ -145,10 +148,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
             return H[static_cast<size_t>(a)].nnz > H[static_cast<size_t>(b)].nnz;
         });
-        constexpr int kXcds = 8;
         act.clear();
-        for (int bin = 0; bin < kXcds; ++bin)
-            for (size_t k = static_cast<size_t>(bin); k < rank.size(); k += kXcds) act.push_back(rank[k]);
+        for (size_t bin = 0; bin < kXcds; ++bin)
+            for (size_t k = bin; k < rank.size(); k += kXcds) act.push_back(rank[k]);
     }
     const int na = static_cast<int>(act.size());
     if (na == 0) {
@@ -612,12 +614,21 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     // Compact: later launches cover only the running instances'
                     // tiles.  Stream order puts the range update after the cycle
                     // already in flight, which still uses the old lists.
+                    // the running LPs dealt into the XCD bins again, by size
+                    std::vector<int> run;
+                    for (int a = 0; a < na; ++a)
+                        if (sv[a] == kRunning) run.push_back(a);
+                    std::stable_sort(run.begin(), run.end(), [&](int a, int b) {
+                        return H[static_cast<size_t>(act[static_cast<size_t>(a)])].nnz >
+                               H[static_cast<size_t>(act[static_cast<size_t>(b)])].nnz;
+                    });
                     std::vector<Tile> nrt, nct;
-                    for (int a = 0; a < na; ++a) {
-                        if (sv[a] != kRunning) continue;
-                        const EnhInst &s0 = hh.inst[static_cast<size_t>(a)];
-                        nrt.insert(nrt.end(), RD.host.begin() + s0.row_tile_begin, RD.host.begin() + s0.row_tile_end);
-                        nct.insert(nct.end(), CD.host.begin() + s0.col_tile_begin, CD.host.begin() + s0.col_tile_end);
+                    for (size_t bin = 0; bin < kXcds; ++bin) {
+                        for (size_t k = bin; k < run.size(); k += kXcds) {
+                            const EnhInst &s0 = hh.inst[static_cast<size_t>(run[k])];
+                            nrt.insert(nrt.end(), RD.host.begin() + s0.row_tile_begin, RD.host.begin() + s0.row_tile_end);
+                            nct.insert(nct.end(), CD.host.begin() + s0.col_tile_begin, CD.host.begin() + s0.col_tile_end);
+                        }
                     }
                     keep_rt.emplace_back();
                     keep_rt.back().upload(nrt, st);
