@@ -106,6 +106,16 @@ void plan_tiles(const std::vector<int> &ptr, int count, int64_t global_off, int6
     }
 }
 
+void stratify_by_size(std::vector<int> &act, const std::vector<HostLp> &H) {
+    std::vector<int> rank(act);
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
+        return H[static_cast<size_t>(a)].nnz > H[static_cast<size_t>(b)].nnz;
+    });
+    act.clear();
+    for (size_t bin = 0; bin < kXcds; ++bin)
+        for (size_t k = bin; k < rank.size(); k += kXcds) act.push_back(rank[k]);
+}
+
 void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H) {
     H.resize(static_cast<size_t>(count));
     std::vector<std::exception_ptr> err(static_cast<size_t>(count));
@@ -357,6 +367,7 @@ void solve_reference(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_c
             act.push_back(q);
         }
     }
+    stratify_by_size(act, H);
     const int na = static_cast<int>(act.size());
     if (na == 0) {
         for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;

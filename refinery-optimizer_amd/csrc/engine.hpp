@@ -141,6 +141,15 @@ struct HostLp {
     std::vector<int> cptr, cidx, rptr, ridx, csc2csr;  // csc2csr[k]: CSR slot of CSC entry k
 };
 void prepare_structure(const mpdhg_lp &lp, HostLp &h);
+// XCDs of an MI355X: each walks a contiguous eighth of a tile list.
+constexpr size_t kXcds = 8;
+// Layout order of a batch's LPs, stratified by size: the LPs (indices into H)
+// ranked by nonzeros are dealt round-robin into kXcds bins, laid out bin
+// after bin, so that every XCD's eighth of the tile list holds the same mix
+// of LP sizes whatever the caller's order (LPs that stop together would
+// otherwise leave whole XCDs idle).  No result depends on the order.
+void stratify_by_size(std::vector<int> &act, const std::vector<HostLp> &H);
+
 // prepare_structure of every LP of a batch, on the host threads; an invalid
 // LP raises the error the lowest-numbered invalid LP gives.
 void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H);

@@ -19,9 +19,6 @@ namespace mpdhg {
 
 namespace {
 
-// XCDs of an MI355X: the tile lists are laid out in one bin per XCD.
-constexpr size_t kXcds = 8;
-
 // Stream bytes per iteration above which the fused kernels load the matrix
 // stream non-temporally: the size of MI355X's last-level (Infinity) cache.
 constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
@@ -137,21 +134,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
             act.push_back(q);
         }
     }
-    // Layout order, stratified by size: the running LPs ranked by nonzeros are
-    // dealt round-robin into 8 bins, laid out bin after bin.  Each XCD walks a
-    // contiguous eighth of the tile list, which then holds the same mix of LP
-    // sizes; in the caller's order, LPs that stop together (a batch sorted by
-    // size, say) would leave whole XCDs idle until the next compaction.  No
-    // result depends on the order.
-    {
-        std::vector<int> rank(act);
-        std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
-            return H[static_cast<size_t>(a)].nnz > H[static_cast<size_t>(b)].nnz;
-        });
-        act.clear();
-        for (size_t bin = 0; bin < kXcds; ++bin)
-            for (size_t k = bin; k < rank.size(); k += kXcds) act.push_back(rank[k]);
-    }
+    stratify_by_size(act, H);
     const int na = static_cast<int>(act.size());
     if (na == 0) {
         for (int q = 0; q < count; ++q) res[q].time_wall_sec = now_sec() - t_call;
