@@ -206,8 +206,8 @@ int mpdhg_solve(mpdhg_ctx *ctx, const mpdhg_lp *lp, const mpdhg_config *cfg,
  * result: every result equals
  * mpdhg_solve's bit for bit.  An LP whose buffers could exceed the scratch is
  * rejected with MPDHG_E_ARG before any device work; reference mode needs about
- * 32 B per nonzero and 60 B per row and column, enhanced mode about 64 B per
- * nonzero and 100 B per row and column (so LPs up to ~1.5M and ~0.7M nonzeros
+ * 32 B per nonzero and 60 B per row and column, enhanced mode about 68 B per
+ * nonzero and 116 B per row and column (so LPs up to ~1.5M and ~0.65M nonzeros
  * fit).  Calls on one GPU share its scratch and leave their buffers in it:
  * calls that overlap on one GPU (from several threads or contexts) race and
  * must be serialised by the caller. */

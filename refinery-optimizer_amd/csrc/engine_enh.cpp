@@ -26,9 +26,9 @@ constexpr double kNtStreamBytes = 256.0 * 1024 * 1024;
 // The batch's host arrays, written in full by the per-instance preparation
 // (hence uninitialised: see HostArray), except the optional warm-start duals.
 struct EnhHost {
-    HostArray<int> cptr, cidx, rptr, ridx;
-    HostArray<float> cval, rval, cs, ls, us, rls, rus, x0, powv;
-    HostArray<double> cval64, rval64, c64, l64, u64, dc, rl64, ru64, dr;
+    HostArray<int> cptr, cidx, rptr, perm;
+    HostArray<float> cs, ls, us, rls, rus, x0, powv;
+    HostArray<double> cval64, c64, l64, u64, dc, rl64, ru64, dr;
     std::vector<float> y0;
     std::vector<EnhInst> inst;
 };
@@ -144,8 +144,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const size_t R = static_cast<size_t>(L.rows), C = static_cast<size_t>(L.cols),
                  Z = static_cast<size_t>(L.nnz);
     EnhHost hh;
-    hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.ridx.resize(Z);
-    hh.cval.resize(Z); hh.rval.resize(Z); hh.cval64.resize(Z); hh.rval64.resize(Z);
+    hh.cptr.resize(C + 1); hh.cidx.resize(Z); hh.rptr.resize(R + 1); hh.perm.resize(Z); hh.cval64.resize(Z);
     hh.cs.resize(C); hh.ls.resize(C); hh.us.resize(C); hh.x0.resize(C); hh.powv.resize(C);
     hh.c64.resize(C); hh.l64.resize(C); hh.u64.resize(C); hh.dc.resize(C);
     hh.rls.resize(R); hh.rus.resize(R); hh.rl64.resize(R); hh.ru64.resize(R); hh.dr.resize(R);
@@ -153,30 +152,57 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     hh.inst.resize(static_cast<size_t>(na));
     std::vector<double> pow0(static_cast<size_t>(na));
 
-    // ---- per-instance host preparation (independent, in parallel) ----
+    // ---- per-instance host preparation, part 1 (independent, in parallel):
+    // the batch's structure and its FP64 values in CSC order ----
     parallel_for(na, [&](int a) {
         const int q = act[static_cast<size_t>(a)];
         const HostLp &h = H[static_cast<size_t>(q)];
         const mpdhg_lp &lp = lps[q];
         const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)],
                       zo = L.nnz_off[static_cast<size_t>(a)];
-        double *dr = hh.dr.data() + ro, *dc = hh.dc.data() + co;
-        scale_instance(lp, h, cfg.ruiz_iterations, dr, dc);
         for (int j = 0; j < h.n; ++j) hh.cptr[static_cast<size_t>(co + j)] = static_cast<int>(zo + h.cptr[j]);
         for (int i = 0; i < h.m; ++i) hh.rptr[static_cast<size_t>(ro + i)] = static_cast<int>(zo + h.rptr[i]);
-        for (int j = 0; j < h.n; ++j)
-            for (int k = h.cptr[j]; k < h.cptr[j + 1]; ++k) {
-                const int i = h.cidx[static_cast<size_t>(k)];
-                const float s = static_cast<float>(dr[i] * lp.vals[k] * dc[j]);
-                const int d = h.csc2csr[static_cast<size_t>(k)];
-                const size_t gk = static_cast<size_t>(zo + k), gd = static_cast<size_t>(zo + d);
-                hh.cval[gk] = s;
-                hh.cval64[gk] = lp.vals[k];
-                hh.cidx[gk] = static_cast<int>(ro + i);
-                hh.rval[gd] = s;
-                hh.rval64[gd] = lp.vals[k];
-                hh.ridx[gd] = static_cast<int>(co + j);
-            }
+        for (int k = 0; k < h.nnz; ++k) {
+            const size_t gk = static_cast<size_t>(zo + k);
+            hh.cidx[gk] = static_cast<int>(ro + h.cidx[static_cast<size_t>(k)]);
+            hh.cval64[gk] = lp.vals[k];
+            hh.perm[gk] = static_cast<int>(zo + h.csc2csr[static_cast<size_t>(k)]);
+        }
+    });
+    hh.cptr[C] = static_cast<int>(Z);
+    hh.rptr[R] = static_cast<int>(Z);
+
+    // ---- on the device: the CSR copy, Ruiz / Pock-Chambolle scaling and the
+    // scaled FP32 values (the matrix in storage order: the layout source) ----
+    hipStream_t st = ctx.stream;
+    DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s(Z);
+    DBuf<float> d_cval_s(Z), d_rval_s(Z), d_cs, d_ls, d_us, d_rls, d_rus;
+    DBuf<double> d_cval64_s, d_rval64_s(Z), d_c64, d_l64, d_u64, d_dc(C), d_rl64, d_ru64, d_dr(R);
+    {
+        DBuf<int> d_perm;
+        DBuf<double> d_rs(R), d_csum(C);
+        d_cptr.upload(hh.cptr.data(), hh.cptr.size(), st);
+        d_cidx_s.upload(hh.cidx.data(), hh.cidx.size(), st);
+        d_rptr.upload(hh.rptr.data(), hh.rptr.size(), st);
+        d_perm.upload(hh.perm.data(), hh.perm.size(), st);
+        d_cval64_s.upload(hh.cval64.data(), hh.cval64.size(), st);
+        launch::scale_batch(st, static_cast<int>(R), static_cast<int>(C), d_cptr.get(), d_cidx_s.get(), d_rptr.get(),
+                            d_perm.get(), d_cval64_s.get(), d_rval64_s.get(), d_ridx_s.get(), cfg.ruiz_iterations,
+                            d_dr.get(), d_dc.get(), d_rs.get(), d_csum.get(), d_cval_s.get(), d_rval_s.get());
+        MPDHG_HIP(hipGetLastError());
+        MPDHG_HIP(hipMemcpyAsync(hh.dr.data(), d_dr.get(), R * sizeof(double), hipMemcpyDeviceToHost, st));
+        MPDHG_HIP(hipMemcpyAsync(hh.dc.data(), d_dc.get(), C * sizeof(double), hipMemcpyDeviceToHost, st));
+        MPDHG_HIP(hipStreamSynchronize(st));
+    }
+
+    // ---- per-instance host preparation, part 2: the scaled vectors, norms
+    // and the instance state (D_r, D_c from the device) ----
+    parallel_for(na, [&](int a) {
+        const int q = act[static_cast<size_t>(a)];
+        const HostLp &h = H[static_cast<size_t>(q)];
+        const mpdhg_lp &lp = lps[q];
+        const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)];
+        const double *dr = hh.dr.data() + ro, *dc = hh.dc.data() + co;
         double cn2 = 0.0, bn2 = 0.0, cinf = 0.0, binf = 0.0, pn = 0.0;
         // a warm start in the scaled space: x^ = x / D_c, y^ = y / D_r
         const mpdhg_warm_start *ws = warm ? &warm[q] : nullptr;
@@ -232,8 +258,6 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.pow_norm2 = pn;
         pow0[static_cast<size_t>(a)] = pn;
     });
-    hh.cptr[C] = static_cast<int>(Z);
-    hh.rptr[R] = static_cast<int>(Z);
     for (int a = 0; a < na; ++a) {
         const int q = act[static_cast<size_t>(a)];
         const HostLp &h = H[static_cast<size_t>(q)];
@@ -245,19 +269,11 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
     }
 
-    // ---- device buffers (the matrix in storage order: the layout source) ----
-    hipStream_t st = ctx.stream;
-    DBuf<int> d_cptr, d_cidx_s, d_rptr, d_ridx_s;
-    DBuf<float> d_cval_s, d_rval_s, d_cs, d_ls, d_us, d_rls, d_rus;
-    DBuf<double> d_cval64_s, d_rval64_s, d_c64, d_l64, d_u64, d_dc, d_rl64, d_ru64, d_dr;
-    d_cptr.upload(hh.cptr.data(), hh.cptr.size(), st); d_cidx_s.upload(hh.cidx.data(), hh.cidx.size(), st); d_rptr.upload(hh.rptr.data(), hh.rptr.size(), st);
-    d_ridx_s.upload(hh.ridx.data(), hh.ridx.size(), st); d_cval_s.upload(hh.cval.data(), hh.cval.size(), st); d_rval_s.upload(hh.rval.data(), hh.rval.size(), st);
-    d_cval64_s.upload(hh.cval64.data(), hh.cval64.size(), st); d_rval64_s.upload(hh.rval64.data(), hh.rval64.size(), st);
+    // ---- device buffers: the vectors ----
     d_cs.upload(hh.cs.data(), hh.cs.size(), st); d_ls.upload(hh.ls.data(), hh.ls.size(), st); d_us.upload(hh.us.data(), hh.us.size(), st);
     d_rls.upload(hh.rls.data(), hh.rls.size(), st); d_rus.upload(hh.rus.data(), hh.rus.size(), st);
     d_c64.upload(hh.c64.data(), hh.c64.size(), st); d_l64.upload(hh.l64.data(), hh.l64.size(), st); d_u64.upload(hh.u64.data(), hh.u64.size(), st);
-    d_dc.upload(hh.dc.data(), hh.dc.size(), st); d_rl64.upload(hh.rl64.data(), hh.rl64.size(), st); d_ru64.upload(hh.ru64.data(), hh.ru64.size(), st);
-    d_dr.upload(hh.dr.data(), hh.dr.size(), st);
+    d_rl64.upload(hh.rl64.data(), hh.rl64.size(), st); d_ru64.upload(hh.ru64.data(), hh.ru64.size(), st);
     DBuf<float> d_x(C), d_x0(C), d_xp(C), d_xbar(C), d_y(R), d_y0(R), d_yp(R), d_ybar(R);
     DBuf<double> d_xbase(C), d_xchk(C), d_lam(C), d_xbest(C), d_lambest(C);
     DBuf<double> d_ybase(R), d_ychk(R), d_ax(R), d_ybest(R), d_axbest(R);
@@ -299,9 +315,16 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     }
     const int bands = (na == 1 && !noalloc && !split_rows) ? band_count(cfg.spmv_bands, L.cols) : 1;
     ColumnBands BD;
-    if (bands > 1)
-        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr.data(), hh.ridx.data(), hh.rval.data(), cfg, lay,
+    if (bands > 1) {
+        // the bands are cut on the host from the CSR the device built
+        std::vector<int> ridx;
+        std::vector<float> rval;
+        d_ridx_s.download(ridx, Z, st);
+        d_rval_s.download(rval, Z, st);
+        MPDHG_HIP(hipStreamSynchronize(st));
+        tune_ms += build_bands(bands, static_cast<int>(R), L.cols, hh.rptr.data(), ridx.data(), rval.data(), cfg, lay,
                                tune_wanted(cfg, L.nnz) && !noalloc, d_x.get(), d_y.get(), st, BD);
+    }
     for (int a = 0; a < na; ++a) {
         EnhInst &s = hh.inst[static_cast<size_t>(a)];
         s.row_tile_begin = RD.first[static_cast<size_t>(a)];

@@ -88,6 +88,39 @@ __global__ __launch_bounds__(kInstBlock) void k_pow_fin(EnhInst *__restrict__ st
     if (threadIdx.x == 0) s.pow_norm2 = acc;
 }
 
+// The iteration's arithmetic per column and per row.
+struct PrimalStep {
+    float v;     // x+ = proj(x - tau (c + A^T y))
+    float xbar;  // 2 x+ - x
+    float x;     // the Halpern update: a ((1 + rho) x+ - rho x) + b x0
+};
+__device__ __forceinline__ PrimalStep primal_step(float g, float xo, float x0j, float cj, float lo, float hi,
+                                                  float tau, float rho, float a, float b) {
+    const float rho1 = 1.0f + rho;
+    const float tt = cj + g;
+    float v = __builtin_fmaf(-tau, tt, xo);
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    const float r = __builtin_fmaf(rho1, v, -(rho * xo));
+    return PrimalStep{v, 2.0f * v - xo, __builtin_fmaf(a, r, b * x0j)};
+}
+struct DualStep {
+    float yn;  // y+, the shifted dual prox
+    float y;   // the Halpern update
+};
+// y+ = sigma (w - clamp(w, rl, ru)) - ybar, w = h + (y + ybar) / sigma:
+// y + sigma (h - bound) on an active bound, -ybar in between.
+__device__ __forceinline__ DualStep dual_step(float h, float yo, float y0i, float yb, float lo, float hi,
+                                              float sigma, float inv_sigma, float rho, float a, float b) {
+    const float rho1 = 1.0f + rho;
+    const float w = __builtin_fmaf(yo + yb, inv_sigma, h);
+    float yn = -yb;
+    if (w < lo) yn = __builtin_fmaf(sigma, h - lo, yo);
+    else if (w > hi) yn = __builtin_fmaf(sigma, h - hi, yo);
+    const float r = __builtin_fmaf(rho1, yn, -(rho * yo));
+    return DualStep{yn, __builtin_fmaf(a, r, b * y0i)};
+}
+
 // ---- K1: A^T y fused with the projected primal step and Halpern update ----
 struct PrimalArgs {
     const int *cptr, *cidx;
@@ -138,7 +171,6 @@ __device__ __forceinline__ void enh_primal_body(const Tile *__restrict__ tiles, 
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
-    const float tau = s.tau, rho = A.rho, rho1 = 1.0f + A.rho;
     double d2 = 0.0, e2 = 0.0;
     if (mine) {
         if (restart) {  // z+ of the last check becomes iterate and anchor
@@ -146,13 +178,10 @@ __device__ __forceinline__ void enh_primal_body(const Tile *__restrict__ tiles, 
             x0j = xo;
             A.x0[j] = x0j;
         }
-        const float tt = cj + g;
-        float v = __builtin_fmaf(-tau, tt, xo);
-        if (v < lo) v = lo;
-        if (v > hi) v = hi;
-        A.xbar[j] = 2.0f * v - xo;
-        const float r = __builtin_fmaf(rho1, v, -(rho * xo));
-        A.x[j] = __builtin_fmaf(a, r, b * x0j);
+        const PrimalStep ps = primal_step(g, xo, x0j, cj, lo, hi, s.tau, A.rho, a, b);
+        const float v = ps.v;
+        A.xbar[j] = ps.xbar;
+        A.x[j] = ps.x;
         if (check) {
             A.xp[j] = v;
             const double d = static_cast<double>(xo) - static_cast<double>(v);
@@ -230,7 +259,6 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
     if (s.status != kRunning) return;  // a stale run hint: store nothing
     float a, b;
     halpern_weights(it, s.last_restart, a, b);
-    const float sigma = s.sigma, inv_sigma = s.inv_sigma, rho = A.rho, rho1 = 1.0f + A.rho;
     double d2 = 0.0, e2 = 0.0;
     if (mine) {
         if (restart) {
@@ -238,14 +266,9 @@ __device__ __forceinline__ void enh_dual_body(const Tile *__restrict__ tiles, in
             y0i = yo;
             A.y0[i] = y0i;
         }
-        // y+ = sigma (w - clamp(w, rl, ru)) - ybar, w = h + (y + ybar) / sigma:
-        // y + sigma (h - bound) on an active bound, -ybar in between.
-        const float w = __builtin_fmaf(yo + yb, inv_sigma, h);
-        float yn = -yb;
-        if (w < lo) yn = __builtin_fmaf(sigma, h - lo, yo);
-        else if (w > hi) yn = __builtin_fmaf(sigma, h - hi, yo);
-        const float r = __builtin_fmaf(rho1, yn, -(rho * yo));
-        A.y[i] = __builtin_fmaf(a, r, b * y0i);
+        const DualStep ds = dual_step(h, yo, y0i, yb, lo, hi, s.sigma, s.inv_sigma, A.rho, a, b);
+        const float yn = ds.yn;
+        A.y[i] = ds.y;
         if (check) {
             A.yp[i] = yn;
             const double d = static_cast<double>(yo) - static_cast<double>(yn);

@@ -137,6 +137,16 @@ void split_segments(hipStream_t s, const SplitChunk *items, int n, const int *id
                     float *blk);
 // *itb += len (end of a check cycle of len iterations).
 void enh_advance(hipStream_t s, int *itb, int len);
+// Set-up of the enhanced mode on the device (kernels_scale.hip), for a whole
+// batch: the CSR copy of the FP64 values and its index words (ridx,
+// rval64[perm[k]] = cval64[k]), then ruiz Ruiz max-norm passes and one
+// Pock-Chambolle pass giving D_r (dr) and D_c (dc), then the scaled FP32 values
+// (float)((dr_i a) dc_j) in both orders.  rs / cs are scratch of rows / cols
+// doubles.  The sums run per row and column in storage order, so D_r and D_c
+// equal scale_instance's bit for bit.
+void scale_batch(hipStream_t s, int rows, int cols, const int *cptr, const int *cidx, const int *rptr,
+                 const int *perm, const double *cval64, double *rval64, int *ridx, int ruiz, double *dr,
+                 double *dc, double *rs, double *cs, float *cval32, float *rval32);
 // FP64 check, per-instance decisions, best-point copies and (when enabled)
 // residual-LP construction, in that stream order.
 void enh_check(hipStream_t s, const TileSet &rts, const TileSet &cts, EnhInst *st, int ninst,

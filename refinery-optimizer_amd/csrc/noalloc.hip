@@ -88,6 +88,9 @@ size_t noalloc_bytes(int mode, int64_t m, int64_t n, int64_t nnz, bool gather) {
         return total;
     }
     bufs(2, Z, 8);  // FP64 values, both directions
+    buf(Z, 4);      // CSC -> CSR permutation (set-up)
+    buf(R, 8);      // row and column norms of the scaling passes (set-up)
+    buf(C, 8);
     bufs(3, C, 4);  // scaled c, l, u
     bufs(2, R, 4);  // scaled rl, ru
     bufs(4, C, 8);  // c, l, u, D_c in FP64
