@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <limits>
@@ -111,6 +113,17 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                                          : std::max<double>(cfg.tol_primal, std::max(cfg.tol_dual, cfg.tol_gap));
     if (cfg.reflection < 0.0f || cfg.reflection > 1.0f) throw Error(MPDHG_E_ARG, "reflection must be in [0, 1]");
     if (cfg.ruiz_iterations < 0) throw Error(MPDHG_E_ARG, "ruiz_iterations must be >= 0");
+    // MPDHG_SETUP_TRACE set: the set-up's phases are timed (each after a
+    // stream synchronisation) and printed to stderr.
+    const bool trace = std::getenv("MPDHG_SETUP_TRACE") != nullptr;
+    double t_mark = t_call;
+    auto mark = [&](const char *what) {
+        if (!trace) return;
+        MPDHG_HIP(hipStreamSynchronize(ctx.stream));
+        const double t = now_sec();
+        std::fprintf(stderr, "[mpdhg setup] %-34s %8.2f ms\n", what, 1e3 * (t - t_mark));
+        t_mark = t;
+    };
     std::vector<HostLp> H(static_cast<size_t>(count));
     prepare_all(lps, count, H);
     std::vector<int> act;
@@ -151,6 +164,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     hh.y0.assign(R, 0.0f);
     hh.inst.resize(static_cast<size_t>(na));
     std::vector<double> pow0(static_cast<size_t>(na));
+    mark("CSR builds, layout, host arrays");
 
     // ---- per-instance host preparation, part 1 (independent, in parallel):
     // the batch's structure and its FP64 values in CSC order ----
@@ -171,6 +185,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     });
     hh.cptr[C] = static_cast<int>(Z);
     hh.rptr[R] = static_cast<int>(Z);
+    mark("structure and FP64 values");
 
     // ---- on the device: the CSR copy, Ruiz / Pock-Chambolle scaling and the
     // scaled FP32 values (the matrix in storage order: the layout source) ----
@@ -194,6 +209,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         MPDHG_HIP(hipMemcpyAsync(hh.dc.data(), d_dc.get(), C * sizeof(double), hipMemcpyDeviceToHost, st));
         MPDHG_HIP(hipStreamSynchronize(st));
     }
+    mark("uploads, scaling on the device");
 
     // ---- per-instance host preparation, part 2: the scaled vectors, norms
     // and the instance state (D_r, D_c from the device) ----
@@ -269,6 +285,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.cgrp_end = static_cast<int32_t>(L.cgrp_off[ua] + (h.n + kGroup - 1) / kGroup);
     }
 
+    mark("scaled vectors, norms");
     // ---- device buffers: the vectors ----
     d_cs.upload(hh.cs.data(), hh.cs.size(), st); d_ls.upload(hh.ls.data(), hh.ls.size(), st); d_us.upload(hh.us.data(), hh.us.size(), st);
     d_rls.upload(hh.rls.data(), hh.rls.size(), st); d_rus.upload(hh.rus.data(), hh.rus.size(), st);
@@ -286,6 +303,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     const int gcn = static_cast<int>(L.cgroups), grn = static_cast<int>(L.rgroups);
     DBuf<double> d_gc(static_cast<size_t>(kGcCount) * L.cgroups), d_gr(static_cast<size_t>(kGrCount) * L.rgroups);
 
+    mark("vector uploads, buffers");
     // ---- tile schedules: searched on this batch's own matrix (the iterate
     // buffers serve as scratch before they are initialised), or by rule ----
     const Layout lay = static_cast<Layout>(cfg.spmv_layout);
@@ -304,6 +322,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         build_dir(RP, choose_schedule(L.nnz, L.rows, cfg.spmv_block, cfg.spmv_chunk), lay, RS, true, true, st, RD);
         build_dir(CP, choose_schedule(L.nnz, L.cols, cfg.spmv_block, cfg.spmv_chunk), lay, CS, true, true, st, CD);
     }
+    mark("tile layouts, schedule search");
     // ---- column bands of the A xbar product of the fused dual kernel (one
     // LP whose xbar exceeds an XCD's L2; bit-identical to the unbanded
     // product, which the no-allocation solve keeps) ----
@@ -366,6 +385,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     // power iteration vectors reuse the iterate buffers before the solve
     MPDHG_HIP(hipMemcpyAsync(d_xbar.get(), hh.powv.data(), C * sizeof(float), hipMemcpyHostToDevice, st));
     MPDHG_HIP(hipStreamSynchronize(st));
+    mark("bands, instance state, start point");
     const double t_loop = now_sec();
 
     // ---- split rows (more than kSplitRow nonzeros, either direction): a
