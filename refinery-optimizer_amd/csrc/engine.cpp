@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <thread>
 
 namespace mpdhg {
 
@@ -33,12 +35,13 @@ Context::~Context() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+
 // Validates the int64 CSC of model::SparseMatrix (sparse_matrix.hpp:38-40)
 // and narrows it to int32 -- element by element, which is the copy the
 // reference gets wrong at gpu_solver.cpp:202-203 by moving int64 words with
-// int32 byte counts.  Builds the CSR by the counting sort of
-// gpu_solver.cpp:92-109 (columns ascending within each row).
-void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
+// int32 byte counts.
+void narrow_structure(const mpdhg_lp &lp, HostLp &h) {
     const int64_t lim = std::numeric_limits<int32_t>::max() - kTileNnz;
     if (lp.m < 0 || lp.n < 0 || lp.nnz < 0 || lp.m > lim || lp.n > lim || lp.nnz > lim)
         throw Error(MPDHG_E_ARG, "LP dimensions out of range (m, n, nnz must fit int32)");
@@ -68,6 +71,11 @@ void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
         if (r < 0 || r >= lp.m) throw Error(MPDHG_E_ARG, "row index out of range");
         h.cidx[static_cast<size_t>(k)] = static_cast<int>(r);
     }
+}
+
+// The CSR by the counting sort of gpu_solver.cpp:92-109 (columns ascending
+// within each row).
+void build_csr(HostLp &h) {
     h.rptr.assign(static_cast<size_t>(h.m) + 1, 0);
     for (int k = 0; k < h.nnz; ++k) h.rptr[static_cast<size_t>(h.cidx[k]) + 1]++;
     for (int i = 0; i < h.m; ++i) h.rptr[static_cast<size_t>(i) + 1] += h.rptr[static_cast<size_t>(i)];
@@ -80,6 +88,63 @@ void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
             h.ridx[static_cast<size_t>(d)] = j;
             h.csc2csr[static_cast<size_t>(k)] = d;
         }
+}
+
+// The same CSR on `parts` host threads: the columns are cut into parts
+// ranges of about equal nonzeros; part t counts its entries per row, a
+// per-row scan over the parts gives part t its first slot in each row, and
+// each part scatters its columns in order -- so every row still lists its
+// columns ascending, as build_csr does.
+void build_csr_parallel(HostLp &h, int parts) {
+    const size_t M = static_cast<size_t>(h.m), P = static_cast<size_t>(parts);
+    std::vector<int> c0(P + 1);
+    for (size_t t = 0; t <= P; ++t) {
+        const int64_t want = static_cast<int64_t>(h.nnz) * static_cast<int64_t>(t) / parts;
+        c0[t] = static_cast<int>(std::lower_bound(h.cptr.begin(), h.cptr.end(), want) - h.cptr.begin());
+    }
+    c0[P] = h.n;
+    std::vector<int> cnt(P * M, 0);  // entries of part t in row i, then part t's first slot in row i
+    parallel_for_each(parts, [&](int t) {
+        int *ct = cnt.data() + static_cast<size_t>(t) * M;
+        for (int k = h.cptr[static_cast<size_t>(c0[static_cast<size_t>(t)])];
+             k < h.cptr[static_cast<size_t>(c0[static_cast<size_t>(t) + 1])]; ++k)
+            ct[h.cidx[static_cast<size_t>(k)]]++;
+    });
+    h.rptr.assign(M + 1, 0);
+    for (size_t i = 0; i < M; ++i) {
+        int len = 0;
+        for (size_t t = 0; t < P; ++t) len += cnt[t * M + i];
+        h.rptr[i + 1] = h.rptr[i] + len;
+    }
+    parallel_for_each(parts, [&](int t) {  // row ranges: first slots
+        const size_t i0 = M * static_cast<size_t>(t) / P, i1 = M * (static_cast<size_t>(t) + 1) / P;
+        for (size_t i = i0; i < i1; ++i) {
+            int at = h.rptr[i];
+            for (size_t u = 0; u < P; ++u) {
+                const int n = cnt[u * M + i];
+                cnt[u * M + i] = at;
+                at += n;
+            }
+        }
+    });
+    h.ridx.resize(static_cast<size_t>(h.nnz));
+    h.csc2csr.resize(static_cast<size_t>(h.nnz));
+    parallel_for_each(parts, [&](int t) {
+        int *fill = cnt.data() + static_cast<size_t>(t) * M;
+        for (int j = c0[static_cast<size_t>(t)]; j < c0[static_cast<size_t>(t) + 1]; ++j)
+            for (int k = h.cptr[static_cast<size_t>(j)]; k < h.cptr[static_cast<size_t>(j) + 1]; ++k) {
+                const int d = fill[h.cidx[static_cast<size_t>(k)]]++;
+                h.ridx[static_cast<size_t>(d)] = j;
+                h.csc2csr[static_cast<size_t>(k)] = d;
+            }
+    });
+}
+
+}  // namespace
+
+void prepare_structure(const mpdhg_lp &lp, HostLp &h) {
+    narrow_structure(lp, h);
+    build_csr(h);
 }
 
 // Rows [start, i) form one tile while the tile has at most `block` rows and
@@ -119,15 +184,26 @@ void stratify_by_size(std::vector<int> &act, const std::vector<HostLp> &H) {
 void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H) {
     H.resize(static_cast<size_t>(count));
     std::vector<std::exception_ptr> err(static_cast<size_t>(count));
+    std::vector<int> large;  // LPs whose CSR is built on all host threads, one after the other
+    std::mutex mu;
     parallel_for_each(count, [&](int q) {
         try {
-            prepare_structure(lps[q], H[static_cast<size_t>(q)]);
+            HostLp &h = H[static_cast<size_t>(q)];
+            narrow_structure(lps[q], h);
+            if (h.nnz >= kParallelCsrNnz) {
+                std::lock_guard<std::mutex> lock(mu);
+                large.push_back(q);
+            } else {
+                build_csr(h);
+            }
         } catch (...) {
             err[static_cast<size_t>(q)] = std::current_exception();
         }
     });
     for (const std::exception_ptr &e : err)
         if (e) std::rethrow_exception(e);
+    const int parts = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    for (int q : large) build_csr_parallel(H[static_cast<size_t>(q)], parts);
 }
 
 BatchLayout make_layout(const std::vector<HostLp> &H, const std::vector<int> &act) {
@@ -213,31 +289,46 @@ double build_bands(int bands, int rows, int64_t cols, const int *rptr, const int
     for (int b = 0; b <= bands; ++b) cb[static_cast<size_t>(b)] = cols * b / bands;
     out.bd = std::vector<ColumnBands::Band>(static_cast<size_t>(bands));
     for (ColumnBands::Band &bd : out.bd) bd.rptr.assign(R + 1, 0);
-    // pass 1: nonzeros per row and band; pass 2: scatter in storage order
-    for (size_t i = 0; i < R; ++i) {
-        int b = 0;
-        for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
-            while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
-            out.bd[static_cast<size_t>(b)].rptr[i + 1]++;
+    // pass 1: nonzeros per row and band; pass 2: scatter in storage order.
+    // Both passes are independent per row: ranges of rows run on the host
+    // threads.
+    const int nr = static_cast<int>(std::min<size_t>(64, std::max<size_t>(1, R / 16384)));
+    auto rows_of = [&](int t, size_t &i0, size_t &i1) {
+        i0 = R * static_cast<size_t>(t) / static_cast<size_t>(nr);
+        i1 = R * (static_cast<size_t>(t) + 1) / static_cast<size_t>(nr);
+    };
+    parallel_for_each(nr, [&](int t) {
+        size_t i0, i1;
+        rows_of(t, i0, i1);
+        for (size_t i = i0; i < i1; ++i) {
+            int b = 0;
+            for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
+                while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
+                out.bd[static_cast<size_t>(b)].rptr[i + 1]++;
+            }
         }
-    }
+    });
     for (ColumnBands::Band &bd : out.bd) {
         for (size_t i = 0; i < R; ++i) bd.rptr[i + 1] += bd.rptr[i];
         bd.ridx.resize(static_cast<size_t>(bd.rptr[R]));
         bd.rval.resize(static_cast<size_t>(bd.rptr[R]));
     }
-    std::vector<int> fill(static_cast<size_t>(bands));
-    for (size_t i = 0; i < R; ++i) {
-        for (int b = 0; b < bands; ++b) fill[static_cast<size_t>(b)] = out.bd[static_cast<size_t>(b)].rptr[i];
-        int b = 0;
-        for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
-            while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
-            ColumnBands::Band &bd = out.bd[static_cast<size_t>(b)];
-            const int d = fill[static_cast<size_t>(b)]++;
-            bd.ridx[static_cast<size_t>(d)] = ridx[static_cast<size_t>(k)];
-            bd.rval[static_cast<size_t>(d)] = rval[static_cast<size_t>(k)];
+    parallel_for_each(nr, [&](int t) {
+        size_t i0, i1;
+        rows_of(t, i0, i1);
+        std::vector<int> fill(static_cast<size_t>(bands));
+        for (size_t i = i0; i < i1; ++i) {
+            for (int b = 0; b < bands; ++b) fill[static_cast<size_t>(b)] = out.bd[static_cast<size_t>(b)].rptr[i];
+            int b = 0;
+            for (int k = rptr[i]; k < rptr[i + 1]; ++k) {
+                while (ridx[static_cast<size_t>(k)] >= cb[static_cast<size_t>(b) + 1]) ++b;
+                ColumnBands::Band &bd = out.bd[static_cast<size_t>(b)];
+                const int d = fill[static_cast<size_t>(b)]++;
+                bd.ridx[static_cast<size_t>(d)] = ridx[static_cast<size_t>(k)];
+                bd.rval[static_cast<size_t>(d)] = rval[static_cast<size_t>(k)];
+            }
         }
-    }
+    });
     double tune_ms = 0.0;
     Schedule bsc{0, 0};
     for (size_t b = 0; b < out.bd.size(); ++b) {

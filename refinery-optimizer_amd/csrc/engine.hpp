@@ -153,6 +153,9 @@ void stratify_by_size(std::vector<int> &act, const std::vector<HostLp> &H);
 // prepare_structure of every LP of a batch, on the host threads; an invalid
 // LP raises the error the lowest-numbered invalid LP gives.
 void prepare_all(const mpdhg_lp *lps, int count, std::vector<HostLp> &H);
+// LPs of at least this many nonzeros get their CSR built on all host threads
+// (the same CSR), one LP after the other.
+constexpr int kParallelCsrNnz = 1 << 20;
 
 // Host staging array whose elements start uninitialised (no zero fill), for
 // the batch arrays that the per-instance preparation writes in full: their

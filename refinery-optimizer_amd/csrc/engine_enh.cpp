@@ -167,16 +167,27 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
     mark("CSR builds, layout, host arrays");
 
     // ---- per-instance host preparation, part 1 (independent, in parallel):
-    // the batch's structure and its FP64 values in CSC order ----
-    parallel_for(na, [&](int a) {
+    // the batch's structure and its FP64 values in CSC order; an LP of more
+    // than kParallelCsrNnz nonzeros is cut into parts of columns and rows ----
+    std::vector<std::pair<int, int>> parts;  // (instance, part of instance_parts)
+    auto instance_parts = [&](int a) {
+        return 1 + H[static_cast<size_t>(act[static_cast<size_t>(a)])].nnz / kParallelCsrNnz;
+    };
+    for (int a = 0; a < na; ++a)
+        for (int t = 0; t < instance_parts(a); ++t) parts.emplace_back(a, t);
+    parallel_for(static_cast<int>(parts.size()), [&](int w) {
+        const int a = parts[static_cast<size_t>(w)].first, t = parts[static_cast<size_t>(w)].second;
+        const int np = instance_parts(a);
         const int q = act[static_cast<size_t>(a)];
         const HostLp &h = H[static_cast<size_t>(q)];
         const mpdhg_lp &lp = lps[q];
         const int64_t ro = L.row_off[static_cast<size_t>(a)], co = L.col_off[static_cast<size_t>(a)],
                       zo = L.nnz_off[static_cast<size_t>(a)];
-        for (int j = 0; j < h.n; ++j) hh.cptr[static_cast<size_t>(co + j)] = static_cast<int>(zo + h.cptr[j]);
-        for (int i = 0; i < h.m; ++i) hh.rptr[static_cast<size_t>(ro + i)] = static_cast<int>(zo + h.rptr[i]);
-        for (int k = 0; k < h.nnz; ++k) {
+        const int j0 = static_cast<int>(int64_t(h.n) * t / np), j1 = static_cast<int>(int64_t(h.n) * (t + 1) / np);
+        const int i0 = static_cast<int>(int64_t(h.m) * t / np), i1 = static_cast<int>(int64_t(h.m) * (t + 1) / np);
+        for (int j = j0; j < j1; ++j) hh.cptr[static_cast<size_t>(co + j)] = static_cast<int>(zo + h.cptr[j]);
+        for (int i = i0; i < i1; ++i) hh.rptr[static_cast<size_t>(ro + i)] = static_cast<int>(zo + h.rptr[i]);
+        for (int k = h.cptr[static_cast<size_t>(j0)]; k < h.cptr[static_cast<size_t>(j1)]; ++k) {
             const size_t gk = static_cast<size_t>(zo + k);
             hh.cidx[gk] = static_cast<int>(ro + h.cidx[static_cast<size_t>(k)]);
             hh.cval64[gk] = lp.vals[k];
