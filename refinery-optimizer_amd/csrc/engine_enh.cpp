@@ -212,6 +212,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         d_rptr.upload(hh.rptr.data(), hh.rptr.size(), st);
         d_perm.upload(hh.perm.data(), hh.perm.size(), st);
         d_cval64_s.upload(hh.cval64.data(), hh.cval64.size(), st);
+        mark("structure and value uploads");
         launch::scale_batch(st, static_cast<int>(R), static_cast<int>(C), d_cptr.get(), d_cidx_s.get(), d_rptr.get(),
                             d_perm.get(), d_cval64_s.get(), d_rval64_s.get(), d_ridx_s.get(), cfg.ruiz_iterations,
                             d_dr.get(), d_dc.get(), d_rs.get(), d_csum.get(), d_cval_s.get(), d_rval_s.get());
@@ -220,7 +221,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         MPDHG_HIP(hipMemcpyAsync(hh.dc.data(), d_dc.get(), C * sizeof(double), hipMemcpyDeviceToHost, st));
         MPDHG_HIP(hipStreamSynchronize(st));
     }
-    mark("uploads, scaling on the device");
+    mark("scaling on the device");
 
     // ---- per-instance host preparation, part 2: the scaled vectors, norms
     // and the instance state (D_r, D_c from the device) ----
