@@ -34,8 +34,11 @@ Other workloads:
              row and one 100k-nonzero column (the row-split rule).
 
 The roofline object reports the dominant fused SpMV kernel: algorithmic
-bytes per launch / its average launch duration from HIP events recorded on
-the solve's stream inside the timed solve; peak = MI355X HBM3E 8 TB/s.
+bytes per launch (the fused-iteration formula 16 nnz + 32 n + 24 m, split by
+launch; see fused_bytes) / its average launch duration from HIP events
+recorded on the solve's stream inside the timed solve; peak = MI355X HBM3E
+8 TB/s.  `engine_achieved` uses the engine's own per-launch count, which adds
+the enhanced mode's Halpern anchor reads.
 `traffic` is the HBM bytes per launch of that kernel measured with the
 FETCH_SIZE and WRITE_SIZE counters (two rocprofv3 --pmc passes over
 `bench.py --max-iterations 128`, every instance active), scaled by what the
@@ -96,19 +99,45 @@ def enh_kernels(nnz: int):
     return ("k_enh_primal" + sfx, "k_enh_dual" + sfx)
 
 
-def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=TRAFFIC_FILE):
+def fused_bytes(probs, vector_steps):
+    """Algorithmic bytes of one fused primal launch (A^T y and the primal
+    step) and one fused dual launch (A xbar and the dual step) over the LPs
+    `probs`, by the fused-iteration formula 16 nnz + 32 n + 24 m (scalar
+    steps; +4 n + 4 m with per-column / per-row steps) split by launch:
+    primal 8 nnz + 4 (n + 1) col pointers + 4 m gathered y + 6 n vector words
+    (c, l, u, x read; x, xbar written; + tau with vector steps); dual 8 nnz +
+    4 (m + 1) row pointers + 4 n gathered xbar + 4 m vector words (rl, ru, y
+    read; y written; + sigma).  The enhanced mode's Halpern anchor reads (x0,
+    y0) and its extrapolated y are real traffic outside this formula; the
+    engine's own per-launch figure counts them."""
+    nnz = sum(p.num_nonzeros() for p in probs)
+    n = sum(p.num_cols() for p in probs)
+    m = sum(p.num_rows() for p in probs)
+    v = 4 if vector_steps else 0
+    return (8.0 * nnz + 4.0 * (n + len(probs)) + 4.0 * m + (24.0 + v) * n,
+            8.0 * nnz + 4.0 * (m + len(probs)) + 4.0 * n + (16.0 + v) * m)
+
+
+def roofline(sol, kernels=("k_enh_primal", "k_enh_dual"), traffic=TRAFFIC_FILE, algo=None):
+    """The dominant fused kernel's rate.  algo = (primal, dual) algorithmic
+    bytes per launch (fused_bytes) gives `achieved`; the engine's own count
+    per launch (sol.bytes_*_launch) is reported beside it."""
     n_l = max(sol.kernel_launches_csc, 1)
     csc_us = 1e3 * sol.kernel_ms_spmv_csc / n_l
     csr_us = 1e3 * sol.kernel_ms_spmv_csr / n_l
-    gbs_csc = sol.bytes_csc_launch / (csc_us * 1e-6) / 1e9 if csc_us > 0 else 0.0
-    gbs_csr = sol.bytes_csr_launch / (csr_us * 1e-6) / 1e9 if csr_us > 0 else 0.0
+    a_csc, a_csr = algo if algo else (sol.bytes_csc_launch, sol.bytes_csr_launch)
+    gbs_csc = a_csc / (csc_us * 1e-6) / 1e9 if csc_us > 0 else 0.0
+    gbs_csr = a_csr / (csr_us * 1e-6) / 1e9 if csr_us > 0 else 0.0
     if csc_us >= csr_us:
-        name, us, b, g = kernels[0], csc_us, sol.bytes_csc_launch, gbs_csc
+        name, us, b, e = kernels[0], csc_us, a_csc, sol.bytes_csc_launch
     else:
-        name, us, b, g = kernels[1], csr_us, sol.bytes_csr_launch, gbs_csr
+        name, us, b, e = kernels[1], csr_us, a_csr, sol.bytes_csr_launch
+    g = b / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    ge = e / (us * 1e-6) / 1e9 if us > 0 else 0.0
     return ({"bound": "hbm", "kernel": name, "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": g / HBM_PEAK_GBS, "traffic": measured_traffic(name, traffic) if traffic else None,
-             "bytes_per_launch": b, "us_per_launch": us},
+             "bytes_per_launch": b, "us_per_launch": us,
+             "engine_bytes_per_launch": e, "engine_achieved": ge, "engine_frac": ge / HBM_PEAK_GBS},
             {"csc_fused_gbs": gbs_csc, "csr_fused_gbs": gbs_csr, "csc_fused_us": csc_us,
              "csr_fused_us": csr_us,
              # tile schedules (threads per workgroup, nonzeros per LDS chunk) the solve used,
@@ -228,7 +257,7 @@ def bench_sweep(args):
             f"({walls[-1]:.3f}s with host setup), max iters {max(x.simplex_iterations for x in sols)}")
     dev = sum(times) / len(times)
     wall = sum(walls) / len(walls)
-    rf, spmv = roofline(sols[0], enh_kernels(nnz))
+    rf, spmv = roofline(sols[0], enh_kernels(nnz), algo=fused_bytes(probs, False))
     # plain: the same engine, one LP per launch sequence, on every 16th LP of
     # each family (64 LPs, half of each family)
     family = [sp[0] for sp in W.sweep_specs(len(probs))]
@@ -325,7 +354,8 @@ def bench_large(args, mode):
         cfg.profile, cfg.max_iterations = True, min(args.steps, 100)
         kd = "k_ref_dual" if sol.csr_bands == 1 else f"k_ref_dual_band x{sol.csr_bands}"
         rf, spmv = roofline(S.solve_gpu_pdhg(p, cfg), ("k_ref_primal", kd),
-                            traffic=ROOT / "profiles" / "pmc_traffic_large_ref.md")
+                            traffic=ROOT / "profiles" / "pmc_traffic_large_ref.md",
+                            algo=fused_bytes([p], True))
         out = {"metric": "PDHG iterations/s (reference mode); SpMV HBM GB/s vs peak",
                "value": args.steps / sol.time_solver_sec, "unit": "iterations/s",
                "ms_per_step": 1e3 * sol.time_solver_sec / args.steps, "dtype": "fp32",
@@ -351,7 +381,8 @@ def bench_large(args, mode):
         kp, kd = enh_kernels(p.num_nonzeros())
         if sol.csr_bands > 1:  # the dual product's time spans its band launches
             kd = kd.replace("k_enh_dual", "k_enh_dual_band") + f" x{sol.csr_bands}"
-        rf, spmv = roofline(sol, (kp, kd), traffic=ROOT / "profiles" / "pmc_traffic_large_enh.md")
+        rf, spmv = roofline(sol, (kp, kd), traffic=ROOT / "profiles" / "pmc_traffic_large_enh.md",
+                            algo=fused_bytes([p], False))
         out = {"metric": "time to 1e-4 rel KKT (enhanced); SpMV HBM GB/s vs peak",
                "value": sol.time_solver_sec, "unit": "s", "higher_is_better": False,
                "ms_per_step": 1e3 * sol.time_solver_sec / max(sol.simplex_iterations, 1),
@@ -459,7 +490,7 @@ def bench_staircase(args):
         log(f"step {s}: status {int(sol.status)} kkt {sol.rel_kkt:.2e} in {sol.simplex_iterations} "
             f"iterations, {times[-1]:.3f}s")
     ref = highs_objective(T, R, Cb, seed)
-    rf, spmv = roofline(sol, enh_kernels(p.num_nonzeros()), traffic=False)
+    rf, spmv = roofline(sol, enh_kernels(p.num_nonzeros()), traffic=False, algo=fused_bytes([p], False))
     dev = sum(times) / len(times)
     return {"metric": "seconds to 1e-8 rel KKT (staircase LP, FP32 iterates + FP64 refinement)",
             "value": dev, "unit": "s", "higher_is_better": False, "ms_per_step": 1e3 * dev,

@@ -55,3 +55,54 @@ def test_comparison_samples_draw_both_families_alike():
         idx = b.strata(512, step)
         assert len(idx) == 2 * per and len(set(idx)) == len(idx)
         assert sum(fam[q] == "random" for q in idx) == sum(fam[q] == "staircase" for q in idx) == per
+
+
+class _Lp:
+    def __init__(self, m, n, nnz):
+        self.m, self.n, self.nnz = m, n, nnz
+
+    def num_rows(self):
+        return self.m
+
+    def num_cols(self):
+        return self.n
+
+    def num_nonzeros(self):
+        return self.nnz
+
+
+def test_fused_bytes_follow_the_iteration_formula():
+    b = _bench()
+    probs = [_Lp(1000, 2400, 19_000), _Lp(200, 480, 3_100), _Lp(0, 5, 0)]
+    nnz, n, m, k = 22_100, 2885, 1200, 3
+    sp, sd = b.fused_bytes(probs, False)
+    vp, vd = b.fused_bytes(probs, True)
+    # 16 nnz + 32 n + 24 m per iteration (scalar steps), + 4 n + 4 m with
+    # per-column / per-row steps; the +4 per LP is each pointer array's end
+    assert sp + sd == 16 * nnz + 32 * n + 24 * m + 4 * 2 * k
+    assert vp + vd == sp + sd + 4 * n + 4 * m
+    assert (vp - sp, vd - sd) == (4 * n, 4 * m)
+    # the engine's reference-mode count per LP (engine.cpp) is the vector-step split
+    for q in probs:
+        ep = 8.0 * q.nnz + 4.0 * (q.n + 1) + 4.0 * q.m + 4.0 * 7 * q.n
+        ed = 8.0 * q.nnz + 4.0 * (q.m + 1) + 4.0 * q.n + 4.0 * 5 * q.m
+        assert b.fused_bytes([q], True) == (ep, ed)
+
+
+def test_roofline_reports_formula_and_engine_bytes():
+    b = _bench()
+
+    class Sol:
+        kernel_launches_csc = 10
+        kernel_ms_spmv_csc = 1.0   # 100 us per launch
+        kernel_ms_spmv_csr = 0.5   # 50 us
+        bytes_csc_launch = 5.2e8
+        bytes_csr_launch = 4.2e8
+        csr_block, csr_chunk, csc_block, csc_chunk, tune_ms = 512, 8192, 512, 4096, 0.0
+
+    rf, spmv = b.roofline(Sol(), ("kp", "kd"), traffic=False, algo=(5.0e8, 4.0e8))
+    assert rf["kernel"] == "kp" and rf["us_per_launch"] == 100.0
+    assert abs(rf["achieved"] - 5000.0) < 1e-9 and abs(rf["frac"] - 0.625) < 1e-12
+    assert abs(rf["engine_achieved"] - 5200.0) < 1e-9
+    assert rf["traffic"] is None
+    assert abs(spmv["csr_fused_gbs"] - 8000.0) < 1e-9
