@@ -30,6 +30,9 @@ Other workloads:
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
   stream     the device's streaming ceilings: float4 kernels reading 1 GiB
              (cached and non-temporal) and copying it, best of five grids.
+  gather     the fused kernels' memory pattern without their arithmetic: the
+             index/value stream of 40.6M nonzeros in 8,192-nonzero chunks,
+             alone and with sorted gathers 0.25 ... 6 vector entries apart.
   longrow    enhanced-mode iterations on a random LP with one 500k-nonzero
              row and one 100k-nonzero column (the row-split rule).
 
@@ -839,6 +842,37 @@ def bench_stream(args):
                          "frac": r["read_nt_gbs"] / HBM_PEAK_GBS, "traffic": float(1 << 30)}}
 
 
+# Chunk strides (vector entries between neighbouring sorted gathers) probed
+# by the gather workload: the staircase family's chunks sit near 0.25, the
+# random family's from 1 (small LPs) to 6 (m = 20,000).
+GATHER_STRIDES = (0.25, 1.0, 2.0, 4.0, 6.0)
+
+
+def bench_gather(args):
+    """Gather cost in the fused kernels' memory pattern (bench-only vendor
+    library): the 40.6M-nonzero stream of 4,960 chunks of 8,192 alone, then
+    with sorted gathers at each stride, then with the LDS pairs and row walk;
+    what the stream and the gathers each cost the dual kernel."""
+    from sih_solver import vendor
+
+    rows = []
+    for st in GATHER_STRIDES:
+        r = vendor.gather(st, reps=max(args.steps, 1) * 10)
+        rows.append({"stride": st, **r})
+        log(f"stride {st:5.2f}: stream {r['stream_us']:.1f} us ({r['stream_gbs']:.0f} GB/s), + gathers "
+            f"{r['gather_us']:.1f} us, + LDS pairs and walk {r['staged_us']:.1f} us")
+    s0 = rows[0]
+    return {"metric": "index/value stream GB/s of the fused kernels' pattern; gather cost by stride",
+            "value": s0["stream_gbs"], "unit": "GB/s", "ms_per_step": 1e-3 * s0["stream_us"],
+            "dtype": "f32",
+            "config": {"workload": "gather: 4,960 chunks of 8,192 (int32 index, FP32 value) words, one "
+                                   "512-thread workgroup per chunk, XCD-contiguous chunks, sorted gathers "
+                                   f"at strides {list(GATHER_STRIDES)}"},
+            "strides": rows,
+            "roofline": {"bound": "hbm", "achieved": s0["stream_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": s0["stream_gbs"] / HBM_PEAK_GBS, "traffic": None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -846,7 +880,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sweep512",
                     choices=["sweep512", "cfg0", "large-ref", "large-enh", "staircase96", "adjoint64",
-                             "hybrid48", "copy", "stream", "longrow"])
+                             "hybrid48", "copy", "stream", "gather", "longrow"])
     ap.add_argument("--max-iterations", type=int, default=100000,
                     help="sweep512 iteration cap (a small cap makes a short profiling run in "
                          "which every instance stays active; its value is then not a result)")
@@ -885,6 +919,8 @@ def main():
         out = bench_copy(args)
     elif args.workload == "stream":
         out = bench_stream(args)
+    elif args.workload == "gather":
+        out = bench_gather(args)
     elif args.workload == "longrow":
         out = bench_longrow(args)
     elif args.workload == "cfg0":

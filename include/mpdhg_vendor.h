@@ -50,6 +50,24 @@ typedef struct mpdhg_vendor_stream_result {
 } mpdhg_vendor_stream_result;
 int mpdhg_vendor_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out);
 
+/* Gather cost in the fused tile kernels' memory pattern (no counterpart in
+ * the reference): 8 x per_xcd chunks of 8192 (int32 index, FP32 value) words,
+ * one 512-thread workgroup per chunk reading 16 of each per lane
+ * non-temporally, chunk c gathering x[base_c + floor(k stride)] (sorted, as a
+ * gather-order chunk whose entries are `stride` vector entries apart), each
+ * XCD over its own contiguous chunks and its own eighth of x.  Mean µs per
+ * launch over `reps` launches after one untimed: the stream alone, the stream
+ * with the gathers summed in registers, and with the (value, x) pairs staged
+ * in LDS and walked 16 per lane. */
+typedef struct mpdhg_vendor_gather_result {
+    int64_t entries;    /* nonzeros per launch: 8 per_xcd 8192 */
+    double stream_us;   /* index and value stream only */
+    double gather_us;   /* + x[index] gathers */
+    double staged_us;   /* + LDS pairs and the row walk */
+    double stream_gbs;  /* 8 entries / stream_us */
+} mpdhg_vendor_gather_result;
+int mpdhg_vendor_gather(int per_xcd, double stride, int reps, mpdhg_vendor_gather_result *out);
+
 #ifdef __cplusplus
 }
 #endif

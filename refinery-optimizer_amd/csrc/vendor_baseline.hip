@@ -285,9 +285,121 @@ int run_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out) {
     return MPDHG_OK;
 }
 
+// The fused tile kernels' memory pattern without their arithmetic: one
+// workgroup of kGatherBlock lanes per chunk of kGatherChunk (index, value)
+// words, each lane loading kGatherChunk / kGatherBlock of each non-temporally
+// (lane t takes entries t, t + kGatherBlock, ...); block b runs on XCD b % 8
+// and takes chunk (b % 8) per + b / 8, so each XCD walks a contiguous eighth
+// of the chunks.  MODE 0 streams only; MODE 1 also gathers x[index] and sums
+// value * x in registers; MODE 2 parks the (value, x) pairs in LDS and each
+// lane walks 16 consecutive pairs, as the kernels' row walk does.  Every mode
+// holds the 64 KB LDS stage (two workgroups per CU, as the dual kernel).
+constexpr int kGatherBlock = 512, kGatherChunk = 8192, kGatherPt = kGatherChunk / kGatherBlock;
+
+template <int MODE>
+__global__ __launch_bounds__(kGatherBlock) void k_gather_probe(const int *__restrict__ idx,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ x, int per,
+                                                              float *__restrict__ out) {
+    __shared__ __align__(16) float2 lds[kGatherChunk];
+    const int c = static_cast<int>(blockIdx.x & 7u) * per + static_cast<int>(blockIdx.x >> 3);
+    const int *__restrict__ ib = idx + static_cast<size_t>(c) * kGatherChunk;
+    const float *__restrict__ vb = val + static_cast<size_t>(c) * kGatherChunk;
+    const int tid = static_cast<int>(threadIdx.x);
+    int id[kGatherPt];
+    float v[kGatherPt];
+#pragma unroll
+    for (int q = 0; q < kGatherPt; ++q) {
+        id[q] = __builtin_nontemporal_load(ib + tid + q * kGatherBlock);
+        v[q] = __builtin_nontemporal_load(vb + tid + q * kGatherBlock);
+    }
+    float s = 0.0f;
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int q = 0; q < kGatherPt; ++q) lds[tid + q * kGatherBlock] = make_float2(v[q], x[id[q]]);
+        __syncthreads();
+        const float2 *row = lds + tid * kGatherPt;
+        for (int k = 0; k < kGatherPt; ++k) s = __builtin_fmaf(row[k].x, row[k].y, s);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kGatherPt; ++q)
+            s = MODE == 1 ? __builtin_fmaf(v[q], x[id[q]], s) : s + v[q] * static_cast<float>(id[q] & 7);
+        lds[tid * 8] = make_float2(s, s);
+        __syncthreads();
+        s = lds[((tid + 1) % kGatherBlock) * 8].x;
+    }
+    out[blockIdx.x * kGatherBlock + tid] = s;
+}
+
+int run_gather(int per_xcd, double stride, int reps, mpdhg_vendor_gather_result *out) {
+    if (!out || per_xcd < 1 || per_xcd > 4096 || !(stride > 0.0) || stride > 64.0 || reps < 1)
+        throw Fail{MPDHG_E_ARG, "per_xcd in [1, 4096], stride in (0, 64], reps >= 1"};
+    hip_ok(hipSetDevice(0), "hipSetDevice");
+    const int grid = 8 * per_xcd;
+    const size_t n = static_cast<size_t>(grid) * kGatherChunk;
+    // Chunk c of XCD w = c / per_xcd gathers x[base + floor(k stride)], k < kGatherChunk,
+    // with base inside XCD w's own eighth of x: consecutive chunks advance by
+    // half their span, as the sorted chunks of consecutive tiles do.
+    const int64_t span = static_cast<int64_t>(kGatherChunk * stride) + 1;
+    const int64_t eighth = std::max<int64_t>(2 * span, 1 << 20);
+    const int64_t nx = 8 * eighth;
+    std::vector<int> h(n);
+    const int64_t step = static_cast<int64_t>(kGatherChunk * stride * 0.5) + 1;
+    for (int c = 0; c < grid; ++c) {
+        const int w = c / per_xcd, k8 = c % per_xcd;
+        const int64_t base = w * eighth + (k8 * step) % (eighth - span);
+        for (int k = 0; k < kGatherChunk; ++k)
+            h[static_cast<size_t>(c) * kGatherChunk + k] = static_cast<int>(base + static_cast<int64_t>(k * stride));
+    }
+    Dev<int> idx(h, 0);
+    Dev<float> val(n), x(static_cast<size_t>(nx)), o(static_cast<size_t>(grid) * kGatherBlock);
+    hip_ok(hipMemset(val.p, 0, n * sizeof(float)), "hipMemset");
+    hip_ok(hipMemset(x.p, 0, static_cast<size_t>(nx) * sizeof(float)), "hipMemset");
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (auto &v : ev.e) hip_ok(hipEventCreate(&v), "hipEventCreate");
+    double us[3];
+    for (int mode = 0; mode < 3; ++mode) {
+        auto launch = [&] {
+            if (mode == 0) hipLaunchKernelGGL(k_gather_probe<0>, dim3(grid), dim3(kGatherBlock), 0, 0, idx.p, val.p, x.p, per_xcd, o.p);
+            else if (mode == 1) hipLaunchKernelGGL(k_gather_probe<1>, dim3(grid), dim3(kGatherBlock), 0, 0, idx.p, val.p, x.p, per_xcd, o.p);
+            else hipLaunchKernelGGL(k_gather_probe<2>, dim3(grid), dim3(kGatherBlock), 0, 0, idx.p, val.p, x.p, per_xcd, o.p);
+        };
+        launch();
+        hip_ok(hipGetLastError(), "gather probe launch");
+        hip_ok(hipEventRecord(ev.e[0], 0), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) launch();
+        hip_ok(hipEventRecord(ev.e[1], 0), "hipEventRecord");
+        hip_ok(hipEventSynchronize(ev.e[1]), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_ok(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "hipEventElapsedTime");
+        us[mode] = 1e3 * ms / reps;
+    }
+    out->entries = static_cast<int64_t>(n);
+    out->stream_us = us[0];
+    out->gather_us = us[1];
+    out->staged_us = us[2];
+    out->stream_gbs = 8.0 * static_cast<double>(n) / (1e-6 * us[0]) / 1e9;
+    return MPDHG_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpdhg_vendor_gather(int per_xcd, double stride, int reps, mpdhg_vendor_gather_result *out) {
+    try {
+        return run_gather(per_xcd, stride, reps, out);
+    } catch (const Fail &f) {
+        g_err = f.msg;
+        return f.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return MPDHG_E_STATE;
+    }
+}
 
 int mpdhg_vendor_stream(size_t bytes, int reps, mpdhg_vendor_stream_result *out) {
     try {

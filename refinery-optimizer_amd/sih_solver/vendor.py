@@ -33,8 +33,14 @@ class MpdhgVendorStreamResult(C.Structure):
     _fields_ = [("read_gbs", C.c_double), ("read_nt_gbs", C.c_double), ("copy_gbs", C.c_double)]
 
 
+class MpdhgVendorGatherResult(C.Structure):
+    _fields_ = [("entries", C.c_int64), ("stream_us", C.c_double), ("gather_us", C.c_double),
+                ("staged_us", C.c_double), ("stream_gbs", C.c_double)]
+
+
 SIGNATURES = {
     "mpdhg_vendor_stream": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(MpdhgVendorStreamResult)]),
+    "mpdhg_vendor_gather": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(MpdhgVendorGatherResult)]),
     "mpdhg_vendor_pdhg": (C.c_int, [C.POINTER(MpdhgLp), C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(MpdhgVendorResult)]),
     "mpdhg_vendor_last_error": (C.c_char_p, []),
@@ -86,6 +92,20 @@ def run(problem: Problem, iterations: int, mode: int = TRANSPOSE_OP) -> VendorRu
                            f"{lib().mpdhg_vendor_last_error().decode(errors='replace')}")
     del keep
     return VendorRun(r.us_per_iter, r.us_spmv_t, r.us_spmv, r.objective, x)
+
+
+def gather(stride: float, per_xcd: int = 620, reps: int = 20) -> dict:
+    """Gather cost in the fused tile kernels' memory pattern
+    (mpdhg_vendor_gather): mean µs per launch of the index/value stream of
+    8 x per_xcd chunks of 8192 nonzeros alone, with sorted gathers `stride`
+    vector entries apart, and with the LDS pair stage and row walk."""
+    r = MpdhgVendorGatherResult()
+    rc = lib().mpdhg_vendor_gather(per_xcd, stride, reps, C.byref(r))
+    if rc != 0:
+        raise RuntimeError(f"mpdhg_vendor_gather failed (code {rc}): "
+                           f"{lib().mpdhg_vendor_last_error().decode(errors='replace')}")
+    return {"entries": r.entries, "stream_us": r.stream_us, "gather_us": r.gather_us,
+            "staged_us": r.staged_us, "stream_gbs": r.stream_gbs}
 
 
 def stream(bytes_: int = 1 << 30, reps: int = 10) -> dict:

@@ -14,6 +14,7 @@ import ctypes as C
 import math
 import re
 import subprocess
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -62,7 +63,9 @@ def _structs():
             "mpdhg_result": _abi.MpdhgResult, "mpdhg_adjoint_io": _abi.MpdhgAdjointIO,
             "mpdhg_lp_meta": _abi.MpdhgLpMeta, "mpdhg_polish_opts": _abi.MpdhgPolishOpts,
             "mpdhg_polish_result": _abi.MpdhgPolishResult,
-            "mpdhg_vendor_result": vendor.MpdhgVendorResult}
+            "mpdhg_vendor_result": vendor.MpdhgVendorResult,
+            "mpdhg_vendor_stream_result": vendor.MpdhgVendorStreamResult,
+            "mpdhg_vendor_gather_result": vendor.MpdhgVendorGatherResult}
 
 
 def test_ctypes_layouts_match_the_header(tmp_path):
@@ -355,3 +358,22 @@ def test_integration_shim_compiles_against_the_reference_headers(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", str(ref_inc), "-I", str(ROOT / "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_gather_probe_checks_its_arguments():
+    # rejected before any device call, so this runs without a GPU
+    code = f"""
+import sys
+sys.path.insert(0, {str(ROOT / "refinery-optimizer_amd")!r})
+from sih_solver import vendor
+for args in ((0.0, 8), (-1.0, 8), (65.0, 8), (1.0, 0), (1.0, 5000)):
+    try:
+        vendor.gather(args[0], per_xcd=args[1], reps=1)
+    except RuntimeError as e:
+        assert "code -1)" in str(e), str(e)
+    else:
+        raise SystemExit(f"accepted {{args}}")
+print("ok")
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]

@@ -55,3 +55,26 @@ def test_vendor_baseline_tracks_the_oracle(need_gpu, mode, tmp_path):
     obj = float(p.c @ xl)
     assert abs(v["objective"] - obj) <= 1e-4 * max(1.0, abs(obj))
     assert v["us_per_iter"] > 0 and v["us_spmv"] > 0 and v["us_spmv_t"] > 0
+
+
+MEASURE = """
+import json, sys
+sys.path.insert(0, {pkg!r})
+from sih_solver import vendor
+g = vendor.gather(2.0, per_xcd=16, reps=3)
+s = vendor.stream(1 << 24, 2)
+print(json.dumps(dict(gather=g, stream=s)))
+"""
+
+
+def test_measurement_kernels_run(need_gpu):
+    # the streaming and gather ceilings the roofline fractions are read against
+    code = MEASURE.format(pkg=str(ROOT / "refinery-optimizer_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    v = json.loads(r.stdout.strip().splitlines()[-1])
+    g = v["gather"]
+    assert g["entries"] == 8 * 16 * 8192
+    assert g["stream_us"] > 0 and g["gather_us"] > 0 and g["staged_us"] > 0
+    assert abs(g["stream_gbs"] - 8 * g["entries"] / (1e-6 * g["stream_us"]) / 1e9) < 1e-6 * g["stream_gbs"]
+    assert min(v["stream"].values()) > 0
