@@ -130,6 +130,24 @@ def kkt(lp: LpArrays, x, y):
     return dict(zip(keys, out.tolist()))
 
 
+def canon_sum(v, is_max=False):
+    """The enhanced mode's fixed-order FP64 total (orc_canon_sum)."""
+    v = np.ascontiguousarray(v, np.float64)
+    f = lib().orc_canon_sum
+    f.restype = C.c_double
+    f.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int]
+    return f(_p(v, C.c_double), len(v), int(is_max))
+
+
+def kkt_engine(lp: LpArrays, x, y):
+    """The enhanced mode's FP64 check in the engine's order (orc_kkt_engine)."""
+    out = np.zeros(6)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    lib().orc_kkt_engine(*lp.args(), _p(x, C.c_double), _p(y, C.c_double), _p(out, C.c_double))
+    return dict(zip(("eps_p", "eps_d", "eps_gap", "kkt", "pobj", "dobj"), out.tolist()))
+
+
 def identify_basis(lp: LpArrays, x):
     cb = np.zeros(lp.n, np.int8)
     rb = np.zeros(lp.m, np.int8)

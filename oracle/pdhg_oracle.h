@@ -96,6 +96,18 @@ int orc_identify_basis(int m, int n, const int *col_ptr, const int *row_ind, con
                        const double *l, const double *u, const double *rl, const double *ru,
                        const double *x, int8_t *col_basis, int8_t *row_basis);
 
+/* The enhanced mode's fixed-order FP64 total of val[0, n): 16-element group
+ * partials by a lane butterfly, then the 256-lane instance tree (sum, or max
+ * with is_max). */
+double orc_canon_sum(const double *val, int n, int is_max);
+
+/* The enhanced mode's FP64 check of (x, y) on the original LP, in the
+ * engine's order: out[0..5] = eps_p, eps_d, eps_gap, kkt, primal_obj,
+ * dual_obj (|b|_inf over finite row bounds, |c|_inf). */
+int orc_kkt_engine(int m, int n, const int *col_ptr, const int *row_ind, const double *vals,
+                   const double *c, const double *l, const double *u, const double *rl,
+                   const double *ru, const double *x, const double *y, double *out);
+
 /* CPU port of the enhanced mode (Ruiz + Pock-Chambolle scaling, power-iteration
  * step, reflected restarted Halpern PDHG, FP64 checks).  No reference
  * counterpart.  Returns status 0 (converged to tol) or 4. */

@@ -3,6 +3,8 @@
 Tolerances:
   * first check cycle (64 iterations, no restart decision yet): x and y equal
     the CPU port bit for bit when both use the same step size eta;
+  * whole solves (restarts, primal weights, refinement rounds, the stop):
+    every output equals the CPU port bit for bit;
   * a batch equals the same LPs solved one by one, bit for bit;
   * converged solutions: the FP64 relative KKT recomputed by the oracle from
     the returned (x, y) is <= the requested tolerance, and the objective
@@ -84,14 +86,40 @@ def test_staircase_12_blocks_to_1e8_matches_highs_fixture(need_gpu):
     assert abs(s.primal_objective - ref["objective"]) <= 1e-7 * abs(ref["objective"])
 
 
-def test_full_solve_tracks_cpu_port(need_gpu):
-    p = S.gen_staircase(2, 400, 900, 1)
-    sol = S.solve_gpu_pdhg(p, enh(tol=1e-6))
-    ref = port(p, sol, tol=1e-6, max_iterations=100000, check_frequency=64, max_refinements=8,
-               refine_ratio=1e-2)
-    assert ref["status"] == 0 and sol.status == S.SolutionStatus.Optimal
-    assert abs(sol.primal_objective - ref["primal_obj"]) <= 1e-5 * (1 + abs(ref["primal_obj"]))
-    assert 0.5 <= sol.simplex_iterations / ref["iterations"] <= 2.0
+FULL_CASES = {
+    "staircase2_1e-8": (lambda: S.gen_staircase(2, 400, 900, 1), 1e-8),
+    "random300_1e-6": (lambda: S.gen_random(300, 720, 8, 3), 1e-6),
+    "random_lp_1e-7": (lambda: random_lp(120, 200, 0.05, 21)[0], 1e-7),
+    "toy01_1e-8": (toy01, 1e-8),
+    # a positive target no solve reaches (tol <= 0 selects the reference's
+    # tolerances, mpdhg_config.tol_kkt)
+    "iteration_limit": (lambda: S.gen_random(1000, 2500, 8, 0), 1e-300),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FULL_CASES))
+def test_full_solve_bitexact_vs_cpu_port(need_gpu, case):
+    # The whole solve -- power-iteration step, every restart and primal-weight
+    # decision, every refinement round, the stop -- equals the CPU port bit
+    # for bit: the port forms each FP64 decision quantity in the engine's
+    # fixed order (tests/test_oracle.py pins that order).
+    make, tol = FULL_CASES[case]
+    p = make()
+    K = 3000 if tol < 1e-100 else 400000
+    sol = S.solve_gpu_pdhg(p, enh(tol=tol, max_iterations=K))
+    ref = O.pdhg_enh(O.LpArrays.of(p), tol=tol, max_iterations=K, check_frequency=64, max_refinements=8,
+                     refine_ratio=1e-2)
+    assert sol.step_size == ref["step"], case
+    got = (int(sol.status), sol.simplex_iterations, sol.restarts, sol.refinements)
+    assert got == (ref["status"], ref["iterations"], ref["restarts"], ref["refinements"]), case
+    assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sol.row_duals, ref["y"]), case
+    assert (sol.rel_kkt, sol.rel_primal, sol.rel_dual, sol.rel_gap) == (
+        ref["kkt"], ref["eps_p"], ref["eps_d"], ref["eps_gap"]), case
+    assert (sol.primal_objective, sol.dual_bound) == (ref["primal_obj"], ref["dual_obj"]), case
+    if K > 3000:
+        assert sol.status == S.SolutionStatus.Optimal and sol.rel_kkt <= tol, case
+    else:
+        assert sol.status == S.SolutionStatus.IterationLimit and sol.simplex_iterations == K, case
 
 
 def test_nan_cost_reports_numerical_failure(need_gpu):
@@ -312,6 +340,22 @@ def _long_row_lp(seed=81):
     p.row_lower, p.row_upper = b.copy(), b.copy()
     p.col_lower, p.col_upper = np.zeros(n), np.full(n, 10.0)
     return p
+
+
+@pytest.mark.parametrize("tol,K", [(1e-4, 60000), (1e-300, 2000)])
+def test_split_rows_full_solve_bitexact_vs_cpu_port(need_gpu, tol, K):
+    # whole solves with split rows in the FP32 products and the FP64 checks:
+    # to 1e-4 (a refinement round, then the stop) and 2,000 iterations
+    # (restarts, a refinement, the iteration limit)
+    p = _long_row_lp()
+    sol = S.solve_gpu_pdhg(p, enh(tol=tol, max_iterations=K))
+    ref = O.pdhg_enh(O.LpArrays.of(p), tol=tol, max_iterations=K)
+    assert sol.step_size == ref["step"]
+    got = (int(sol.status), sol.simplex_iterations, sol.restarts, sol.refinements)
+    assert got == (ref["status"], ref["iterations"], ref["restarts"], ref["refinements"])
+    assert ref["refinements"] >= 1
+    assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sol.row_duals, ref["y"])
+    assert (sol.rel_kkt, sol.primal_objective, sol.dual_bound) == (ref["kkt"], ref["primal_obj"], ref["dual_obj"])
 
 
 def test_split_rows_first_cycle_bitexact_vs_cpu_port(need_gpu):

@@ -339,3 +339,71 @@ def test_enhanced_long_rows_sum_in_segments_and_blocks():
     # distinct small integers sum exactly: no nonzero dropped or repeated
     u = (np.arange(12800) % 7 + 1).astype(np.float32)
     assert row(u) == np.float32(u.astype(np.int64).sum())
+
+
+# ---- the enhanced mode's fixed-order FP64 reductions (CPU port) ------------
+# Each case is built so that rounding alone fixes the answer: 2^53 + 1 rounds
+# to 2^53 (ties to even), 2^53 + 2 is exact.  The GPU equals the port bit for
+# bit (tests/test_gpu_enhanced.py), so these pin the engine's order as well.
+BIG = 2.0 ** 53
+
+
+def test_canon_group_butterfly_pairs_lanes_eight_apart():
+    # 16-lane butterfly (o = 8, 4, 2, 1, read at lane 0): lanes 1 and 9 meet
+    # first (1 + 1 = 2), and 2 then adds to 2^53 exactly.  A sequential sum
+    # loses both ones; a butterfly starting at o = 1 loses the one at lane 1.
+    v = np.zeros(16)
+    v[0], v[1], v[9] = BIG, 1.0, 1.0
+    assert O.canon_sum(v) == BIG + 2.0
+    assert math.fsum(v) == BIG + 2.0 and float(np.cumsum(v)[-1]) == BIG
+
+
+def test_canon_lanes_take_groups_256_apart():
+    # Group partials 0 and 256 belong to lane 0 and cancel there before the
+    # lanes are combined; group 1 (lane 1) then contributes its 1 exactly.  In
+    # group order the 1 would be lost against 2^53 first.
+    v = np.zeros(257 * 16)
+    v[0], v[16], v[256 * 16] = BIG, 1.0, -BIG
+    assert O.canon_sum(v) == 1.0
+
+
+def test_canon_waves_are_added_in_order():
+    # Lanes 0, 64 and 128 head waves 0, 1 and 2: ((2^53 + 1) + 1) + 0 = 2^53,
+    # where adding waves 1 and 2 first would give 2^53 + 2.
+    v = np.zeros(129 * 16)
+    v[0], v[64 * 16], v[128 * 16] = BIG, 1.0, 1.0
+    assert O.canon_sum(v) == BIG
+
+
+def test_canon_is_exact_on_integers_and_max_is_max():
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 15, 16, 17, 4096, 70001):
+        v = rng.integers(-1000, 1000, n).astype(np.float64)
+        assert O.canon_sum(v) == float(v.sum())
+        a = np.abs(v)
+        assert O.canon_sum(a, is_max=True) == (float(a.max()) if n else 0.0)
+
+
+def test_engine_kkt_closed_forms_toy01():
+    # the engine-order check gives the closed forms of the reference KKT
+    lp = O.LpArrays.of(toy01())
+    k = O.kkt_engine(lp, [1.5, 2.5], [0.5, 0.5])
+    assert k["kkt"] < 1e-15 and k["pobj"] == -6.5 and k["dobj"] == -6.5
+    k0 = O.kkt_engine(lp, [0.0, 0.0], [0.0, 0.0])
+    assert k0["eps_p"] == 0 and k0["eps_d"] == 0 and k0["dobj"] == -30.0
+    assert math.isclose(k0["eps_gap"], 30.0 / 31.0, rel_tol=1e-15)
+    kv = O.kkt_engine(lp, [2.0, 2.5], [0.5, 0.5])
+    assert math.isclose(kv["eps_p"], 0.5 / 10.0, rel_tol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_engine_kkt_agrees_with_the_reference_kkt(seed):
+    # different summation orders, the same quantities
+    p, _ = random_lp(40, 70, 0.2, seed)
+    lp = O.LpArrays.of(p)
+    rng = np.random.default_rng(seed)
+    x = np.clip(rng.normal(size=lp.n), lp.l, lp.u)
+    y = rng.normal(size=lp.m)
+    a, b = O.kkt_engine(lp, x, y), O.kkt(lp, x, y)
+    for key in ("eps_p", "eps_d", "eps_gap", "kkt", "pobj", "dobj"):
+        assert math.isclose(a[key], b[key], rel_tol=1e-12, abs_tol=1e-14), key
