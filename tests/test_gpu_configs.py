@@ -121,4 +121,23 @@ def test_scenario_sweep_512_batched_to_1e6(need_gpu):
         assert (p.num_rows(), p.num_cols(), p.num_nonzeros()) == (c["m"], c["n"], c["nnz"]), q
         assert abs(s.primal_objective - c["objective"]) <= 10 * tol * (1 + abs(c["objective"])), \
             (q, c["kind"], s.primal_objective, c["objective"])
-    print(f"sweep: 512/512 at <= {tol:g}, worst recomputed KKT {worst:.3e}")
+    # A sample of the batched results equals the CPU port bit for bit: per
+    # family, the two smallest LPs with and without a zeroed capacity row and
+    # the LP nearest m = 2,000 (every LP carries its price shocks).
+    sample = []
+    for kind in ("random", "staircase"):
+        for zc in (False, True):
+            cls = [q for q in range(512) if specs[q][0] == kind and specs[q][4] == zc]
+            sample += sorted(cls, key=lambda q: probs[q].num_rows())[:2]
+        fam = [q for q in range(512) if specs[q][0] == kind]
+        sample.append(min(fam, key=lambda q: abs(probs[q].num_rows() - 2000)))
+    for q in sample:
+        ref = O.pdhg_enh(O.LpArrays.of(probs[q]), tol=tol, max_iterations=100000)
+        s = sols[q]
+        label = (q, specs[q][0], specs[q][4], probs[q].num_rows())
+        assert (int(s.status), s.simplex_iterations, s.restarts, s.refinements) == (
+            ref["status"], ref["iterations"], ref["restarts"], ref["refinements"]), label
+        assert np.array_equal(s.x, ref["x"]) and np.array_equal(s.row_duals, ref["y"]), label
+        assert s.rel_kkt == ref["kkt"], label
+    print(f"sweep: 512/512 at <= {tol:g}, worst recomputed KKT {worst:.3e}; "
+          f"{len(sample)} LPs equal to the CPU port bit for bit")
