@@ -35,7 +35,8 @@ class EnhCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("check_frequency", C.c_int),
                 ("tol", C.c_double), ("reflection", C.c_double), ("ruiz_iters", C.c_int),
                 ("threads", C.c_int), ("step", C.c_double), ("max_refinements", C.c_int),
-                ("refine_ratio", C.c_double)]
+                ("refine_ratio", C.c_double), ("x_start", C.POINTER(C.c_double)),
+                ("y_start", C.POINTER(C.c_double)), ("omega_start", C.c_double)]
 
 
 class EnhOut(C.Structure):
@@ -217,10 +218,14 @@ def spmv_enh_f32(rows, ptr, idx, v, x):
 
 def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
              reflection=1.0, ruiz_iters=10, threads=1, step=0.0, max_refinements=8,
-             refine_ratio=1e-2):
-    """CPU port of the enhanced mode; returns a dict with the best point."""
+             refine_ratio=1e-2, x0=None, y0=None, omega0=0.0):
+    """CPU port of the enhanced mode; returns a dict with the best point.
+    x0 / y0 / omega0 / step: a warm start (y0 in the reference dual sign)."""
+    xs = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+    ys = None if y0 is None else np.ascontiguousarray(y0, np.float64)
     cfg = EnhCfg(max_iterations, check_frequency, tol, reflection, ruiz_iters, threads, step,
-                 max_refinements, refine_ratio)
+                 max_refinements, refine_ratio, None if xs is None else _p(xs, C.c_double),
+                 None if ys is None else _p(ys, C.c_double), omega0)
     out = EnhOut()
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)

@@ -397,7 +397,7 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     for (int i = 0; i < m; ++i) { double b = finite_or_zero(rus[i], rls[i]); bn += b * b; }
     cn = sqrt(cn);
     bn = sqrt(bn);
-    double omega = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    double omega = cfg->omega_start > 0.0 ? cfg->omega_start : ((cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0);
 
     float *x = (float *)xmalloc(sizeof(float) * (size_t)n), *x0 = (float *)xmalloc(sizeof(float) * (size_t)n);
     float *xp = (float *)xmalloc(sizeof(float) * (size_t)n), *xb = (float *)xmalloc(sizeof(float) * (size_t)n);
@@ -408,12 +408,15 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     double *ax = (double *)xmalloc(sizeof(double) * (size_t)m), *lam = (double *)xmalloc(sizeof(double) * (size_t)n);
     double *xbest = (double *)xmalloc(sizeof(double) * (size_t)n), *ybest = (double *)calloc((size_t)m, sizeof(double));
     double *axbest = (double *)xmalloc(sizeof(double) * (size_t)m), *lambest = (double *)xmalloc(sizeof(double) * (size_t)n);
+    /* start point: the (warm) start in the scaled space, x clamped to the box */
     for (int j = 0; j < n; ++j) {
-        float v = 0.0f;
+        float v = cfg->x_start ? (float)(cfg->x_start[j] / P.dc[j]) : 0.0f;
         if (v < ls[j]) v = ls[j];
         if (v > us[j]) v = us[j];
         x[j] = x0[j] = xp[j] = v;
     }
+    if (cfg->y_start)
+        for (int i = 0; i < m; ++i) y[i] = y0[i] = yp[i] = (float)(cfg->y_start[i] / P.dr[i]);
     const float rho = (float)cfg->reflection, rho1 = 1.0f + rho;
     const int F = cfg->check_frequency;
     int last_restart = 0, restarts = 0, restart_next = 0, rounds = 0;

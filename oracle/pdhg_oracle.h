@@ -121,6 +121,9 @@ typedef struct orc_enh_cfg {
     double step;           /* eta; <= 0: 0.998 / sigma_max by power iteration */
     int max_refinements;   /* outer FP64 refinement rounds */
     double refine_ratio;   /* at a restart, refine once KKT <= ratio * KKT at the last re-centring */
+    const double *x_start; /* [n] warm start (engine: mpdhg_warm_start.x), or NULL: clamp(0, l, u) */
+    const double *y_start; /* [m] warm start in the reference dual sign, or NULL: 0 */
+    double omega_start;    /* primal weight to start from; <= 0: |c| / |b| of the scaled LP */
 } orc_enh_cfg;
 typedef struct orc_enh_out {
     int status, iterations, restarts, refinements;

@@ -6,7 +6,9 @@
   step size) stops at the first check with KKT <= 1e-5, with the duals given
   in either dual sign; started from negated duals it does not.
 * Reference mode from a given start point equals the oracle's restatement
-  started there, bit for bit, in both dual signs.
+  started there, bit for bit, in both dual signs; enhanced mode from a given
+  start (a random point, or a solution with its primal weight and step)
+  equals the CPU port started there, bit for bit.
 * PdhgReoptimizer (reoptimization.py:80-158 with the GPU PDHG): every
   perturbation of a 4-block staircase LP solves cold and warm to the same
   objective (10 x tol relative), and the warm solves need fewer iterations
@@ -75,6 +77,31 @@ def test_reference_warm_start_bitexact_with_the_oracle(need_gpu, dual_sign):
     y = sol.row_duals if dual_sign == 0 else -sol.row_duals
     assert np.array_equal(y, r["y"])
     assert sol.ref_primal == r["last_primal"] and sol.ref_dual == r["last_dual"]
+
+
+@pytest.mark.parametrize("dual_sign", [0, 1])
+def test_enhanced_warm_start_bitexact_with_the_cpu_port(need_gpu, dual_sign):
+    # Enhanced mode from a given start -- a random point (outside the bounds
+    # too; the power iteration and the |c| / |b| weight as cold) and a 1e-4
+    # solution with its primal weight and step -- equals the CPU port started
+    # at the same point, bit for bit, to 1e-7.
+    p = S.gen_staircase(3, 200, 450, 6)
+    lp = O.LpArrays.of(p)
+    rng = np.random.default_rng(17)
+    sg = -1.0 if dual_sign else 1.0
+    cfg = S.GpuPdhgConfig.enhanced(tol=1e-7, dual_sign=dual_sign)
+    first = S.solve_gpu_pdhg(p, S.GpuPdhgConfig.enhanced(tol=1e-4, dual_sign=dual_sign))
+    assert first.status == S.SolutionStatus.Optimal
+    starts = [(rng.uniform(-100, 1100, p.num_cols()), rng.normal(size=p.num_rows()), 0.0, 0.0),
+              (first.x, sg * first.row_duals, first.primal_weight, first.step_size)]
+    for xs, ys, om, eta in starts:
+        sol = S.solve_batch([p], cfg, [S.WarmStart(xs, sg * ys, om, eta)])[0]
+        ref = O.pdhg_enh(lp, tol=1e-7, max_iterations=100000, x0=xs, y0=ys, omega0=om, step=eta)
+        got = (int(sol.status), sol.simplex_iterations, sol.restarts, sol.refinements)
+        assert got == (ref["status"], ref["iterations"], ref["restarts"], ref["refinements"]), (om, got)
+        assert sol.status == S.SolutionStatus.Optimal and sol.step_size == ref["step"]
+        assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sg * sol.row_duals, ref["y"])
+        assert sol.rel_kkt == ref["kkt"]
 
 
 def test_scenario_reoptimization_warm_beats_cold(need_gpu):
