@@ -540,7 +540,8 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
         MPDHG_HIP(hipEventCreateWithFlags(&poll_ev[1], hipEventDisableTiming));
     }
-    int it = 0, cycle = 0, covered = na;
+    int it = 0, cycle = 0, covered = na;  // instances (and their nonzeros) the tile lists cover
+    int64_t covered_nnz = L.nnz;
     bool finished = false;
     std::vector<DBuf<Tile>> keep_rt, keep_ct;  // compacted tile lists, alive until the end
     std::vector<std::vector<Tile>> host_keep_rt, host_keep_ct;  // sources of their async uploads
@@ -648,10 +649,14 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                 const int32_t *sv = slots + prev * na;
                 const int running = static_cast<int>(std::count(sv, sv + na, kRunning));
                 finished = (running == 0);
-                if (!finished && 10 * running <= 9 * covered) {
-                    // Compact: later launches cover only the running instances'
-                    // tiles.  Stream order puts the range update after the cycle
-                    // already in flight, which still uses the old lists.
+                int64_t running_nnz = 0;
+                for (int a = 0; a < na; ++a)
+                    if (sv[a] == kRunning) running_nnz += H[static_cast<size_t>(act[static_cast<size_t>(a)])].nnz;
+                if (!finished && 20 * running_nnz <= 19 * covered_nnz) {
+                    // Compact once the running instances hold 95 % of the covered
+                    // nonzeros or less: later launches cover only their tiles.
+                    // Stream order puts the range update after the cycle already
+                    // in flight, which still uses the old lists.
                     // the running LPs dealt into the XCD bins again, by size
                     std::vector<int> run;
                     for (int a = 0; a < na; ++a)
@@ -677,6 +682,7 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
                     rts.stream_nt = cts.stream_nt =
                         stream_bytes([&](int a) { return sv[a] == kRunning; }) > kNtStreamBytes;
                     covered = running;
+                    covered_nnz = running_nnz;
                     ++lists;
                     host_keep_rt.push_back(std::move(nrt));
                     host_keep_ct.push_back(std::move(nct));
