@@ -23,7 +23,13 @@
  *                            the reference CSC upload defect
  *   orc_kkt                 hand-built optimal / perturbed points, HiGHS duals
  *   orc_identify_basis      hand-computed rankings on tiny inputs
- *   orc_pdhg_enh            FP64 KKT of its output + HiGHS objectives
+ *   orc_pdhg_enh            FP64 KKT of its output + HiGHS objectives; its
+ *                            row sums by the split-rule KATs (long rows)
+ *   orc_canon_sum           sums whose results rounding alone fixes (2^53
+ *                            plus ones the lane butterfly pairs first, +-2^53
+ *                            in one lane, waves added in order), exact integer
+ *                            sums (tests/test_oracle.py)
+ *   orc_kkt_engine          toy01 closed forms, orc_kkt to 1e-12
  */
 #ifndef PDHG_ORACLE_H
 #define PDHG_ORACLE_H
