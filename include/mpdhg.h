@@ -161,7 +161,8 @@ typedef struct mpdhg_result {
     int64_t kernel_launches_csr, kernel_launches_csc;
     double bytes_csr_launch, bytes_csc_launch;     /* algorithmic bytes per fused launch */
     double step_size;        /* enhanced: eta (tau = eta/omega, sigma = eta*omega) */
-    double primal_weight;    /* enhanced: final omega */
+    double primal_weight;    /* enhanced: final effective omega (the weight with the refinement
+                              * rescaling undone; a warm start's primal_weight) */
     int32_t csr_block, csr_chunk;  /* tile schedule the solve used for A x (CSR) ... */
     int32_t csc_block, csc_chunk;  /* ... and for A^T y (CSC) */
     double tune_ms;                /* host + device time of the schedule search (0: rule) */

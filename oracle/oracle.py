@@ -43,7 +43,7 @@ class EnhOut(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("restarts", C.c_int),
                 ("refinements", C.c_int), ("eps_p", C.c_double), ("eps_d", C.c_double), ("eps_gap", C.c_double),
                 ("kkt", C.c_double), ("primal_obj", C.c_double), ("dual_obj", C.c_double),
-                ("step", C.c_double)]
+                ("step", C.c_double), ("omega", C.c_double)]
 
 
 _lib = None
@@ -234,4 +234,4 @@ def pdhg_enh(lp: LpArrays, tol=1e-6, max_iterations=100000, check_frequency=64,
     assert rc == 0
     return dict(status=out.status, iterations=out.iterations, restarts=out.restarts,
                 refinements=out.refinements, eps_p=out.eps_p, eps_d=out.eps_d, eps_gap=out.eps_gap, kkt=out.kkt,
-                primal_obj=out.primal_obj, dual_obj=out.dual_obj, step=out.step, x=x, y=y)
+                primal_obj=out.primal_obj, dual_obj=out.dual_obj, step=out.step, omega=out.omega, x=x, y=y)

@@ -25,6 +25,7 @@
  * engine's 256-lane tree (grp_part, canon).  So the whole solve, restarts,
  * refinements and stop included, is the GPU's bit for bit.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -423,7 +424,12 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     double pw_int = 0.0; /* primal-weight controller: sum of log-balance errors */
     double r_anchor = -1.0, r_prev = -1.0;
     double best[10] = {0}, kk[10] = {0};
-    double best_kkt = INFINITY, round_start_kkt = INFINITY;
+    /* Refinement baseline: the first check's KKT (INFINITY: not yet taken).  A
+     * warm start that carries its primal weight is a previous solve's point,
+     * already at its FP32 iterates' precision: DBL_MAX keeps the first check
+     * from setting the baseline, so the first restart re-centres the point in
+     * FP64 (engine_enh.cpp sets the same per instance). */
+    double best_kkt = INFINITY, round_start_kkt = cfg->omega_start > 0.0 ? DBL_MAX : INFINITY;
     int it, status = 4, have_best = 0;
     for (it = 0; it < cfg->max_iterations; ++it) {
         if (restart_next) { /* take z+ as iterate and anchor */
@@ -562,6 +568,7 @@ int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const dou
     out->eps_p = best[0]; out->eps_d = best[1]; out->eps_gap = best[2]; out->kkt = best[3];
     out->primal_obj = best[4]; out->dual_obj = best[5];
     out->step = eta;
+    out->omega = omega * alpha_p / alpha_d; /* effective weight (alpha = 1 frame) */
     if (x_out) memcpy(x_out, xbest, sizeof(double) * (size_t)n);
     if (y_out) memcpy(y_out, ybest, sizeof(double) * (size_t)m);
     free(kw); free(rq);

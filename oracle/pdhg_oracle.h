@@ -134,6 +134,8 @@ typedef struct orc_enh_cfg {
 typedef struct orc_enh_out {
     int status, iterations, restarts, refinements;
     double eps_p, eps_d, eps_gap, kkt, primal_obj, dual_obj, step;
+    double omega;      /* the effective primal weight at the end, omega alpha_p / alpha_d
+                        * (the frame a new solve starts in: a warm start's omega_start) */
 } orc_enh_out;
 int orc_pdhg_enh(int m, int n, const int *col_ptr, const int *row_ind, const double *vals,
                  const double *c, const double *l, const double *u, const double *rl,

@@ -282,7 +282,13 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         s.cnorm = cinf;
         s.tol = tol;
         s.alpha_p = s.alpha_d = s.inv_ap = s.inv_ad = 1.0;
-        s.round_start_kkt = s.best_kkt = HUGE_VAL;
+        s.best_kkt = HUGE_VAL;
+        // Refinement baseline: HUGE_VAL lets the first check set it.  A warm start
+        // with its primal weight is a previous solve's point, already at the FP32
+        // iterates' precision; the largest finite double keeps the first check from
+        // setting the baseline, so the first restart re-centres it in FP64 and
+        // the iterates carry only the correction.
+        s.round_start_kkt = (ws && ws->primal_weight > 0.0) ? std::numeric_limits<double>::max() : HUGE_VAL;
         s.pow_norm2 = pn;
         pow0[static_cast<size_t>(a)] = pn;
     });
@@ -744,7 +750,9 @@ void solve_enhanced(Context &ctx, const mpdhg_lp *lps, int count, const mpdhg_co
         r.rel_gap = s.best_eps_gap;
         r.rel_kkt = s.best_kkt;
         r.step_size = s.eta;
-        r.primal_weight = s.omega;
+        // the effective weight: a refinement rescales omega by its alpha ratio,
+        // which omega alpha_p / alpha_d undoes (a new solve starts at alpha = 1)
+        r.primal_weight = s.omega * s.alpha_p / s.alpha_d;
         r.csr_block = bands > 1 ? BD.bd[0].dir.sc.block : RD.sc.block;  // the schedule of A xbar
         r.csr_chunk = bands > 1 ? BD.bd[0].dir.sc.tn : RD.sc.tn;
         r.csr_bands = bands;

@@ -547,7 +547,10 @@ __global__ __launch_bounds__(kInstBlock) void k_enh_finalize(EnhInst *__restrict
         A.status_out[q] = 4;
         return;
     }
-    if (s.round_start_kkt == HUGE_VAL) s.round_start_kkt = kkt;  // first check of the solve
+    // the refinement baseline is the first check's KKT; a warm start with its
+    // primal weight holds the largest finite double instead, so that its first
+    // restart refines
+    if (s.round_start_kkt == HUGE_VAL) s.round_start_kkt = kkt;
     const double r = sqrt(s.omega * dx2 + dy2 / s.omega);
     const int kc = done - s.last_restart;
     if (s.r_anchor < 0.0) {

@@ -133,3 +133,26 @@ def test_vertex_check_states_each_results_error(need_gpu):
     assert big.vertex_status == S.SolutionStatus.Unknown and math.isnan(big.vertex_rel_error)
     plain = S.solve_batch(probs[:1], S.GpuPdhgConfig.enhanced(tol=tol))[0]
     assert np.array_equal(plain.x, sols[0].x) and math.isnan(plain.vertex_objective)
+
+
+@pytest.mark.parametrize("dual_sign", [0, 1])
+def test_enhanced_warm_start_to_a_tighter_tolerance_bitexact(need_gpu, dual_sign):
+    # From its own 1e-6 solution (x, y, effective primal weight, step) to
+    # 1e-7: the first restart re-centres the start in FP64, so the solve
+    # refines and converges; the engine equals the CPU port started at the
+    # same point bit for bit, its reported primal weight included.
+    p = S.gen_staircase(4, seed=3)
+    lp = O.LpArrays.of(p)
+    sg = -1.0 if dual_sign else 1.0
+    first = S.solve_gpu_pdhg(p, S.GpuPdhgConfig.enhanced(tol=1e-6, dual_sign=dual_sign))
+    ref1 = O.pdhg_enh(lp, tol=1e-6)
+    assert first.status == S.SolutionStatus.Optimal and first.primal_weight == ref1["omega"]
+    ws = S.WarmStart.of(first)
+    sol = S.solve_batch([p], S.GpuPdhgConfig.enhanced(tol=1e-7, dual_sign=dual_sign), [ws])[0]
+    ref = O.pdhg_enh(lp, tol=1e-7, x0=first.x, y0=sg * first.row_duals, omega0=first.primal_weight,
+                     step=first.step_size)
+    got = (int(sol.status), sol.simplex_iterations, sol.restarts, sol.refinements)
+    assert got == (ref["status"], ref["iterations"], ref["restarts"], ref["refinements"]), got
+    assert sol.status == S.SolutionStatus.Optimal and sol.refinements >= 1
+    assert np.array_equal(sol.x, ref["x"]) and np.array_equal(sg * sol.row_duals, ref["y"])
+    assert sol.rel_kkt == ref["kkt"] and sol.primal_weight == ref["omega"]

@@ -407,3 +407,21 @@ def test_engine_kkt_agrees_with_the_reference_kkt(seed):
     a, b = O.kkt_engine(lp, x, y), O.kkt(lp, x, y)
     for key in ("eps_p", "eps_d", "eps_gap", "kkt", "pobj", "dobj"):
         assert math.isclose(a[key], b[key], rel_tol=1e-12, abs_tol=1e-14), key
+
+
+def test_port_warm_start_with_its_weight_refines_at_the_first_restart():
+    # A start that carries its primal weight is a previous solve's point, at
+    # its FP32 iterates' precision: the first restart re-centres it in FP64
+    # (without that the refinement baseline is the first check's KKT, about
+    # the start's own, and the solve stalls at the FP32 floor with no
+    # refinement).  A start at the cold point without a weight is the cold
+    # solve bit for bit; the reported weight is the effective one.
+    lp = O.LpArrays.of(S.gen_staircase(4, seed=3))
+    r = O.pdhg_enh(lp, tol=1e-6)
+    assert r["status"] == 0 and r["omega"] > 0.0
+    w = O.pdhg_enh(lp, tol=1e-7, max_iterations=60000, x0=r["x"], y0=r["y"], omega0=r["omega"], step=r["step"])
+    assert w["status"] == 0 and w["refinements"] >= 1, (w["iterations"], w["kkt"])
+    assert O.kkt(lp, w["x"], w["y"])["kkt"] <= 1e-7 * 1.0001
+    z = O.pdhg_enh(lp, tol=1e-6, x0=np.clip(0.0, lp.l, lp.u), y0=np.zeros(lp.m))
+    assert (z["iterations"], z["refinements"]) == (r["iterations"], r["refinements"])
+    assert np.array_equal(z["x"], r["x"]) and np.array_equal(z["y"], r["y"]) and z["omega"] == r["omega"]
