@@ -5,6 +5,9 @@
 * Enhanced mode warm-started from its own 1e-6 solution (x, y, primal weight,
   step size) stops at the first check with KKT <= 1e-5, with the duals given
   in either dual sign; started from negated duals it does not.
+* Enhanced mode from its 1e-6 solution re-solved to 1e-7: the first restart
+  refines (the start brings its weight), the solve converges, and it equals
+  the CPU port bit for bit, the reported effective weight included.
 * Reference mode from a given start point equals the oracle's restatement
   started there, bit for bit, in both dual signs; enhanced mode from a given
   start (a random point, or a solution with its primal weight and step)
